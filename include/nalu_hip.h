@@ -1,0 +1,198 @@
+/*
+ * nalu_hip.h -- C ABI of the MI355X-native Hex8 CVFEM assembly + linear-solve
+ * path.  This is the drop-in boundary that replaces Nalu's Tpetra-backed
+ * LinearSystem / LinearSolver and the consolidated element-assembly loop.
+ *
+ * Conventions
+ *   - Every function returns an int status unless stated: 0 ok, >0 "not
+ *     converged" (solve only), <0 error.  nalu_last_error() returns the
+ *     message of the last error on the calling thread.  (The reference throws
+ *     on logic errors, src/TpetraLinearSystem.C:1626-1629; the C++ wrapper in
+ *     nalu_hip.hpp turns negative codes back into exceptions.)
+ *   - Pointers named *_dev are device (HBM) pointers; *_host are host
+ *     pointers, borrowed only for the duration of the call.
+ *   - Node arrays are node-major: a scalar field is [N], a vector field is
+ *     [N][3] (the STK field layout, src/ScratchViews.C:408-483).  Element
+ *     fields are element-major: mdot is [E][12] (mass_flow_rate_scs).
+ *   - Connectivity is [E][8] int32 node indices in Hex8 order
+ *     (src/master_element/Hex8CVFEM.C nodeLoc_ :420-436).
+ *   - One handle is not re-entrant; all work of a handle runs in order on the
+ *     handle's HIP stream (default: the null stream).
+ */
+#ifndef NALU_HIP_H
+#define NALU_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NALU_OK 0
+#define NALU_NOT_CONVERGED 1
+#define NALU_ERR_ARG (-1)
+#define NALU_ERR_STATE (-2)
+#define NALU_ERR_HIP (-3)
+#define NALU_ERR_UNSUPPORTED (-4)
+#define NALU_ERR_NUMERIC (-5)
+
+/* Solver methods: LinearSolverConfig "method" (src/LinearSolverConfig.C:34-53). */
+#define NALU_METHOD_GMRES 0
+#define NALU_METHOD_CG 1
+#define NALU_METHOD_FGMRES 2
+/* Preconditioners: LinearSolverConfig "preconditioner" (src/LinearSolverConfig.C:71-85).
+ * SGS is the multicolor ("mt_sgs") symmetric Gauss-Seidel, 1 sweep. */
+#define NALU_PREC_NONE 0
+#define NALU_PREC_JACOBI 1
+#define NALU_PREC_SGS 2
+#define NALU_PREC_LEARNED 3 /* learned message-passing net, SGS fallback; no reference counterpart */
+/* Arithmetic of the Krylov inner loop. */
+#define NALU_PRECISION_FP64 0
+#define NALU_PRECISION_MIXED 1 /* FP32 inner Krylov/SpMV/SGS inside FP64 iterative refinement */
+
+typedef struct nalu_mesh nalu_mesh;
+typedef struct nalu_linsys nalu_linsys;
+
+typedef struct nalu_solver_config {
+  int method;             /* NALU_METHOD_* */
+  int precond;            /* NALU_PREC_* */
+  double tolerance;       /* "tolerance" */
+  double final_tolerance; /* "final_tolerance" (used when final_outer_iter != 0) */
+  int max_iterations;     /* "max_iterations" */
+  int kspace;             /* "kspace" (GMRES restart length) */
+  int precision;          /* NALU_PRECISION_* */
+  int final_outer_iter;   /* Realm::isFinalOuterIter_ */
+} nalu_solver_config;
+
+/* ---------------------------------------------------------------- errors */
+const char *nalu_last_error(void);
+/* 1 when a HIP device is present and usable. */
+int nalu_device_available(void);
+/* Set the HIP stream (hipStream_t) used by a linear system; NULL = null stream. */
+int nalu_linsys_set_stream(nalu_linsys *ls, void *stream);
+
+/* ------------------------------------------------------------------ mesh */
+/* A device-resident Hex8 block: coordinates [N][3], connectivity [E][8] and
+ * optional global ids [N] (NULL: id = index + 1).  Replaces the STK bulk data
+ * the reference's algorithms iterate (src/AssembleElemSolverAlgorithm.C:75-100).
+ * Arrays are copied from the host. */
+int nalu_mesh_create(int64_t num_nodes, int64_t num_elems, const int32_t *conn_host,
+                     const double *coords_host, const int64_t *node_ids_host,
+                     nalu_mesh **out);
+int nalu_mesh_destroy(nalu_mesh *m);
+/* Device pointers owned by the mesh (coords [N][3], conn [E][8]). */
+int nalu_mesh_device_ptrs(const nalu_mesh *m, const double **coords_dev, const int32_t **conn_dev);
+
+/* Synthetic structured Hex8 lattice on [0,1]^3 with STK generated-mesh
+ * numbering: node id = 1 + i + (n+1)(j + (n+1)k); element nodes in Hex8 order
+ * (unit_tests/UnitTestTpetra.C:97-98).  Interior nodes are jittered uniformly
+ * by +-jitter*h with the reference's machine-independent LCG
+ * (unit_tests/UnitTestUtils.C:41-53), three draws per node in node-id order.
+ * stretch > 1 grades the spacing geometrically from all six faces toward the
+ * centre (ratio = stretch per interval), the growth stopping once the spacing
+ * reaches max_aspect times the wall spacing (max_aspect <= 0: no cap).
+ * coords_host [(n+1)^3][3], conn_host [n^3][8]. */
+int nalu_lattice_generate(int n, double jitter, double stretch, double max_aspect,
+                          uint32_t seed, double *coords_host, int32_t *conn_host);
+/* n lognormal(0, sigma) samples, Box-Muller on the same LCG. */
+int nalu_lognormal_field(int64_t n, double sigma, uint32_t seed, double *out_host);
+/* n N(0, sigma^2) samples (Box-Muller on the LCG). */
+int nalu_normal_field(int64_t n, double sigma, uint32_t seed, double *out_host);
+
+/* --------------------------------------------------------- linear system */
+/* LinearSystem::create (src/LinearSystem.C:85-97) with PetraType PT_HIP. */
+int nalu_linsys_create(int num_dof, const nalu_solver_config *cfg, nalu_linsys **out);
+/* ~TpetraLinearSystem (src/TpetraLinearSystem.C:97-102). */
+int nalu_linsys_destroy(nalu_linsys *ls);
+/* buildElemToNodeGraph (src/TpetraLinearSystem.C:430-435). */
+int nalu_linsys_build_elem_to_node_graph(nalu_linsys *ls, const nalu_mesh *m);
+/* finalizeLinearSystem (src/TpetraLinearSystem.C:1256-1353): CSR built on the
+ * GPU, rows ordered by global id, columns sorted, numDof rows interleaved. */
+int nalu_linsys_finalize(nalu_linsys *ls);
+/* zeroSystem (src/TpetraLinearSystem.C:1355-1372): values, rhs and solution. */
+int nalu_linsys_zero(nalu_linsys *ls);
+/* sumInto, Kokkos overload (src/TpetraLinearSystem.C:1457-1522), batched over
+ * num_elems elements: lhs_dev [E][R][R] row-major, rhs_dev [E][R], R = 8*numDof,
+ * conn_dev [E][8].  Atomic scatter. */
+int nalu_linsys_sum_into_batch(nalu_linsys *ls, int64_t num_elems, const int32_t *conn_dev,
+                               const double *lhs_dev, const double *rhs_dev);
+/* sumInto, std::vector overload (src/TpetraLinearSystem.C:1524-1577), one
+ * element from host arrays: entities_host [n] node indices, lhs_host [R][R]. */
+int nalu_linsys_sum_into(nalu_linsys *ls, int num_entities, const int32_t *entities_host,
+                         const double *rhs_host, const double *lhs_host);
+/* applyDirichletBCs (src/TpetraLinearSystem.C:1579-1653): rows of the listed
+ * nodes become e_i, rhs = bc - current for dofs [begin_pos, end_pos). */
+int nalu_linsys_apply_dirichlet(nalu_linsys *ls, int64_t num_bc_nodes,
+                                const int32_t *bc_nodes_dev, const double *solution_dev,
+                                const double *bc_values_dev, int begin_pos, int end_pos);
+/* resetRows (include/LinearSystem.h:116-120): rows of the listed nodes
+ * become diag_value*e_i with rhs_residual in the rhs, for dofs [begin, end). */
+int nalu_linsys_reset_rows(nalu_linsys *ls, int64_t num_nodes, const int32_t *nodes_dev,
+                           int begin_pos, int end_pos, double diag_value, double rhs_residual);
+/* loadComplete (src/TpetraLinearSystem.C:1757-1778): serial, no export. */
+int nalu_linsys_load_complete(nalu_linsys *ls);
+/* solve (src/TpetraLinearSystem.C:1780-1847): preconditioner compute + Krylov
+ * solve from x0 = 0; writes the delta-solution [rows] to delta_out_dev.
+ * Returns 0 converged, NALU_NOT_CONVERGED when max_iterations was reached. */
+int nalu_linsys_solve(nalu_linsys *ls, double *delta_out_dev);
+int nalu_linsys_iterations(const nalu_linsys *ls);
+double nalu_linsys_linear_residual(const nalu_linsys *ls);     /* ||b - A x||_2 */
+double nalu_linsys_nonlinear_residual(const nalu_linsys *ls);  /* ||rhs||_2 (l2Scaling = 1) */
+double nalu_linsys_scaled_nonlinear_residual(const nalu_linsys *ls);
+/* Set Realm::l2Scaling_ (default 1). */
+int nalu_linsys_set_l2_scaling(nalu_linsys *ls, double l2_scaling);
+
+/* Sizes and copies of the assembled system (for checks and export). */
+int nalu_linsys_sizes(const nalu_linsys *ls, int64_t *num_rows, int64_t *nnz);
+int nalu_linsys_copy_csr(nalu_linsys *ls, int64_t *row_ptr_host, int32_t *col_idx_host,
+                         double *values_host, double *rhs_host);
+/* Device pointers to the CSR arrays (row_ptr/col_idx int32, values/rhs FP64). */
+int nalu_linsys_device_ptrs(nalu_linsys *ls, const int32_t **row_ptr_dev,
+                            const int32_t **col_idx_dev, double **values_dev, double **rhs_dev);
+/* Multicolor ordering used by the SGS preconditioner: colors, and rows per color. */
+int nalu_linsys_coloring(nalu_linsys *ls, int *num_colors, int64_t *color_ptr_host,
+                         int32_t *color_rows_host);
+
+/* Per-phase accumulated times in seconds (hipEvent), the reference's timer
+ * split (src/EquationSystem.C:270-292): [0] graph/init, [1] zero, [2] assemble,
+ * [3] dirichlet, [4] load_complete, [5] precond setup, [6] solve. */
+int nalu_linsys_phase_times(const nalu_linsys *ls, double *times7);
+
+/* Solver building blocks on the handle's matrix (device vectors of length rows). */
+int nalu_linsys_spmv(nalu_linsys *ls, const double *x_dev, double *y_dev);
+int nalu_linsys_spmv_f32(nalu_linsys *ls, const float *x_dev, float *y_dev);
+int nalu_linsys_precond_setup(nalu_linsys *ls);
+int nalu_linsys_precond_apply(nalu_linsys *ls, const double *r_dev, double *z_dev);
+/* Solve A x = b for a caller-provided rhs (x in/out is the initial guess). */
+int nalu_linsys_solve_rhs(nalu_linsys *ls, const double *b_dev, double *x_dev, int *iters,
+                          double *final_rel_resid);
+
+/* ------------------------------------------------------ assembly kernels */
+/* Each fused kernel performs, for every element: gather, Hex8 SCS geometry,
+ * the element kernel, and the scatter into the system (the consolidated
+ * AssembleElemSolverAlgorithm::execute, src/AssembleElemSolverAlgorithm.C:75-100).
+ * Contributions are ADDED to the system (call nalu_linsys_zero first).
+ * mode: 0 = owner-computes row gather (deterministic, no atomics, scalar
+ * systems), 1 = element-centric atomic scatter. */
+/* ScalarDiffElemKernel (src/kernel/ScalarDiffElemKernel.C:66-111). */
+int nalu_assemble_scalar_diff(nalu_linsys *ls, const nalu_mesh *m, const double *q_dev,
+                              const double *diff_coeff_dev, int shifted_grad_op, int mode);
+/* ComputeMdotElemAlgorithm::execute (src/ComputeMdotElemAlgorithm.C:85-250):
+ * mdot_dev [E][12]. */
+int nalu_compute_mdot(const nalu_mesh *m, const double *rho_dev, const double *p_dev,
+                      const double *u_dev, const double *gpdx_dev, double proj_time_scale,
+                      int shift_mdot, int shift_poisson, double *mdot_dev, void *stream);
+/* ContinuityAdvElemKernel (src/kernel/ContinuityAdvElemKernel.C:88-148). */
+int nalu_assemble_continuity(nalu_linsys *ls, const nalu_mesh *m, const double *rho_dev,
+                             const double *p_dev, const double *u_dev, const double *gpdx_dev,
+                             double proj_time_scale, int shift_mdot, int shift_poisson,
+                             int reduced_sens, int mode);
+/* MomentumAdvDiffElemKernel (src/kernel/MomentumAdvDiffElemKernel.C:72-186), numDof 3. */
+int nalu_assemble_momentum(nalu_linsys *ls, const nalu_mesh *m, const double *u_dev,
+                           const double *mu_dev, const double *mdot_dev, double include_divu,
+                           int skew_symmetric, int shifted_grad_op, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,383 @@
+"""MI355X-native Hex8 CVFEM assembly + linear solve (Nalu hot path).
+
+Python host binding over the C ABI of ``libnalu_hip.so`` (``include/nalu_hip.h``).
+The classes mirror the reference's interfaces:
+
+* :class:`LinearSystem` -- ``sierra::nalu::LinearSystem`` (include/LinearSystem.h:39-167)
+  as implemented by TpetraLinearSystem (src/TpetraLinearSystem.C); method names
+  follow the reference (``buildElemToNodeGraph``, ``finalizeLinearSystem``,
+  ``zeroSystem``, ``sumInto``, ``applyDirichletBCs``, ``loadComplete``, ``solve``).
+* :class:`SolverConfig` -- the ``linear_solvers`` YAML block
+  (src/LinearSolverConfig.C:34-108).
+* :func:`assemble_scalar_diff` / :func:`compute_mdot` / :func:`assemble_continuity` /
+  :func:`assemble_momentum` -- the consolidated element algorithms with
+  ScalarDiffElemKernel, ComputeMdotElemAlgorithm, ContinuityAdvElemKernel and
+  MomentumAdvDiffElemKernel.
+
+Device arrays are torch CUDA tensors (HIP on ROCm); torch only provides memory
+and streams.  All compute runs in the HIP kernels of libnalu_hip.so; there is no
+CPU fallback: without the library every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "lib", "available", "NaluError", "SolverConfig", "Mesh", "LinearSystem", "lattice", "normal_field",
+    "lognormal_field", "assemble_scalar_diff", "compute_mdot", "assemble_continuity", "assemble_momentum",
+    "METHOD", "PRECOND", "PRECISION",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnalu_hip.so")
+
+METHOD = {"gmres": 0, "cg": 1, "fgmres": 2}
+PRECOND = {"none": 0, "jacobi": 1, "sgs": 2, "mt_sgs": 2, "learned": 3}
+PRECISION = {"fp64": 0, "mixed": 1}
+
+_i64 = C.c_int64
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class NaluError(RuntimeError):
+    pass
+
+
+class _Config(C.Structure):
+    _fields_ = [("method", C.c_int), ("precond", C.c_int), ("tolerance", C.c_double),
+                ("final_tolerance", C.c_double), ("max_iterations", C.c_int), ("kspace", C.c_int),
+                ("precision", C.c_int), ("final_outer_iter", C.c_int)]
+
+
+_L = None
+
+
+def lib():
+    """Load libnalu_hip.so (built in-tree by __graft_entry__.build())."""
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise NaluError("libnalu_hip.so is not built (run __graft_entry__.build()): %s" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.nalu_last_error.restype = C.c_char_p
+        for name in ("nalu_linsys_linear_residual", "nalu_linsys_nonlinear_residual",
+                     "nalu_linsys_scaled_nonlinear_residual"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [_vp]
+        L.nalu_linsys_iterations.argtypes = [_vp]
+        L.nalu_lattice_generate.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint32, _dp, _i32p]
+        L.nalu_normal_field.argtypes = [_i64, C.c_double, C.c_uint32, _dp]
+        L.nalu_lognormal_field.argtypes = [_i64, C.c_double, C.c_uint32, _dp]
+        L.nalu_mesh_create.argtypes = [_i64, _i64, _i32p, _dp, _i64p, C.POINTER(_vp)]
+        L.nalu_mesh_destroy.argtypes = [_vp]
+        L.nalu_mesh_device_ptrs.argtypes = [_vp, C.POINTER(_vp), C.POINTER(_vp)]
+        L.nalu_linsys_create.argtypes = [C.c_int, C.POINTER(_Config), C.POINTER(_vp)]
+        L.nalu_linsys_destroy.argtypes = [_vp]
+        L.nalu_linsys_set_stream.argtypes = [_vp, _vp]
+        L.nalu_linsys_build_elem_to_node_graph.argtypes = [_vp, _vp]
+        L.nalu_linsys_finalize.argtypes = [_vp]
+        L.nalu_linsys_zero.argtypes = [_vp]
+        L.nalu_linsys_sum_into_batch.argtypes = [_vp, _i64, _vp, _vp, _vp]
+        L.nalu_linsys_sum_into.argtypes = [_vp, C.c_int, _i32p, _dp, _dp]
+        L.nalu_linsys_apply_dirichlet.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int, C.c_int]
+        L.nalu_linsys_reset_rows.argtypes = [_vp, _i64, _vp, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.nalu_linsys_load_complete.argtypes = [_vp]
+        L.nalu_linsys_solve.argtypes = [_vp, _vp]
+        L.nalu_linsys_set_l2_scaling.argtypes = [_vp, C.c_double]
+        L.nalu_linsys_sizes.argtypes = [_vp, _i64p, _i64p]
+        L.nalu_linsys_copy_csr.argtypes = [_vp, _i64p, _i32p, _dp, _dp]
+        L.nalu_linsys_device_ptrs.argtypes = [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]
+        L.nalu_linsys_coloring.argtypes = [_vp, C.POINTER(C.c_int), _i64p, _i32p]
+        L.nalu_linsys_phase_times.argtypes = [_vp, _dp]
+        L.nalu_linsys_spmv.argtypes = [_vp, _vp, _vp]
+        L.nalu_linsys_spmv_f32.argtypes = [_vp, _vp, _vp]
+        L.nalu_linsys_precond_setup.argtypes = [_vp]
+        L.nalu_linsys_precond_apply.argtypes = [_vp, _vp, _vp]
+        L.nalu_linsys_solve_rhs.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int), _dp]
+        L.nalu_assemble_scalar_diff.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int]
+        L.nalu_compute_mdot.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp]
+        L.nalu_assemble_continuity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int,
+                                               C.c_int, C.c_int]
+        L.nalu_assemble_momentum.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_int]
+        _L = L
+    return _L
+
+
+def available():
+    """True when the library is built and a HIP device is usable."""
+    try:
+        return bool(lib().nalu_device_available())
+    except (NaluError, OSError):
+        return False
+
+
+def _check(rc, what=""):
+    if rc < 0:
+        msg = lib().nalu_last_error().decode(errors="replace")
+        raise NaluError("%s failed (%d): %s" % (what, rc, msg))
+    return rc
+
+
+def _ptr(t):
+    """Device pointer of a torch CUDA tensor (or an int address)."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    if not t.is_cuda:
+        raise NaluError("expected a device (cuda) tensor")
+    if not t.is_contiguous():
+        raise NaluError("expected a contiguous tensor")
+    return t.data_ptr()
+
+
+def _np(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ------------------------------------------------------------------ generators
+def lattice(n, jitter=0.0, stretch=1.0, max_aspect=0.0, seed=1):
+    """Structured Hex8 lattice on the unit cube: (coords [(n+1)^3, 3], conn [n^3, 8])."""
+    nn = (n + 1) ** 3
+    coords = np.zeros((nn, 3), dtype=np.float64)
+    conn = np.zeros((n ** 3, 8), dtype=np.int32)
+    _check(lib().nalu_lattice_generate(int(n), float(jitter), float(stretch), float(max_aspect), int(seed),
+                                       coords.ctypes.data_as(_dp), conn.ctypes.data_as(_i32p)), "lattice")
+    return coords, conn
+
+
+def normal_field(n, sigma=1.0, seed=1):
+    out = np.zeros(int(n), dtype=np.float64)
+    _check(lib().nalu_normal_field(int(n), float(sigma), int(seed), out.ctypes.data_as(_dp)), "normal_field")
+    return out
+
+
+def lognormal_field(n, sigma=1.0, seed=1):
+    out = np.zeros(int(n), dtype=np.float64)
+    _check(lib().nalu_lognormal_field(int(n), float(sigma), int(seed), out.ctypes.data_as(_dp)), "lognormal_field")
+    return out
+
+
+# ------------------------------------------------------------------ mesh
+class Mesh:
+    """Device-resident Hex8 block (coords [N,3], conn [E,8], optional global ids)."""
+
+    def __init__(self, coords, conn, node_ids=None):
+        coords = _np(coords, np.float64)
+        conn = _np(conn, np.int32)
+        if coords.ndim != 2 or coords.shape[1] != 3 or conn.ndim != 2 or conn.shape[1] != 8:
+            raise NaluError("Mesh: coords must be [N,3] and conn [E,8]")
+        self.num_nodes = coords.shape[0]
+        self.num_elems = conn.shape[0]
+        ids = None if node_ids is None else _np(node_ids, np.int64)
+        h = _vp()
+        _check(lib().nalu_mesh_create(self.num_nodes, self.num_elems, conn.ctypes.data_as(_i32p),
+                                      coords.ctypes.data_as(_dp),
+                                      None if ids is None else ids.ctypes.data_as(_i64p), C.byref(h)), "mesh_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_ptrs(self):
+        cp, kp = _vp(), _vp()
+        _check(lib().nalu_mesh_device_ptrs(self._h, C.byref(cp), C.byref(kp)), "mesh_device_ptrs")
+        return cp.value, kp.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _L is not None:
+            _L.nalu_mesh_destroy(h)
+            self._h = None
+
+
+# ------------------------------------------------------------------ solver config
+class SolverConfig:
+    """The ``linear_solvers`` block: method, preconditioner, tolerance, final_tolerance,
+    max_iterations, kspace (src/LinearSolverConfig.C:34-108), plus the precision of
+    the Krylov inner loop."""
+
+    def __init__(self, method="gmres", preconditioner="sgs", tolerance=1e-8, final_tolerance=None,
+                 max_iterations=1000, kspace=50, precision="fp64", final_outer_iter=False):
+        self.method = method
+        self.preconditioner = preconditioner
+        self.tolerance = tolerance
+        self.final_tolerance = tolerance if final_tolerance is None else final_tolerance
+        self.max_iterations = max_iterations
+        self.kspace = kspace
+        self.precision = precision
+        self.final_outer_iter = final_outer_iter
+
+    def _struct(self):
+        return _Config(METHOD[self.method], PRECOND[self.preconditioner], float(self.tolerance),
+                       float(self.final_tolerance), int(self.max_iterations), int(self.kspace),
+                       PRECISION[self.precision], int(bool(self.final_outer_iter)))
+
+
+# ------------------------------------------------------------------ linear system
+class LinearSystem:
+    """HIP linear system with the reference's LinearSystem call sequence."""
+
+    def __init__(self, num_dof=1, config=None):
+        self.num_dof = num_dof
+        self.config = config or SolverConfig()
+        self._cfg = self.config._struct()
+        h = _vp()
+        _check(lib().nalu_linsys_create(int(num_dof), C.byref(self._cfg), C.byref(h)), "linsys_create")
+        self._h = h
+        self.mesh = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _L is not None:
+            _L.nalu_linsys_destroy(h)
+            self._h = None
+
+    # --- construction
+    def buildElemToNodeGraph(self, mesh):
+        self.mesh = mesh
+        _check(lib().nalu_linsys_build_elem_to_node_graph(self._h, mesh.handle), "buildElemToNodeGraph")
+
+    def finalizeLinearSystem(self):
+        _check(lib().nalu_linsys_finalize(self._h), "finalizeLinearSystem")
+
+    # --- assembly
+    def zeroSystem(self):
+        _check(lib().nalu_linsys_zero(self._h), "zeroSystem")
+
+    def sumInto(self, entities, rhs, lhs):
+        """std::vector overload: one element, host arrays (entities = node indices)."""
+        ents = _np(entities, np.int32)
+        rhs = _np(rhs, np.float64)
+        lhs = _np(lhs, np.float64)
+        _check(lib().nalu_linsys_sum_into(self._h, ents.shape[0], ents.ctypes.data_as(_i32p),
+                                          rhs.ctypes.data_as(_dp), lhs.ctypes.data_as(_dp)), "sumInto")
+
+    def sumIntoBatch(self, lhs, rhs, num_elems=None):
+        """Kokkos overload batched over the graph mesh's elements: lhs [E,R,R], rhs [E,R] device tensors."""
+        _, conn = self.mesh.device_ptrs()
+        ne = self.mesh.num_elems if num_elems is None else num_elems
+        _check(lib().nalu_linsys_sum_into_batch(self._h, ne, conn, _ptr(lhs), _ptr(rhs)), "sumIntoBatch")
+
+    def applyDirichletBCs(self, bc_nodes, solution, bc_values, begin_pos=0, end_pos=None):
+        end_pos = self.num_dof if end_pos is None else end_pos
+        _check(lib().nalu_linsys_apply_dirichlet(self._h, int(bc_nodes.numel()), _ptr(bc_nodes), _ptr(solution),
+                                                 _ptr(bc_values), begin_pos, end_pos), "applyDirichletBCs")
+
+    def resetRows(self, nodes, begin_pos=0, end_pos=None, diag_value=1.0, rhs_residual=0.0):
+        end_pos = self.num_dof if end_pos is None else end_pos
+        _check(lib().nalu_linsys_reset_rows(self._h, int(nodes.numel()), _ptr(nodes), begin_pos, end_pos,
+                                            float(diag_value), float(rhs_residual)), "resetRows")
+
+    def loadComplete(self):
+        _check(lib().nalu_linsys_load_complete(self._h), "loadComplete")
+
+    # --- solve
+    def solve(self, delta_out=None):
+        """Returns 0 when converged, 1 when max_iterations was reached."""
+        return _check(lib().nalu_linsys_solve(self._h, _ptr(delta_out)), "solve")
+
+    def solve_rhs(self, b, x):
+        it = C.c_int(0)
+        rel = C.c_double(0)
+        st = _check(lib().nalu_linsys_solve_rhs(self._h, _ptr(b), _ptr(x), C.byref(it), C.byref(rel)), "solve_rhs")
+        return st, it.value, rel.value
+
+    def linearSolveIterations(self):
+        return lib().nalu_linsys_iterations(self._h)
+
+    def linearResidual(self):
+        return lib().nalu_linsys_linear_residual(self._h)
+
+    def nonLinearResidual(self):
+        return lib().nalu_linsys_nonlinear_residual(self._h)
+
+    def scaledNonLinearResidual(self):
+        return lib().nalu_linsys_scaled_nonlinear_residual(self._h)
+
+    # --- introspection
+    def sizes(self):
+        nr, nz = C.c_int64(0), C.c_int64(0)
+        _check(lib().nalu_linsys_sizes(self._h, C.byref(nr), C.byref(nz)), "sizes")
+        return nr.value, nz.value
+
+    def csr(self):
+        """Host copies (row_ptr int64, col_idx int32, values, rhs)."""
+        nr, nz = self.sizes()
+        rp = np.zeros(nr + 1, dtype=np.int64)
+        ci = np.zeros(nz, dtype=np.int32)
+        v = np.zeros(nz, dtype=np.float64)
+        r = np.zeros(nr, dtype=np.float64)
+        _check(lib().nalu_linsys_copy_csr(self._h, rp.ctypes.data_as(_i64p), ci.ctypes.data_as(_i32p),
+                                          v.ctypes.data_as(_dp), r.ctypes.data_as(_dp)), "copy_csr")
+        return rp, ci, v, r
+
+    def device_ptrs(self):
+        a, b, c, d = _vp(), _vp(), _vp(), _vp()
+        _check(lib().nalu_linsys_device_ptrs(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)), "device_ptrs")
+        return a.value, b.value, c.value, d.value
+
+    def coloring(self):
+        nr, _ = self.sizes()
+        nc = C.c_int(0)
+        _check(lib().nalu_linsys_coloring(self._h, C.byref(nc), None, None), "coloring")
+        cp = np.zeros(nc.value + 1, dtype=np.int64)
+        cr = np.zeros(nr, dtype=np.int32)
+        _check(lib().nalu_linsys_coloring(self._h, C.byref(nc), cp.ctypes.data_as(_i64p), cr.ctypes.data_as(_i32p)),
+               "coloring")
+        return cp, cr
+
+    def phase_times(self):
+        t = np.zeros(7, dtype=np.float64)
+        _check(lib().nalu_linsys_phase_times(self._h, t.ctypes.data_as(_dp)), "phase_times")
+        return dict(zip(["init", "zero", "assemble", "dirichlet", "load_complete", "precond", "solve"], t))
+
+    def spmv(self, x, y):
+        _check(lib().nalu_linsys_spmv(self._h, _ptr(x), _ptr(y)), "spmv")
+
+    def spmv_f32(self, x, y):
+        _check(lib().nalu_linsys_spmv_f32(self._h, _ptr(x), _ptr(y)), "spmv_f32")
+
+    def precond_setup(self):
+        _check(lib().nalu_linsys_precond_setup(self._h), "precond_setup")
+
+    def precond_apply(self, r, z):
+        _check(lib().nalu_linsys_precond_apply(self._h, _ptr(r), _ptr(z)), "precond_apply")
+
+    def set_l2_scaling(self, s):
+        _check(lib().nalu_linsys_set_l2_scaling(self._h, float(s)), "set_l2_scaling")
+
+
+# ------------------------------------------------------------------ assembly algorithms
+MODE = {"gather": 0, "atomic": 1}
+
+
+def assemble_scalar_diff(ls, mesh, q, k, shifted=False, mode="gather"):
+    _check(lib().nalu_assemble_scalar_diff(ls.handle, mesh.handle, _ptr(q), _ptr(k), int(shifted), MODE[mode]),
+           "assemble_scalar_diff")
+
+
+def compute_mdot(mesh, rho, p, u, gpdx, mdot, tau=1.0, shift_mdot=False, shift_poisson=False):
+    _check(lib().nalu_compute_mdot(mesh.handle, _ptr(rho), _ptr(p), _ptr(u), _ptr(gpdx), float(tau),
+                                   int(shift_mdot), int(shift_poisson), _ptr(mdot), None), "compute_mdot")
+
+
+def assemble_continuity(ls, mesh, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False,
+                        reduced_sens=False, mode="gather"):
+    _check(lib().nalu_assemble_continuity(ls.handle, mesh.handle, _ptr(rho), _ptr(p), _ptr(u), _ptr(gpdx),
+                                          float(tau), int(shift_mdot), int(shift_poisson), int(reduced_sens),
+                                          MODE[mode]), "assemble_continuity")
+
+
+def assemble_momentum(ls, mesh, u, mu, mdot, include_divu=0.0, skew=False, shifted=False, mode="atomic"):
+    _check(lib().nalu_assemble_momentum(ls.handle, mesh.handle, _ptr(u), _ptr(mu), _ptr(mdot), float(include_divu),
+                                        int(skew), int(shifted), MODE[mode]), "assemble_momentum")

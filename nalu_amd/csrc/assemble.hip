@@ -1,0 +1,539 @@
+// Fused Hex8 CVFEM assembly: gather + scs geometry + element kernel + scatter
+// in one pass over the mesh (the consolidated AssembleElemSolverAlgorithm,
+// src/AssembleElemSolverAlgorithm.C:75-100 and include/AssembleElemSolverAlgorithm.h:52-105,
+// whose per-element lhs/rhs go through SolverAlgorithm::apply_coeff ->
+// TpetraLinearSystem::sumInto, src/TpetraLinearSystem.C:1457-1522).
+//
+// Two scatter strategies share the same per-(element, row-node) arithmetic
+// (hex8_device.hpp):
+//   * owner-computes row gather (mode 0): one thread per matrix row walks the
+//     elements incident to its node, accumulates its row in LDS and the block
+//     writes its contiguous CSR segment once, coalesced.  No atomics, fully
+//     deterministic, and zeroSystem's memset of the values can be folded in.
+//   * element-centric atomic scatter (mode 1): eight waves per block, wave w
+//     computing row w of 64 elements, FP64 atomic adds into the CSR.
+#include <hip/hip_runtime.h>
+
+#include "hex8_device.hpp"
+#include "internal.hpp"
+
+namespace nalu {
+namespace {
+
+inline unsigned blocks_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
+
+__device__ __forceinline__ void load_nodes(const int32_t* __restrict__ conn, int64_t e, int32_t (&nd)[8]) {
+  const int4* c4 = reinterpret_cast<const int4*>(conn + 8 * e);
+  const int4 a = c4[0], b = c4[1];
+  nd[0] = a.x; nd[1] = a.y; nd[2] = a.z; nd[3] = a.w;
+  nd[4] = b.x; nd[5] = b.y; nd[6] = b.z; nd[7] = b.w;
+}
+
+template <typename R>
+__device__ __forceinline__ void load_vec3(const double* __restrict__ f, const int32_t (&nd)[8], R (&v)[8][3]) {
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[n][d] = R(f[3 * (int64_t)nd[n] + d]);
+}
+
+template <typename R>
+__device__ __forceinline__ void load_scalar(const double* __restrict__ f, const int32_t (&nd)[8], R (&v)[8]) {
+#pragma unroll
+  for (int n = 0; n < 8; ++n) v[n] = R(f[nd[n]]);
+}
+
+// ---------------------------------------------------------------- physics ops
+// Each op evaluates row LA of its element matrix, calling emit(c, value) once
+// per column node c, and returns the row's rhs contribution.
+template <bool SHIFT>
+struct ScalarDiffOp {
+  const double* q;
+  const double* k;
+  template <int LA, class Emit>
+  __device__ __forceinline__ double row(const int32_t (&nd)[8], const double (&x)[8][3], Emit&& emit) const {
+    const hex8::Gather gt{nd};
+    return hex8::row_scalar_diff<LA, SHIFT, double>(x, gt, q, k, emit);
+  }
+};
+
+template <bool SHIFT_MDOT, bool SHIFT_POISSON, bool REDUCED>
+struct ContinuityOp {
+  const double* rho;
+  const double* p;
+  const double* u;
+  const double* gp;
+  double tau;
+  template <int LA, class Emit>
+  __device__ __forceinline__ double row(const int32_t (&nd)[8], const double (&x)[8][3], Emit&& emit) const {
+    const hex8::Gather gt{nd};
+    return hex8::row_continuity<LA, SHIFT_MDOT, SHIFT_POISSON, REDUCED, double>(x, gt, rho, p, u, gp, tau, emit);
+  }
+};
+
+// dispatch a wave-uniform (or mostly uniform) runtime row index to the
+// compile-time specialisation
+template <class Op, class Emit>
+__device__ __forceinline__ double dispatch_row(const Op& op, int la, const int32_t (&nd)[8], const double (&x)[8][3],
+                                               Emit&& emit) {
+  switch (la) {
+    case 0: return op.template row<0>(nd, x, emit);
+    case 1: return op.template row<1>(nd, x, emit);
+    case 2: return op.template row<2>(nd, x, emit);
+    case 3: return op.template row<3>(nd, x, emit);
+    case 4: return op.template row<4>(nd, x, emit);
+    case 5: return op.template row<5>(nd, x, emit);
+    case 6: return op.template row<6>(nd, x, emit);
+    default: return op.template row<7>(nd, x, emit);
+  }
+}
+
+__device__ __forceinline__ int32_t pick(const int32_t (&nd)[8], int la) {
+  int32_t node = nd[0];
+#pragma unroll
+  for (int c = 1; c < 8; ++c) node = (la == c) ? nd[c] : node;
+  return node;
+}
+
+// ---------------------------------------------------------------- scalar systems
+// Element-centric kernels: a block handles one local row node LA of a chunk of
+// kChunk elements.  The eight LA-blocks of a chunk are consecutive in the
+// dispatch order of one XCD (block b runs on XCD b % 8), so the chunk's nodal
+// data is shared through that XCD's L2.
+constexpr int kChunk = 256;
+
+struct ChunkMap {
+  int64_t e;   // element of this thread
+  int la;      // block-uniform local row node (0..NL-1)
+  bool valid;
+};
+template <int NL>
+__device__ __forceinline__ ChunkMap chunk_map(int64_t nelem) {
+  const unsigned b = blockIdx.x;
+  const unsigned xcd = b & 7, la = (b >> 3) % NL, grp = (b >> 3) / NL;
+  const int64_t chunk = (int64_t)grp * 8 + xcd;
+  const int64_t e = chunk * kChunk + threadIdx.x;
+  return {e, (int)la, e < nelem};
+}
+template <int NL>
+inline unsigned chunk_blocks(int64_t nelem) {
+  const int64_t nchunk = (nelem + kChunk - 1) / kChunk;
+  return (unsigned)(((nchunk + 7) / 8) * 8 * NL);
+}
+
+template <class Op, int LA>
+__device__ __forceinline__ void scalar_atomic_body(const Op& op, const int32_t* __restrict__ conn,
+                                                   const double* __restrict__ coords, const int32_t* __restrict__ lid,
+                                                   const int32_t* __restrict__ nrp,
+                                                   const uint64_t* __restrict__ elem_pos, int64_t e,
+                                                   double* __restrict__ vals, double* __restrict__ rhs) {
+  int32_t nd[8];
+  load_nodes(conn, e, nd);
+  double x[8][3];
+  load_vec3(coords, nd, x);
+  const int32_t row = lid[nd[LA]];
+  double* rowp = vals + nrp[row];
+  const uint64_t pk = elem_pos[8 * e + LA];
+  const double rr = op.template row<LA>(nd, x, [&](int c, double v) {
+    unsafeAtomicAdd(rowp + (int)((pk >> (8 * c)) & 0xff), v);
+  });
+  unsafeAtomicAdd(rhs + row, rr);
+}
+
+template <class Op>
+__global__ __launch_bounds__(kChunk) void k_scalar_atomic(Op op, const int32_t* __restrict__ conn,
+                                                          const double* __restrict__ coords,
+                                                          const int32_t* __restrict__ lid,
+                                                          const int32_t* __restrict__ nrp,
+                                                          const uint64_t* __restrict__ elem_pos, int64_t nelem,
+                                                          double* __restrict__ vals, double* __restrict__ rhs) {
+  const ChunkMap cm = chunk_map<8>(nelem);
+  if (!cm.valid) return;
+  switch (cm.la) {
+#define NALU_SA(L) \
+  case L: scalar_atomic_body<Op, L>(op, conn, coords, lid, nrp, elem_pos, cm.e, vals, rhs); break;
+    NALU_SA(0) NALU_SA(1) NALU_SA(2) NALU_SA(3) NALU_SA(4) NALU_SA(5) NALU_SA(6) NALU_SA(7)
+#undef NALU_SA
+  }
+}
+
+// Owner-computes row gather.  Block b owns rows [b*B, b*B+B); its CSR segment
+// lives in LDS while the rows accumulate, then is streamed out once.
+template <class Op>
+__global__ __launch_bounds__(256) void k_scalar_gather(Op op, const int32_t* __restrict__ conn,
+                                                       const double* __restrict__ coords,
+                                                       const int32_t* __restrict__ row_node,
+                                                       const int32_t* __restrict__ inc_ptr,
+                                                       const int32_t* __restrict__ inc_elem,
+                                                       const int32_t* __restrict__ nrp,
+                                                       const uint64_t* __restrict__ elem_pos, int64_t nrows,
+                                                       double* __restrict__ vals, double* __restrict__ rhs,
+                                                       int accumulate) {
+  extern __shared__ double seg[];
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t rend = min(r0 + (int64_t)blockDim.x, nrows);
+  const int64_t r = r0 + threadIdx.x;
+  const int32_t s0 = nrp[r0], s1 = nrp[rend];
+  const int nseg = s1 - s0;
+  for (int i = threadIdx.x; i < nseg; i += blockDim.x) seg[i] = 0.0;
+  __syncthreads();
+  double rr = 0.0;
+  if (r < nrows) {
+    const int32_t a = row_node ? row_node[r] : (int32_t)r;
+    double* myrow = seg + (nrp[r] - s0);
+    const int32_t q1 = inc_ptr[a + 1];
+    for (int32_t q = inc_ptr[a]; q < q1; ++q) {
+      const int32_t t = inc_elem[q];
+      const int64_t e = t >> 3;
+      const int la = t & 7;
+      int32_t nd[8];
+      load_nodes(conn, e, nd);
+      double x[8][3];
+      load_vec3(coords, nd, x);
+      const uint64_t pk = elem_pos[t];
+      rr += dispatch_row(op, la, nd, x, [&](int c, double v) { myrow[(int)((pk >> (8 * c)) & 0xff)] += v; });
+    }
+  }
+  __syncthreads();
+  if (accumulate) {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
+    if (r < nrows) rhs[r] += rr;
+  } else {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] = seg[i];
+    if (r < nrows) rhs[r] = rr;
+  }
+}
+
+// ---------------------------------------------------------------- momentum
+template <int LA, bool SKEW, bool SHIFT>
+__device__ __forceinline__ void momentum_row_atomic(const int32_t (&nd)[8], const double (&x)[8][3],
+                                                    const double* __restrict__ u, const double* __restrict__ mu,
+                                                    const double* __restrict__ md, double incl_divu,
+                                                    double* __restrict__ rowp, int32_t L, uint64_t pk,
+                                                    double (&rh)[3]) {
+  const hex8::Gather gt{nd};
+  const double m3[3] = {md[hex8::adj_ip(LA, 0)], md[hex8::adj_ip(LA, 1)], md[hex8::adj_ip(LA, 2)]};
+  hex8::row_momentum<LA, SKEW, SHIFT, double>(x, gt, u, mu, m3, incl_divu, rh, [&](int c, const double (&blk)[3][3]) {
+    const int p = (int)((pk >> (8 * c)) & 0xff);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) unsafeAtomicAdd(rowp + 3 * i * L + 3 * p + j, blk[i][j]);
+  });
+}
+
+template <int LA, bool SKEW, bool SHIFT>
+__device__ __forceinline__ void momentum_body(const int32_t* __restrict__ conn, const double* __restrict__ coords,
+                                              const double* __restrict__ u, const double* __restrict__ mu,
+                                              const double* __restrict__ mdot, double incl_divu,
+                                              const int32_t* __restrict__ lid, const int32_t* __restrict__ nrp,
+                                              const uint64_t* __restrict__ elem_pos, int64_t e,
+                                              double* __restrict__ vals, double* __restrict__ rhs) {
+  int32_t nd[8];
+  load_nodes(conn, e, nd);
+  double x[8][3];
+  load_vec3(coords, nd, x);
+  const int32_t nr = lid[nd[LA]];
+  const int32_t nb = nrp[nr];
+  const int32_t L = nrp[nr + 1] - nb;
+  double* rowp = vals + 9 * (int64_t)nb;
+  const uint64_t pk = elem_pos[8 * e + LA];
+  double rh[3] = {0, 0, 0};
+  momentum_row_atomic<LA, SKEW, SHIFT>(nd, x, u, mu, mdot + 12 * e, incl_divu, rowp, L, pk, rh);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) unsafeAtomicAdd(rhs + 3 * (int64_t)nr + i, rh[i]);
+}
+
+template <bool SKEW, bool SHIFT>
+__global__ __launch_bounds__(kChunk) void k_momentum_atomic(
+    const int32_t* __restrict__ conn, const double* __restrict__ coords, const double* __restrict__ u,
+    const double* __restrict__ mu, const double* __restrict__ mdot, double incl_divu,
+    const int32_t* __restrict__ lid, const int32_t* __restrict__ nrp, const uint64_t* __restrict__ elem_pos,
+    int64_t nelem, double* __restrict__ vals, double* __restrict__ rhs) {
+  const ChunkMap cm = chunk_map<8>(nelem);
+  if (!cm.valid) return;
+  switch (cm.la) {
+#define NALU_MA(L)                                                                                          \
+  case L:                                                                                                   \
+    momentum_body<L, SKEW, SHIFT>(conn, coords, u, mu, mdot, incl_divu, lid, nrp, elem_pos, cm.e, vals, rhs); \
+    break;
+    NALU_MA(0) NALU_MA(1) NALU_MA(2) NALU_MA(3) NALU_MA(4) NALU_MA(5) NALU_MA(6) NALU_MA(7)
+#undef NALU_MA
+  }
+}
+
+// ---------------------------------------------------------------- mdot
+template <int IP, bool SHIFT_MDOT, bool SHIFT_POISSON>
+__device__ __forceinline__ double mdot_one(const int32_t (&nd)[8], const double (&x)[8][3], const double* rho,
+                                           const double* p, const double* u, const double* gp, double tau) {
+  hex8::IpGeo<double, double> G;
+  hex8::ip_geo<IP, SHIFT_POISSON>(x, G);
+  return hex8::ip_mdot_stream<IP, SHIFT_MDOT, SHIFT_POISSON>(G, hex8::Gather{nd}, rho, p, u, gp, tau);
+}
+
+template <bool SHIFT_MDOT, bool SHIFT_POISSON>
+__global__ __launch_bounds__(kChunk) void k_mdot(const int32_t* __restrict__ conn, const double* __restrict__ coords,
+                                                 const double* __restrict__ rho, const double* __restrict__ p,
+                                                 const double* __restrict__ u, const double* __restrict__ gp,
+                                                 double tau, int64_t nelem, double* __restrict__ mdot) {
+  const ChunkMap cm = chunk_map<12>(nelem);  // block-uniform scs index
+  if (!cm.valid) return;
+  int32_t nd[8];
+  load_nodes(conn, cm.e, nd);
+  double x[8][3];
+  load_vec3(coords, nd, x);
+  double m = 0.0;
+  switch (cm.la) {
+#define NALU_MD(IP) \
+  case IP: m = mdot_one<IP, SHIFT_MDOT, SHIFT_POISSON>(nd, x, rho, p, u, gp, tau); break;
+    NALU_MD(0) NALU_MD(1) NALU_MD(2) NALU_MD(3) NALU_MD(4) NALU_MD(5)
+    NALU_MD(6) NALU_MD(7) NALU_MD(8) NALU_MD(9) NALU_MD(10) NALU_MD(11)
+#undef NALU_MD
+  }
+  mdot[12 * cm.e + cm.la] = m;
+}
+
+// ---------------------------------------------------------------- generic sumInto
+// Batched sumInto of caller-provided dense element matrices.
+__global__ void k_sum_into_batch(const int32_t* __restrict__ conn, const double* __restrict__ lhs,
+                                 const double* __restrict__ rhs_e, const int32_t* __restrict__ lid,
+                                 const int32_t* __restrict__ nrp, const uint64_t* __restrict__ elem_pos,
+                                 int64_t nelem, int nd, double* __restrict__ vals, double* __restrict__ rhs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (element, row node, dof)
+  if (t >= nelem * 8 * nd) return;
+  const int64_t e = t / (8 * nd);
+  const int rem = (int)(t - e * 8 * nd);
+  const int a = rem / nd, d = rem % nd;
+  const int R = 8 * nd;
+  const int32_t nr = lid[conn[8 * e + a]];
+  const int32_t nb = nrp[nr];
+  const int32_t L = nrp[nr + 1] - nb;
+  const uint64_t pk = elem_pos[8 * e + a];
+  const double* lrow = lhs + (e * R + (int64_t)(a * nd + d)) * R;
+  double* rowp = vals + (int64_t)nd * nd * nb + (int64_t)d * nd * L;
+  for (int c = 0; c < 8; ++c) {
+    const int p = (int)((pk >> (8 * c)) & 0xff);
+    for (int j = 0; j < nd; ++j) unsafeAtomicAdd(rowp + nd * p + j, lrow[c * nd + j]);
+  }
+  unsafeAtomicAdd(rhs + (int64_t)nd * nr + d, rhs_e[e * R + a * nd + d]);
+}
+
+// ---------------------------------------------------------------- Dirichlet / resetRows
+__global__ void k_dirichlet(const int32_t* __restrict__ nodes, int64_t nbc, int64_t nnodes, int nd, int begin,
+                            int end, const int32_t* __restrict__ lid, const int32_t* __restrict__ rp,
+                            const int32_t* __restrict__ ci, const double* __restrict__ sol,
+                            const double* __restrict__ bcv, double* __restrict__ vals, double* __restrict__ rhs,
+                            int reset_mode, double diag_value, double rhs_residual, int* __restrict__ bad) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nsub = end - begin;
+  if (t >= nbc * nsub) return;
+  const int64_t k = t / nsub;
+  const int d = begin + (int)(t % nsub);
+  const int32_t node = nodes[k];
+  if (node < 0 || node >= nnodes) {  // caller-provided list: never index outside the mesh
+    atomicOr(bad, 1);
+    return;
+  }
+  const int64_t row = (int64_t)nd * lid[node] + d;
+  const double dv = reset_mode ? diag_value : 1.0;
+  for (int32_t q = rp[row]; q < rp[row + 1]; ++q) vals[q] = (ci[q] == row) ? dv : 0.0;
+  rhs[row] = reset_mode ? rhs_residual : bcv[(int64_t)nd * node + d] - sol[(int64_t)nd * node + d];
+}
+
+template <class Op>
+void launch_scalar(nalu_linsys* ls, const nalu_mesh* m, const Op& op, int mode) {
+  hipStream_t s = ls->stream;
+  const int64_t E = m->num_elems;
+  if (mode == 0) {
+    build_node_incidence(const_cast<nalu_mesh*>(m), s);
+    const int accumulate = ls->values_pending_zero ? 0 : 1;
+    NALU_REQUIRE(ls->values_pending_zero == ls->rhs_pending_zero, NALU_ERR_STATE,
+                 "gather assembly: values/rhs zero state diverged");
+    int B = 256;
+    while (B > 32 && (size_t)B * ls->max_node_row_len * sizeof(double) > 64 * 1024) B >>= 1;
+    const size_t lds = (size_t)B * ls->max_node_row_len * sizeof(double);
+    hipLaunchKernelGGL((k_scalar_gather<Op>), dim3(blocks_for(ls->nrows, B)), dim3(B), lds, s, op, m->conn.p,
+                       m->coords.p, (const int32_t*)ls->row_node.p, m->inc_ptr.p, m->inc_elem.p, ls->nrp.p,
+                       ls->elem_pos.p, ls->nrows, ls->values.p, ls->rhs.p, accumulate);
+    ls->values_pending_zero = ls->rhs_pending_zero = false;
+  } else {
+    materialize_pending_zero(ls);
+    hipLaunchKernelGGL((k_scalar_atomic<Op>), dim3(chunk_blocks<8>(E)), dim3(kChunk), 0, s, op, m->conn.p,
+                       m->coords.p, ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, E, ls->values.p, ls->rhs.p);
+  }
+  NALU_HIP_CHECK(hipGetLastError());
+}
+
+void check_flag(DevBuf<int>& flag, hipStream_t s, const char* msg) {
+  int h = 0;
+  NALU_HIP_CHECK(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  NALU_HIP_CHECK(hipStreamSynchronize(s));
+  NALU_REQUIRE(h == 0, NALU_ERR_ARG, msg);
+}
+
+void check_assembly_args(nalu_linsys* ls, const nalu_mesh* m, int nd) {
+  NALU_REQUIRE(ls && m, NALU_ERR_ARG, "assembly: null handle");
+  NALU_REQUIRE(ls->finalized, NALU_ERR_STATE, "assembly: finalizeLinearSystem not called");
+  NALU_REQUIRE(ls->mesh == m, NALU_ERR_ARG, "assembly: mesh differs from the one the graph was built on");
+  NALU_REQUIRE(ls->num_dof == nd, NALU_ERR_ARG, "assembly: numDof mismatch for this kernel");
+}
+
+}  // namespace
+
+void materialize_pending_zero(nalu_linsys* ls) {
+  if (ls->values_pending_zero)
+    NALU_HIP_CHECK(hipMemsetAsync(ls->values.p, 0, ls->values.bytes(), ls->stream));
+  if (ls->rhs_pending_zero) NALU_HIP_CHECK(hipMemsetAsync(ls->rhs.p, 0, ls->rhs.bytes(), ls->stream));
+  ls->values_pending_zero = ls->rhs_pending_zero = false;
+}
+
+}  // namespace nalu
+
+using namespace nalu;
+
+extern "C" int nalu_assemble_scalar_diff(nalu_linsys* ls, const nalu_mesh* m, const double* q, const double* k,
+                                         int shifted, int mode) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 1);
+    NALU_REQUIRE(q && k, NALU_ERR_ARG, "scalar_diff: null field");
+    NALU_REQUIRE(mode == 0 || mode == 1, NALU_ERR_ARG, "scalar_diff: mode must be 0 or 1");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    if (shifted) launch_scalar(ls, m, ScalarDiffOp<true>{q, k}, mode);
+    else launch_scalar(ls, m, ScalarDiffOp<false>{q, k}, mode);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_assemble_continuity(nalu_linsys* ls, const nalu_mesh* m, const double* rho, const double* p,
+                                        const double* u, const double* gp, double tau, int shift_mdot,
+                                        int shift_poisson, int reduced_sens, int mode) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 1);
+    NALU_REQUIRE(rho && p && u && gp, NALU_ERR_ARG, "continuity: null field");
+    NALU_REQUIRE(tau > 0.0, NALU_ERR_ARG, "continuity: projTimeScale must be positive");
+    NALU_REQUIRE(mode == 0 || mode == 1, NALU_ERR_ARG, "continuity: mode must be 0 or 1");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    const int key = (shift_mdot ? 4 : 0) | (shift_poisson ? 2 : 0) | (reduced_sens ? 1 : 0);
+    switch (key) {
+#define NALU_CT(K, A, B, C) \
+  case K: launch_scalar(ls, m, ContinuityOp<A, B, C>{rho, p, u, gp, tau}, mode); break;
+      NALU_CT(0, false, false, false)
+      NALU_CT(1, false, false, true)
+      NALU_CT(2, false, true, false)
+      NALU_CT(3, false, true, true)
+      NALU_CT(4, true, false, false)
+      NALU_CT(5, true, false, true)
+      NALU_CT(6, true, true, false)
+      NALU_CT(7, true, true, true)
+#undef NALU_CT
+    }
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_compute_mdot(const nalu_mesh* m, const double* rho, const double* p, const double* u,
+                                 const double* gp, double tau, int shift_mdot, int shift_poisson, double* mdot,
+                                 void* stream) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(m && rho && p && u && gp && mdot, NALU_ERR_ARG, "mdot: null argument");
+    NALU_REQUIRE(tau > 0.0, NALU_ERR_ARG, "mdot: projTimeScale must be positive");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t E = m->num_elems;
+    const dim3 g(chunk_blocks<12>(E)), b(kChunk);
+    if (shift_mdot && shift_poisson)
+      hipLaunchKernelGGL((k_mdot<true, true>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
+    else if (shift_mdot)
+      hipLaunchKernelGGL((k_mdot<true, false>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
+    else if (shift_poisson)
+      hipLaunchKernelGGL((k_mdot<false, true>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
+    else
+      hipLaunchKernelGGL((k_mdot<false, false>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
+    NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_assemble_momentum(nalu_linsys* ls, const nalu_mesh* m, const double* u, const double* mu,
+                                      const double* mdot, double include_divu, int skew, int shifted, int mode) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 3);
+    NALU_REQUIRE(u && mu && mdot, NALU_ERR_ARG, "momentum: null field");
+    NALU_REQUIRE(mode == 1, NALU_ERR_UNSUPPORTED, "momentum: only the atomic scatter (mode 1) is implemented");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    materialize_pending_zero(ls);
+    const int64_t E = m->num_elems;
+    const dim3 g(chunk_blocks<8>(E)), b(kChunk);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, g, b, 0, ls->stream, m->conn.p, m->coords.p, u, mu, mdot, include_divu,
+                         ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, E, ls->values.p, ls->rhs.p);
+    };
+    if (skew && shifted) go(k_momentum_atomic<true, true>);
+    else if (skew) go(k_momentum_atomic<true, false>);
+    else if (shifted) go(k_momentum_atomic<false, true>);
+    else go(k_momentum_atomic<false, false>);
+    NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_linsys_sum_into_batch(nalu_linsys* ls, int64_t nelem, const int32_t* conn, const double* lhs,
+                                          const double* rhs_e) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized, NALU_ERR_STATE, "sum_into_batch: system not finalized");
+    NALU_REQUIRE(nelem >= 0 && (nelem == 0 || (conn && lhs && rhs_e)), NALU_ERR_ARG, "sum_into_batch: bad args");
+    NALU_REQUIRE(ls->mesh && nelem <= ls->mesh->num_elems, NALU_ERR_ARG, "sum_into_batch: more elements than the graph");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    materialize_pending_zero(ls);
+    if (nelem == 0) return NALU_OK;
+    // the element scatter table indexes the graph's own elements
+    NALU_REQUIRE(conn == ls->mesh->conn.p, NALU_ERR_ARG,
+                 "sum_into_batch: conn must be the graph mesh's connectivity (nalu_mesh_device_ptrs)");
+    const int64_t n = nelem * 8 * ls->num_dof;
+    hipLaunchKernelGGL(k_sum_into_batch, dim3(blocks_for(n, 256)), dim3(256), 0, ls->stream, conn, lhs, rhs_e,
+                       ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, nelem, ls->num_dof, ls->values.p, ls->rhs.p);
+    NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_linsys_apply_dirichlet(nalu_linsys* ls, int64_t nbc, const int32_t* nodes, const double* sol,
+                                           const double* bcv, int begin, int end) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized, NALU_ERR_STATE, "applyDirichletBCs: system not finalized");
+    NALU_REQUIRE(begin >= 0 && end <= ls->num_dof && begin <= end, NALU_ERR_ARG, "applyDirichletBCs: bad dof range");
+    NALU_REQUIRE(nbc == 0 || (nodes && sol && bcv), NALU_ERR_ARG, "applyDirichletBCs: null array");
+    PhaseTimer t(ls, PH_DIRICHLET);
+    materialize_pending_zero(ls);
+    const int64_t n = nbc * (end - begin);
+    if (n == 0) return NALU_OK;
+    DevBuf<int> bad(1);
+    NALU_HIP_CHECK(hipMemsetAsync(bad.p, 0, sizeof(int), ls->stream));
+    hipLaunchKernelGGL(k_dirichlet, dim3(blocks_for(n, 256)), dim3(256), 0, ls->stream, nodes, nbc, ls->num_nodes,
+                       ls->num_dof, begin, end, ls->node_lid.p, ls->row_ptr.p, ls->col_idx.p, sol, bcv,
+                       ls->values.p, ls->rhs.p, 0, 0.0, 0.0, bad.p);
+    NALU_HIP_CHECK(hipGetLastError());
+    check_flag(bad, ls->stream, "applyDirichletBCs: node index outside the mesh");
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_linsys_reset_rows(nalu_linsys* ls, int64_t nn, const int32_t* nodes, int begin, int end,
+                                      double diag_value, double rhs_residual) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized, NALU_ERR_STATE, "resetRows: system not finalized");
+    NALU_REQUIRE(begin >= 0 && end <= ls->num_dof && begin <= end, NALU_ERR_ARG, "resetRows: bad dof range");
+    NALU_REQUIRE(nn == 0 || nodes, NALU_ERR_ARG, "resetRows: null node list");
+    PhaseTimer t(ls, PH_DIRICHLET);
+    materialize_pending_zero(ls);
+    const int64_t n = nn * (end - begin);
+    if (n == 0) return NALU_OK;
+    DevBuf<int> bad(1);
+    NALU_HIP_CHECK(hipMemsetAsync(bad.p, 0, sizeof(int), ls->stream));
+    hipLaunchKernelGGL(k_dirichlet, dim3(blocks_for(n, 256)), dim3(256), 0, ls->stream, nodes, nn, ls->num_nodes,
+                       ls->num_dof, begin, end, ls->node_lid.p, ls->row_ptr.p, ls->col_idx.p, (const double*)nullptr,
+                       (const double*)nullptr, ls->values.p, ls->rhs.p, 1, diag_value, rhs_residual, bad.p);
+    NALU_HIP_CHECK(hipGetLastError());
+    check_flag(bad, ls->stream, "resetRows: node index outside the mesh");
+    return NALU_OK;
+  });
+}

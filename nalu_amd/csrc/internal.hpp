@@ -1,0 +1,191 @@
+// Internal definitions shared by the translation units of libnalu_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nalu_hip.h"
+
+namespace nalu {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define NALU_HIP_CHECK(expr)                                                              \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      throw ::nalu::Error(NALU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e) + \
+                                            " (" __FILE__ ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+#define NALU_REQUIRE(cond, code, msg)                       \
+  do {                                                      \
+    if (!(cond)) throw ::nalu::Error((code), (msg));        \
+  } while (0)
+
+// Runs f() and converts exceptions into status codes + last-error text.
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return NALU_ERR_ARG;
+  }
+}
+
+// -------------------------------------------------------- device buffers
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count) NALU_HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------- timers
+enum Phase { PH_INIT = 0, PH_ZERO, PH_ASSEMBLE, PH_DIRICHLET, PH_LOADCOMPLETE, PH_PRECOND, PH_SOLVE, PH_COUNT };
+
+struct PendingTimer {
+  int phase;
+  hipEvent_t a, b;
+};
+
+}  // namespace nalu
+
+// ------------------------------------------------------------------- mesh
+struct nalu_mesh {
+  int64_t num_nodes = 0, num_elems = 0;
+  nalu::DevBuf<double> coords;   // [N][3]
+  nalu::DevBuf<int32_t> conn;    // [E][8]
+  nalu::DevBuf<int64_t> ids;     // [N] global ids (empty: id = index+1)
+  // node -> element incidence (built on demand by the graph build)
+  nalu::DevBuf<int32_t> inc_ptr;   // [N+1]
+  nalu::DevBuf<int32_t> inc_elem;  // [8E] element*8 + local node index, sorted per node
+  int max_incidence = 0;
+};
+
+// ---------------------------------------------------------- linear system
+struct nalu_linsys {
+  int num_dof = 1;
+  nalu_solver_config cfg{};
+  hipStream_t stream = nullptr;
+
+  // construction state
+  const nalu_mesh* mesh = nullptr;  // the mesh whose elements define the graph
+  bool in_construction = false;
+  bool finalized = false;
+
+  int64_t num_nodes = 0;
+  int64_t nrows = 0;
+  int64_t nnz = 0;
+  int max_node_row_len = 0;  // longest node-graph row (neighbors incl. self)
+
+  // node graph (numDof = 1): rows ordered by global id
+  nalu::DevBuf<int32_t> node_lid;  // [N] rank of node id (row of dof 0 = num_dof*lid)
+  nalu::DevBuf<int32_t> nrp;       // node graph row_ptr [N+1]
+  nalu::DevBuf<int32_t> nci;       // node graph col idx (node lids) [P]
+  // element scatter table: for element e, local row node a: 8 bytes, byte c =
+  // position of column node c in node-row of node a.
+  nalu::DevBuf<uint64_t> elem_pos; // [E][8]
+
+  // expanded CSR (numDof rows interleaved, src/TpetraLinearSystem.C:1227-1246)
+  nalu::DevBuf<int32_t> row_ptr;   // [nrows+1]
+  nalu::DevBuf<int32_t> col_idx;   // [nnz]
+  nalu::DevBuf<int32_t> diag_pos;  // [nrows]
+  nalu::DevBuf<double> values;     // [nnz]
+  nalu::DevBuf<double> rhs;        // [nrows]
+  nalu::DevBuf<double> sln;        // [nrows]
+  bool values_pending_zero = false;  // zeroSystem deferred into the first gather assembly
+  bool rhs_pending_zero = false;
+
+  // multicolor ordering for SGS
+  int num_colors = 0;
+  std::vector<int64_t> color_ptr_host;
+  nalu::DevBuf<int32_t> color_rows;  // rows sorted by color
+
+  // preconditioner / solver workspace
+  nalu::DevBuf<double> inv_diag;   // [nrows]
+  nalu::DevBuf<float> values32;    // FP32 copy of values (mixed precision)
+  nalu::DevBuf<float> inv_diag32;
+  nalu::DevBuf<double> work;       // Krylov vectors
+  nalu::DevBuf<float> work32;
+  nalu::DevBuf<double> red;        // reduction partials
+  nalu::DevBuf<double> red_out;    // reduction results
+  std::vector<double> red_host;
+  bool precond_ready = false;
+
+  // learned preconditioner (optional)
+  void* learned = nullptr;
+
+  // results
+  int iters = 0;
+  double linear_residual = 0.0;
+  double nonlinear_residual = 0.0;
+  double first_nonlinear_residual = -1.0;
+  double scaled_nonlinear_residual = 0.0;
+  double l2_scaling = 1.0;
+  double phase[nalu::PH_COUNT] = {0};
+  bool timing = true;
+  std::vector<nalu::PendingTimer> pending_timers;
+  nalu::DevBuf<int32_t> row_node;  // inverse of node_lid when global ids are explicit
+};
+
+namespace nalu {
+// hipEvent pair around one API phase; accumulated lazily (no synchronization
+// on the hot path) -- the reference's per-equation timers, src/EquationSystem.C:270-292.
+struct PhaseTimer {
+  nalu_linsys* ls;
+  int ph;
+  hipEvent_t a = nullptr, b = nullptr;
+  PhaseTimer(nalu_linsys* l, int p);
+  ~PhaseTimer();
+};
+void flush_timers(nalu_linsys* ls);
+
+// graph.hip
+void build_node_incidence(nalu_mesh* m, hipStream_t s);
+void build_graph(nalu_linsys* ls, hipStream_t s);
+void build_coloring(nalu_linsys* ls, hipStream_t s);
+// assemble.hip
+void materialize_pending_zero(nalu_linsys* ls);
+// krylov.hip
+void precond_setup(nalu_linsys* ls);
+void precond_apply(nalu_linsys* ls, const double* r, double* z);
+void spmv(nalu_linsys* ls, const double* x, double* y);
+int krylov_solve(nalu_linsys* ls, const double* b, double* x, int* iters, double* final_rel_resid);
+double norm2(nalu_linsys* ls, const double* x, int64_t n);
+}  // namespace nalu

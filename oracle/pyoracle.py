@@ -1,0 +1,356 @@
+"""ctypes bindings for the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product (nalu_amd) never
+imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = C.POINTER(C.c_double)
+_i32 = C.POINTER(C.c_int32)
+_i64 = C.POINTER(C.c_int64)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.or_graph_build.restype = C.c_int64
+        L.or_gmres.restype = C.c_int
+        L.or_cg.restype = C.c_int
+        L.or_hex_volume_grandy.restype = C.c_double
+        L.or_wtime.restype = C.c_double
+        L.or_hexscs_intgloc.restype = _d
+        L.or_hexscv_intgloc.restype = _d
+        L.or_hexscs_lrscv.restype = C.POINTER(C.c_int)
+        I, I64, D = C.c_int, C.c_int64, C.c_double
+        sig = {
+            "or_hex8_shape_fcn": [I, _d, _d], "or_hex8_derivative": [I, _d, _d],
+            "or_subdivide_hex8": [_d, _d], "or_hex_scs_areav": [_d, _d],
+            "or_hexscs_grad_op": [_d, I, _d, _d], "or_hexscv_grad_op": [_d, I, _d, _d],
+            "or_hex_volume_grandy": [_d], "or_hexscv_volume": [_d, _d],
+            "or_scalar_diff_elem": [_d, _d, _d, I, _d, _d],
+            "or_continuity_adv_elem": [_d, _d, _d, _d, _d, D, I, I, I, _d, _d],
+            "or_mdot_elem": [_d, _d, _d, _d, _d, D, I, I, _d],
+            "or_mdot_rhou_elem": [_d, _d, _d, _d],
+            "or_momentum_advdiff_elem": [_d, _d, _d, _d, D, I, I, _d, _d],
+            "or_graph_build": [I64, _i64, I64, I, _i32, I, _i64, _i32, _i32],
+            "or_sum_into": [I, _i64, _i32, _d, _d, _i32, I, _i32, _d, _d],
+            "or_apply_dirichlet": [I, I64, _i64, _i32, _d, _d, _i32, I64, _i32, _d, _d, I, I],
+            "or_assemble_scalar_diff": [I64, _i32, _d, _d, _d, I, _i64, _i32, _i32, _d, _d],
+            "or_compute_mdot": [I64, _i32, _d, _d, _d, _d, _d, D, I, I, _d],
+            "or_assemble_continuity": [I64, _i32, _d, _d, _d, _d, _d, D, I, I, I, _i64, _i32, _i32, _d, _d],
+            "or_assemble_momentum": [I64, _i32, _d, _d, _d, _d, D, I, I, _i64, _i32, _i32, _d, _d],
+            "or_spmv": [I64, _i64, _i32, _d, _d, _d],
+            "or_sgs_apply": [I64, _i64, _i32, _d, _d, _d],
+            "or_mcsgs_apply": [I64, _i64, _i32, _d, I, _i64, _i32, _d, _d],
+            "or_jacobi_apply": [I64, _i64, _i32, _d, _d, _d],
+            "or_gmres": [I64, _i64, _i32, _d, _d, _d, I, I, D, I, I, _i64, _i32, _d],
+            "or_cg": [I64, _i64, _i32, _d, _d, _d, I, D, I, I, _i64, _i32, _d],
+            "or_greedy_color": [I64, _i64, _i32, _i32],
+        }
+        for name, args in sig.items():
+            getattr(L, name).argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=np.float64):
+    assert a.dtype == t and a.flags.c_contiguous, (a.dtype, t)
+    if t == np.float64:
+        return a.ctypes.data_as(_d)
+    if t == np.int32:
+        return a.ctypes.data_as(_i32)
+    return a.ctypes.data_as(_i64)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+# ---- master element ----
+def scs_intgloc(shifted=False):
+    p = lib().or_hexscs_intgloc(int(shifted))
+    return np.ctypeslib.as_array(p, shape=(12, 3)).copy()
+
+
+def scv_intgloc(shifted=False):
+    p = lib().or_hexscv_intgloc(int(shifted))
+    return np.ctypeslib.as_array(p, shape=(8, 3)).copy()
+
+
+def lrscv():
+    p = lib().or_hexscs_lrscv()
+    return np.ctypeslib.as_array(p, shape=(12, 2)).copy()
+
+
+def shape_fcn(loc):
+    loc = f64(loc)
+    out = np.zeros((loc.shape[0], 8))
+    lib().or_hex8_shape_fcn(loc.shape[0], _p(loc), _p(out))
+    return out
+
+
+def derivative(loc):
+    loc = f64(loc)
+    out = np.zeros((loc.shape[0], 8, 3))
+    lib().or_hex8_derivative(loc.shape[0], _p(loc), _p(out))
+    return out
+
+
+def subdivide(coords):
+    coords = f64(coords)
+    out = np.zeros((27, 3))
+    lib().or_subdivide_hex8(_p(coords), _p(out))
+    return out
+
+
+def scs_areav(coords):
+    coords = f64(coords)
+    out = np.zeros((12, 3))
+    lib().or_hex_scs_areav(_p(coords), _p(out))
+    return out
+
+
+def scs_grad_op(coords, shifted=False):
+    coords = f64(coords)
+    dndx = np.zeros((12, 8, 3))
+    deriv = np.zeros((12, 8, 3))
+    lib().or_hexscs_grad_op(_p(coords), int(shifted), _p(dndx), _p(deriv))
+    return dndx, deriv
+
+
+def scv_grad_op(coords, shifted=False):
+    coords = f64(coords)
+    dndx = np.zeros((8, 8, 3))
+    deriv = np.zeros((8, 8, 3))
+    lib().or_hexscv_grad_op(_p(coords), int(shifted), _p(dndx), _p(deriv))
+    return dndx, deriv
+
+
+def grandy(coords):
+    coords = f64(coords)
+    return lib().or_hex_volume_grandy(_p(coords))
+
+
+def scv_volume(coords):
+    coords = f64(coords)
+    out = np.zeros(8)
+    lib().or_hexscv_volume(_p(coords), _p(out))
+    return out
+
+
+# ---- element kernels ----
+def scalar_diff_elem(coords, q, k, shifted=False):
+    lhs = np.zeros((8, 8))
+    rhs = np.zeros(8)
+    lib().or_scalar_diff_elem(_p(f64(coords)), _p(f64(q)), _p(f64(k)), int(shifted), _p(lhs), _p(rhs))
+    return lhs, rhs
+
+
+def continuity_adv_elem(coords, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False,
+                        reduced_sens=False):
+    lhs = np.zeros((8, 8))
+    rhs = np.zeros(8)
+    lib().or_continuity_adv_elem(_p(f64(coords)), _p(f64(rho)), _p(f64(p)), _p(f64(u)), _p(f64(gpdx)),
+                                 C.c_double(tau), int(shift_mdot), int(shift_poisson),
+                                 int(reduced_sens), _p(lhs), _p(rhs))
+    return lhs, rhs
+
+
+def mdot_elem(coords, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False):
+    out = np.zeros(12)
+    lib().or_mdot_elem(_p(f64(coords)), _p(f64(rho)), _p(f64(p)), _p(f64(u)), _p(f64(gpdx)),
+                       C.c_double(tau), int(shift_mdot), int(shift_poisson), _p(out))
+    return out
+
+
+def mdot_rhou_elem(coords, rho, u):
+    out = np.zeros(12)
+    lib().or_mdot_rhou_elem(_p(f64(coords)), _p(f64(rho)), _p(f64(u)), _p(out))
+    return out
+
+
+def momentum_advdiff_elem(coords, u, mu, mdot, include_divu=0.0, skew=False, shifted=False):
+    lhs = np.zeros((24, 24))
+    rhs = np.zeros(24)
+    lib().or_momentum_advdiff_elem(_p(f64(coords)), _p(f64(u)), _p(f64(mu)), _p(f64(mdot)),
+                                   C.c_double(include_divu), int(skew), int(shifted), _p(lhs), _p(rhs))
+    return lhs, rhs
+
+
+# ---- graph / scatter / assembly ----
+class Graph:
+    def __init__(self, row_ptr, col_idx, node_lid, num_dof):
+        self.row_ptr = row_ptr
+        self.col_idx = col_idx
+        self.node_lid = node_lid
+        self.num_dof = num_dof
+        self.nrows = row_ptr.shape[0] - 1
+        self.nnz = int(row_ptr[-1])
+
+
+def graph_build(nnodes, conn, num_dof=1, node_ids=None):
+    conn = i32(conn)
+    nelem = conn.shape[0]
+    if node_ids is None:
+        node_ids = np.arange(1, nnodes + 1, dtype=np.int64)
+    node_ids = i64(node_ids)
+    L = lib()
+    nnz = L.or_graph_build(nnodes, _p(node_ids, np.int64), nelem, 8, _p(conn, np.int32), num_dof,
+                           None, None, None)
+    rp = np.zeros(nnodes * num_dof + 1, dtype=np.int64)
+    ci = np.zeros(nnz, dtype=np.int32)
+    lid = np.zeros(nnodes, dtype=np.int32)
+    L.or_graph_build(nnodes, _p(node_ids, np.int64), nelem, 8, _p(conn, np.int32), num_dof,
+                     _p(rp, np.int64), _p(ci, np.int32), _p(lid, np.int32))
+    return Graph(rp, ci, lid, num_dof)
+
+
+def sum_into(g, values, rhs, entities, lhs_e, rhs_e):
+    entities = i32(entities)
+    lib().or_sum_into(g.num_dof, _p(g.row_ptr, np.int64), _p(g.col_idx, np.int32), _p(values), _p(rhs),
+                      _p(g.node_lid, np.int32), entities.shape[0], _p(entities, np.int32),
+                      _p(f64(lhs_e)), _p(f64(rhs_e)))
+
+
+def apply_dirichlet(g, values, rhs, bc_nodes, solution, bcvalues, begin=0, end=None):
+    if end is None:
+        end = g.num_dof
+    bc_nodes = i32(bc_nodes)
+    lib().or_apply_dirichlet(g.num_dof, g.nrows, _p(g.row_ptr, np.int64), _p(g.col_idx, np.int32),
+                             _p(values), _p(rhs), _p(g.node_lid, np.int32), bc_nodes.shape[0],
+                             _p(bc_nodes, np.int32), _p(f64(solution)), _p(f64(bcvalues)), begin, end)
+
+
+def assemble_scalar_diff(g, conn, coords, q, k, shifted=False):
+    conn = i32(conn)
+    values = np.zeros(g.nnz)
+    rhs = np.zeros(g.nrows)
+    lib().or_assemble_scalar_diff(conn.shape[0], _p(conn, np.int32), _p(f64(coords)), _p(f64(q)),
+                                  _p(f64(k)), int(shifted), _p(g.row_ptr, np.int64),
+                                  _p(g.col_idx, np.int32), _p(g.node_lid, np.int32), _p(values), _p(rhs))
+    return values, rhs
+
+
+def compute_mdot(conn, coords, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False):
+    conn = i32(conn)
+    mdot = np.zeros((conn.shape[0], 12))
+    lib().or_compute_mdot(conn.shape[0], _p(conn, np.int32), _p(f64(coords)), _p(f64(rho)), _p(f64(p)),
+                          _p(f64(u)), _p(f64(gpdx)), C.c_double(tau), int(shift_mdot),
+                          int(shift_poisson), _p(mdot))
+    return mdot
+
+
+def assemble_continuity(g, conn, coords, rho, p, u, gpdx, tau=1.0, shift_mdot=False,
+                        shift_poisson=False, reduced_sens=False):
+    conn = i32(conn)
+    values = np.zeros(g.nnz)
+    rhs = np.zeros(g.nrows)
+    lib().or_assemble_continuity(conn.shape[0], _p(conn, np.int32), _p(f64(coords)), _p(f64(rho)),
+                                 _p(f64(p)), _p(f64(u)), _p(f64(gpdx)), C.c_double(tau),
+                                 int(shift_mdot), int(shift_poisson), int(reduced_sens),
+                                 _p(g.row_ptr, np.int64), _p(g.col_idx, np.int32),
+                                 _p(g.node_lid, np.int32), _p(values), _p(rhs))
+    return values, rhs
+
+
+def assemble_momentum(g, conn, coords, u, mu, mdot, include_divu=0.0, skew=False, shifted=False):
+    conn = i32(conn)
+    values = np.zeros(g.nnz)
+    rhs = np.zeros(g.nrows)
+    lib().or_assemble_momentum(conn.shape[0], _p(conn, np.int32), _p(f64(coords)), _p(f64(u)),
+                               _p(f64(mu)), _p(f64(mdot)), C.c_double(include_divu), int(skew),
+                               int(shifted), _p(g.row_ptr, np.int64), _p(g.col_idx, np.int32),
+                               _p(g.node_lid, np.int32), _p(values), _p(rhs))
+    return values, rhs
+
+
+# ---- solvers ----
+def spmv(rp, ci, v, x):
+    x = f64(x)
+    y = np.zeros(rp.shape[0] - 1)
+    lib().or_spmv(rp.shape[0] - 1, _p(rp, np.int64), _p(ci, np.int32), _p(v), _p(x), _p(y))
+    return y
+
+
+def sgs_apply(rp, ci, v, r):
+    z = np.zeros(rp.shape[0] - 1)
+    lib().or_sgs_apply(rp.shape[0] - 1, _p(rp, np.int64), _p(ci, np.int32), _p(v), _p(f64(r)), _p(z))
+    return z
+
+
+def mcsgs_apply(rp, ci, v, color_ptr, color_rows, r):
+    z = np.zeros(rp.shape[0] - 1)
+    color_ptr = i64(color_ptr)
+    color_rows = i32(color_rows)
+    lib().or_mcsgs_apply(rp.shape[0] - 1, _p(rp, np.int64), _p(ci, np.int32), _p(v),
+                         color_ptr.shape[0] - 1, _p(color_ptr, np.int64), _p(color_rows, np.int32),
+                         _p(f64(r)), _p(z))
+    return z
+
+
+def jacobi_apply(rp, ci, v, r):
+    z = np.zeros(rp.shape[0] - 1)
+    lib().or_jacobi_apply(rp.shape[0] - 1, _p(rp, np.int64), _p(ci, np.int32), _p(v), _p(f64(r)), _p(z))
+    return z
+
+
+PRECOND = {"none": 0, "jacobi": 1, "sgs": 2, "mcsgs": 3}
+
+
+def _colors(color_ptr, color_rows):
+    if color_ptr is None:
+        return 0, None, None
+    cp = i64(color_ptr)
+    cr = i32(color_rows)
+    return cp.shape[0] - 1, _p(cp, np.int64), _p(cr, np.int32)
+
+
+def gmres(rp, ci, v, b, x0=None, kspace=50, maxit=1000, tol=1e-8, precond="sgs", color_ptr=None,
+          color_rows=None):
+    n = rp.shape[0] - 1
+    x = np.zeros(n) if x0 is None else f64(x0).copy()
+    res = C.c_double(0)
+    nc, cp, cr = _colors(color_ptr, color_rows)
+    it = lib().or_gmres(n, _p(rp, np.int64), _p(ci, np.int32), _p(v), _p(f64(b)), _p(x), kspace,
+                        maxit, tol, PRECOND[precond], nc, cp, cr, C.cast(C.pointer(res), _d))
+    return x, it, res.value
+
+
+def cg(rp, ci, v, b, x0=None, maxit=1000, tol=1e-8, precond="sgs", color_ptr=None, color_rows=None):
+    n = rp.shape[0] - 1
+    x = np.zeros(n) if x0 is None else f64(x0).copy()
+    res = C.c_double(0)
+    nc, cp, cr = _colors(color_ptr, color_rows)
+    it = lib().or_cg(n, _p(rp, np.int64), _p(ci, np.int32), _p(v), _p(f64(b)), _p(x), maxit,
+                     tol, PRECOND[precond], nc, cp, cr, C.cast(C.pointer(res), _d))
+    return x, it, res.value
+
+
+def greedy_color(rp, ci):
+    n = rp.shape[0] - 1
+    col = np.zeros(n, dtype=np.int32)
+    nc = lib().or_greedy_color(n, _p(rp, np.int64), _p(ci, np.int32), _p(col, np.int32))
+    return nc, col
+
+
+def wtime():
+    return lib().or_wtime()

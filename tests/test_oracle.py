@@ -1,0 +1,351 @@
+"""Pin the CPU oracle: reference golden vectors, closed forms and invariants.
+
+Each check is chosen so that a plausible mistake (a dropped term, a wrong sign
+or index, a transposed operand) fails it.  CPU only.
+"""
+import numpy as np
+import pytest
+
+from tests.fixtures import (UNIT_HEX, lattice_1x1xN, trig_dpdx, trig_pressure, trig_temperature,
+                            trig_velocity)
+
+# ----------------------------------------------------------------- golden vectors
+
+
+def test_scalar_diff_gold(oracle, golds):
+    g = golds["scalar_diff"]
+    lhs, rhs = oracle.scalar_diff_elem(UNIT_HEX, trig_temperature(UNIT_HEX), np.ones(8))
+    np.testing.assert_allclose(lhs, np.array(g["lhs"]).reshape(8, 8), rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(rhs, g["rhs"], rtol=0, atol=g["tol"])
+
+
+@pytest.mark.parametrize("name,reduced", [("continuity_default", False), ("continuity_reduced_sens", True)])
+def test_continuity_gold(oracle, golds, name, reduced):
+    g = golds[name]
+    X = UNIT_HEX
+    lhs, rhs = oracle.continuity_adv_elem(X, np.ones(8), trig_pressure(X), trig_velocity(X), trig_dpdx(X),
+                                          tau=1.0, reduced_sens=reduced)
+    np.testing.assert_allclose(lhs, np.array(g["lhs"]).reshape(8, 8), rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(rhs, g["rhs"], rtol=0, atol=g["tol"])
+
+
+def test_continuity_shift_poisson_gold(oracle, golds):
+    # shift_cvfem_poisson + reduced sensitivities use the same golds
+    # (unit_tests/kernels/UnitTestContinuityAdvElem.C:150-190)
+    g = golds["continuity_reduced_sens"]
+    X = UNIT_HEX
+    lhs, rhs = oracle.continuity_adv_elem(X, np.ones(8), trig_pressure(X), trig_velocity(X), trig_dpdx(X),
+                                          tau=1.0, shift_poisson=True, reduced_sens=True)
+    np.testing.assert_allclose(lhs, np.array(g["lhs"]).reshape(8, 8), rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(rhs, g["rhs"], rtol=0, atol=g["tol"])
+
+
+def test_momentum_gold(oracle, golds):
+    g = golds["momentum_advdiff"]
+    X = UNIT_HEX
+    u = trig_velocity(X)
+    mdot = oracle.mdot_rhou_elem(X, np.ones(8), u)
+    lhs, rhs = oracle.momentum_advdiff_elem(X, u, np.full(8, g["mu"]), mdot, include_divu=0.0)
+    np.testing.assert_allclose(lhs, np.array(g["lhs"]).reshape(24, 24), rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(rhs, g["rhs"], rtol=0, atol=g["tol"])
+
+
+@pytest.mark.parametrize("shifted,key", [(False, "hex8_scv_grad_op"), (True, "hex8_scv_shifted_grad_op")])
+def test_scv_grad_op_gold(oracle, golds, shifted, key):
+    X = np.array(golds["perturbed_hex8_coords"]["values"]).reshape(8, 3)
+    dndx, _ = oracle.scv_grad_op(X, shifted)
+    np.testing.assert_allclose(dndx.ravel(), golds[key]["values"], rtol=0, atol=golds[key]["tol"])
+
+
+# ----------------------------------------------------------------- closed forms
+
+def test_shape_functions_closed_form(oracle):
+    for shifted in (False, True):
+        loc = oracle.scs_intgloc(shifted)
+        N = oracle.shape_fcn(loc)
+        np.testing.assert_allclose(N.sum(axis=1), 1.0, atol=1e-15)  # partition of unity
+        signs = UNIT_HEX * 2 - 1
+        expect = np.prod(0.5 + signs[None, :, :] * loc[:, None, :], axis=2)
+        np.testing.assert_allclose(N, expect, atol=1e-15)
+        D = oracle.derivative(loc)
+        np.testing.assert_allclose(D.sum(axis=1), 0.0, atol=1e-15)
+        for d in range(3):
+            other = [e for e in range(3) if e != d]
+            ex = signs[None, :, d] * np.prod(0.5 + signs[None, :, other] * loc[:, None, other], axis=2)
+            np.testing.assert_allclose(D[:, :, d], ex, atol=1e-15)
+
+
+def test_scs_layout_matches_lrscv(oracle):
+    # each scs ip sits between its left/right nodes: zero along the edge, +-1/4 across
+    lr = oracle.lrscv()
+    loc = oracle.scs_intgloc(False)
+    signs = UNIT_HEX * 2 - 1
+    for ip, (a, b) in enumerate(lr):
+        edge = np.nonzero(signs[a] != signs[b])[0]
+        assert len(edge) == 1
+        assert loc[ip, edge[0]] == 0.0
+        for d in range(3):
+            if d != edge[0]:
+                assert loc[ip, d] == 0.25 * signs[a, d]
+
+
+def test_scs_areas_unit_cube(oracle):
+    A = oracle.scs_areav(UNIT_HEX)
+    lr = oracle.lrscv()
+    for ip, (a, b) in enumerate(lr):
+        expect = 0.25 * (UNIT_HEX[b] - UNIT_HEX[a])  # normal points from left to right node, |A| = 1/4
+        np.testing.assert_allclose(A[ip], expect, atol=1e-15)
+
+
+def test_scs_areas_affine_covariance(oracle, golds):
+    X = np.array(golds["perturbed_hex8_coords"]["values"]).reshape(8, 3)
+    Q = np.array([[1.3, 0.2, -0.1], [0.05, 0.9, 0.3], [-0.2, 0.1, 1.1]])
+    t = np.array([0.3, -1.0, 2.0])
+    A = oracle.scs_areav(X)
+    A2 = oracle.scs_areav(X @ Q.T + t)
+    cof = np.linalg.det(Q) * np.linalg.inv(Q).T
+    np.testing.assert_allclose(A2, A @ cof.T, atol=1e-13)
+
+
+def test_grad_op_exact_on_linear_fields(oracle, golds):
+    X = np.array(golds["perturbed_hex8_coords"]["values"]).reshape(8, 3)
+    G = np.array([0.7, -1.3, 2.1])
+    f = X @ G + 0.4
+    for shifted in (False, True):
+        dndx, _ = oracle.scs_grad_op(X, shifted)
+        np.testing.assert_allclose(np.einsum("icd,c->id", dndx, f), np.tile(G, (12, 1)), atol=1e-12)
+        np.testing.assert_allclose(np.einsum("icd,ce->ide", dndx, X), np.tile(np.eye(3), (12, 1, 1)), atol=1e-12)
+
+
+def test_grandy_volumes(oracle):
+    np.testing.assert_allclose(oracle.scv_volume(UNIT_HEX), 0.125, atol=np.finfo(float).eps)
+    Q = np.array([[2.1, 0.3, -0.2], [0.1, 2.7, 0.4], [-0.3, 0.2, 3.2]])
+    np.testing.assert_allclose(oracle.grandy(UNIT_HEX @ Q.T), np.linalg.det(Q), rtol=1e-13)
+    np.testing.assert_allclose(oracle.scv_volume(UNIT_HEX @ Q.T).sum(), np.linalg.det(Q), rtol=1e-13)
+
+
+def test_diffusion_row_and_column_sums(oracle, golds):
+    X = np.array(golds["perturbed_hex8_coords"]["values"]).reshape(8, 3)
+    k = np.linspace(0.5, 2.0, 8)
+    lhs, rhs = oracle.scalar_diff_elem(X, np.zeros(8), k)
+    np.testing.assert_allclose(lhs.sum(axis=1), 0.0, atol=1e-14)  # constants in the kernel
+    np.testing.assert_allclose(lhs.sum(axis=0), 0.0, atol=1e-14)  # conservation: fluxes cancel
+    q = np.arange(8.0)
+    _, rhs = oracle.scalar_diff_elem(X, q, k)
+    np.testing.assert_allclose(rhs, -lhs @ q, atol=1e-14)  # residual form rhs = -A q
+
+
+def test_mdot_matches_continuity_rhs(oracle, golds):
+    X = np.array(golds["perturbed_hex8_coords"]["values"]).reshape(8, 3)
+    rng = np.random.default_rng(3)
+    rho, p, u, gp = rng.uniform(0.5, 1.5, 8), rng.normal(size=8), rng.normal(size=(8, 3)), rng.normal(size=(8, 3))
+    tau = 0.7
+    md = oracle.mdot_elem(X, rho, p, u, gp, tau)
+    _, rhs = oracle.continuity_adv_elem(X, rho, p, u, gp, tau)
+    lr = oracle.lrscv()
+    expect = np.zeros(8)
+    for ip, (a, b) in enumerate(lr):
+        expect[a] -= md[ip] / tau
+        expect[b] += md[ip] / tau
+    np.testing.assert_allclose(rhs, expect, atol=1e-13)
+    # constant velocity, no pressure terms: mdot = rho u . A
+    md2 = oracle.mdot_elem(X, np.ones(8), np.zeros(8), np.tile([1.0, 2.0, 3.0], (8, 1)), np.zeros((8, 3)), tau)
+    np.testing.assert_allclose(md2, oracle.scs_areav(X) @ np.array([1.0, 2.0, 3.0]), atol=1e-13)
+
+
+def test_momentum_structure(oracle, golds):
+    X = np.array(golds["perturbed_hex8_coords"]["values"]).reshape(8, 3)
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=(8, 3))
+    mu = rng.uniform(0.1, 0.2, 8)
+    md = rng.normal(size=12)
+    lhs, rhs = oracle.momentum_advdiff_elem(X, u, mu, md, include_divu=0.0)
+    # rhs is the residual of the linear operator for the advection/diffusion terms
+    np.testing.assert_allclose(rhs, -lhs @ u.ravel(), atol=1e-13)
+    # diffusion-only: rigid translation produces no stress
+    lhs0, _ = oracle.momentum_advdiff_elem(X, u, mu, np.zeros(12))
+    np.testing.assert_allclose(lhs0 @ np.tile([1.0, -2.0, 0.5], 8), 0.0, atol=1e-13)
+
+
+# ----------------------------------------------------------------- graph + scatter
+
+def test_tpetra_1x1x2_graph_and_values(oracle, golds):
+    gold = golds["tpetra_1x1x2"]
+    coords, conn = lattice_1x1xN(2)
+    g = oracle.graph_build(coords.shape[0], conn, 1)
+    np.testing.assert_array_equal(np.diff(g.row_ptr), gold["row_lengths"])
+    vals = np.zeros(g.nnz)
+    rhs = np.zeros(g.nrows)
+    ev = np.array(gold["elem_vals"]).reshape(8, 8)
+    for e in range(conn.shape[0]):
+        oracle.sum_into(g, vals, rhs, conn[e], ev, np.zeros(8))
+    L = np.array(gold["lhs_vals"]).reshape(12, 12)
+    dense = np.zeros((12, 12))
+    for r in range(12):
+        for q in range(g.row_ptr[r], g.row_ptr[r + 1]):
+            dense[r, g.col_idx[q]] = vals[q]
+    np.testing.assert_allclose(dense, L, atol=gold["tol"])
+    # the structural pattern is exactly the nonzero pattern of the gold
+    pattern = np.zeros((12, 12), dtype=bool)
+    for r in range(12):
+        pattern[r, g.col_idx[g.row_ptr[r]:g.row_ptr[r + 1]]] = True
+    np.testing.assert_array_equal(pattern, L != 0)
+
+
+def test_graph_lattice_closed_form(oracle):
+    import nalu_amd as na
+    for n in (1, 2, 3):
+        coords, conn = na.lattice(n)
+        for nd in (1, 3):
+            g = oracle.graph_build(coords.shape[0], conn, nd)
+            assert g.nnz == (3 * n + 1) ** 3 * nd * nd
+            n1 = n + 1
+            for node in range(coords.shape[0]):
+                i, j, k = node % n1, (node // n1) % n1, node // (n1 * n1)
+                nbr = sorted(a + n1 * (b + n1 * c) for c in range(k - 1, k + 2) for b in range(j - 1, j + 2)
+                             for a in range(i - 1, i + 2) if 0 <= a < n1 and 0 <= b < n1 and 0 <= c < n1)
+                cols = [nd * m + e for m in nbr for e in range(nd)]
+                for d in range(nd):
+                    r = nd * node + d
+                    np.testing.assert_array_equal(g.col_idx[g.row_ptr[r]:g.row_ptr[r + 1]], cols)
+
+
+def test_graph_nonidentity_ids(oracle):
+    # rows follow the sorted global id, not the storage order
+    coords, conn = lattice_1x1xN(1)
+    ids = np.array([5, 3, 8, 1, 7, 2, 6, 4], dtype=np.int64)
+    g = oracle.graph_build(8, conn, 2, node_ids=ids)
+    np.testing.assert_array_equal(g.node_lid, 2 * (np.argsort(np.argsort(ids))))
+
+
+def test_sum_into_block_order(oracle):
+    # numDof = 3: entry (row node a, dof i; column node c, dof j) lands at lhs[3a+i, 3c+j]
+    coords, conn = lattice_1x1xN(1)
+    g = oracle.graph_build(8, conn, 3)
+    R = 24
+    lhs = np.arange(R * R, dtype=float).reshape(R, R)
+    rhs_e = np.arange(R, dtype=float)
+    vals = np.zeros(g.nnz)
+    rhs = np.zeros(g.nrows)
+    oracle.sum_into(g, vals, rhs, conn[0], lhs, rhs_e)
+    dense = np.zeros((R, R))
+    for r in range(R):
+        for q in range(g.row_ptr[r], g.row_ptr[r + 1]):
+            dense[r, g.col_idx[q]] = vals[q]
+    perm = np.array([3 * conn[0][a] + i for a in range(8) for i in range(3)])
+    expect = np.zeros((R, R))
+    expect[np.ix_(perm, perm)] = lhs
+    np.testing.assert_array_equal(dense, expect)
+    exr = np.zeros(R)
+    exr[perm] = rhs_e
+    np.testing.assert_array_equal(rhs, exr)
+
+
+def test_dirichlet_closed_form(oracle):
+    coords, conn = lattice_1x1xN(2)
+    g = oracle.graph_build(12, conn, 1)
+    vals = np.random.default_rng(0).normal(size=g.nnz)
+    rhs = np.ones(g.nrows)
+    sol = np.linspace(0, 1, 12)
+    bcv = np.linspace(2, 3, 12)
+    oracle.apply_dirichlet(g, vals, rhs, [0, 5], sol, bcv)
+    for r in (0, 5):
+        seg = slice(g.row_ptr[r], g.row_ptr[r + 1])
+        np.testing.assert_array_equal(vals[seg], (g.col_idx[seg] == r).astype(float))
+        assert rhs[r] == bcv[r] - sol[r]
+    assert rhs[1] == 1.0
+
+
+# ----------------------------------------------------------------- relaxation + Krylov
+
+def _heat_system(oracle, n=3, seed=7):
+    import nalu_amd as na
+    from nalu_amd.problems import heat_problem
+    P = heat_problem(n, seed=seed)
+    g = oracle.graph_build(P["coords"].shape[0], P["conn"])
+    v, r = oracle.assemble_scalar_diff(g, P["conn"], P["coords"], P["T"], P["k"])
+    oracle.apply_dirichlet(g, v, r, P["bc_nodes"], P["T"], P["bc_values"])
+    return P, g, v, r
+
+
+def _dense(g, v):
+    A = np.zeros((g.nrows, g.nrows))
+    for i in range(g.nrows):
+        A[i, g.col_idx[g.row_ptr[i]:g.row_ptr[i + 1]]] = v[g.row_ptr[i]:g.row_ptr[i + 1]]
+    return A
+
+
+def test_sgs_operator_identity(oracle):
+    _, g, v, _ = _heat_system(oracle, 2)
+    A = _dense(g, v)
+    D = np.diag(np.diag(A))
+    M = (np.tril(A)) @ np.linalg.inv(D) @ (np.triu(A))  # (D+L) D^-1 (D+U)
+    Minv = np.stack([oracle.sgs_apply(g.row_ptr, g.col_idx, v, e) for e in np.eye(g.nrows)], axis=1)
+    np.testing.assert_allclose(Minv, np.linalg.inv(M), atol=1e-12)
+    np.testing.assert_allclose(oracle.jacobi_apply(g.row_ptr, g.col_idx, v, np.ones(g.nrows)), 1.0 / np.diag(A))
+
+
+def test_multicolor_sgs_is_sgs_in_color_order(oracle):
+    _, g, v, _ = _heat_system(oracle, 3)
+    nc, col = oracle.greedy_color(g.row_ptr, g.col_idx)
+    assert nc == 8  # 27-point lattice stencil, natural order
+    order = np.argsort(col, kind="stable")
+    cp = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=nc))])
+    r = np.random.default_rng(1).normal(size=g.nrows)
+    z = oracle.mcsgs_apply(g.row_ptr, g.col_idx, v, cp, order, r)
+    A = _dense(g, v)
+    Ap = A[np.ix_(order, order)]
+    D = np.diag(np.diag(Ap))
+    zp = np.linalg.solve(np.tril(Ap) @ np.linalg.inv(D) @ np.triu(Ap), r[order])
+    np.testing.assert_allclose(z[order], zp, atol=1e-12)
+    # rows of one color are independent: no coupling inside a color
+    for c in range(nc):
+        rows = set(order[cp[c]:cp[c + 1]])
+        for i in rows:
+            nb = set(g.col_idx[g.row_ptr[i]:g.row_ptr[i + 1]]) - {i}
+            assert not (nb & rows)
+
+
+@pytest.mark.parametrize("precond", ["none", "jacobi", "sgs", "mcsgs"])
+def test_gmres_matches_dense_solve(oracle, precond):
+    _, g, v, r = _heat_system(oracle, 3)
+    cp = order = None
+    if precond == "mcsgs":
+        nc, col = oracle.greedy_color(g.row_ptr, g.col_idx)
+        order = np.argsort(col, kind="stable")
+        cp = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=nc))])
+    x, it, res = oracle.gmres(g.row_ptr, g.col_idx, v, r, kspace=10, maxit=500, tol=1e-10, precond=precond,
+                              color_ptr=cp, color_rows=order)
+    xd = np.linalg.solve(_dense(g, v), r)
+    assert res <= 1e-10
+    np.testing.assert_allclose(x, xd, atol=1e-8 * np.abs(xd).max())
+    assert 0 < it < 500
+
+
+def test_cg_matches_dense_solve(oracle):
+    # CG on a symmetric positive-definite system: the undistorted lattice Laplacian
+    # with both-side Dirichlet rows removed symmetrically
+    _, g, v, r = _heat_system(oracle, 3)
+    A = _dense(g, v)
+    S = 0.5 * (A + A.T) + np.eye(g.nrows)  # SPD test matrix on the same graph
+    vs = np.concatenate([S[i, g.col_idx[g.row_ptr[i]:g.row_ptr[i + 1]]] for i in range(g.nrows)])
+    b = np.random.default_rng(2).normal(size=g.nrows)
+    for pc in ("none", "jacobi", "sgs"):
+        x, it, res = oracle.cg(g.row_ptr, g.col_idx, vs, b, maxit=500, tol=1e-10, precond=pc)
+        assert res <= 1e-10
+        np.testing.assert_allclose(x, np.linalg.solve(S, b), atol=1e-8)
+
+
+def test_heat_solution_is_linear_on_uniform_mesh(oracle):
+    # uniform lattice, constant conductivity, T=0/1 at x=0/1: the exact discrete solution is T = x
+    import nalu_amd as na
+    n = 4
+    coords, conn = na.lattice(n)
+    g = oracle.graph_build(coords.shape[0], conn)
+    v, r = oracle.assemble_scalar_diff(g, conn, coords, np.zeros(len(coords)), np.ones(len(coords)))
+    i = np.arange(len(coords)) % (n + 1)
+    bc = np.nonzero((i == 0) | (i == n))[0]
+    bcv = (i == n).astype(float)
+    oracle.apply_dirichlet(g, v, r, bc, np.zeros(len(coords)), bcv)
+    x, it, res = oracle.gmres(g.row_ptr, g.col_idx, v, r, kspace=50, maxit=500, tol=1e-12, precond="sgs")
+    np.testing.assert_allclose(x, coords[:, 0], atol=1e-10)
