@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark: low-Mach Hex8 CVFEM assembly + pressure-Poisson solve on one MI355X.
+
+Workload (default): the 128^3 low-Mach case -- unit cube, interior nodes
+jittered +-0.1h, Taylor-Green velocity + N(0, 0.05^2) noise, density 1,
+viscosity 1e-3, dt = gamma1 = 1.  One step is the hot path once:
+
+  1. mdot at the 12 scs of every element (ComputeMdotElemAlgorithm)
+  2. continuity system: zeroSystem, ContinuityAdvElemKernel assembly, loadComplete
+  3. momentum system (3 dof/node, 27-point block rows): zeroSystem, MomentumAdvDiffElemKernel
+  4. pressure-Poisson solve: CG + multicolor SGS to 1e-8 FP64 true relative residual
+     on the assembled continuity matrix with its zero-mean-projected rhs.
+
+`value` is the assembly throughput (elements of steps 1-3 per second, Melem/s);
+the pressure solve time, iterations and residual sit beside it.  Inputs are
+resident in HBM before the timed region.  The dominant kernel's roofline and a
+CPU baseline (the FP64 oracle, one core, on a bounded sample) are reported too.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Melem/s assembly + pressure solve s to 1e-8 resid, FP64 plain vs mixed+learned"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128, help="elements per direction")
+    ap.add_argument("--cpu-n", type=int, default=48, help="mesh size of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(N, E, P, rows, nnz1, nnz3):
+    w, i = 8, 4
+    return {
+        "mdot": N * 11 * w + E * 32 + E * 12 * w,
+        "continuity_assembly": N * 11 * w + E * 32 + P * w + N * w,
+        "momentum_assembly": N * 7 * w + E * (32 + 12 * w) + 9 * P * w + 3 * N * w,
+        "spmv": nnz1 * (w + i) + (rows + 1) * i + 2 * rows * w,
+        "sgs_apply": 2 * (nnz1 * (w + i) + (rows + 1) * i) + 4 * rows * w,
+    }
+
+
+def cpu_baseline(n_cpu):
+    """The FP64 oracle, single core, on an n_cpu^3 mesh of the same workload."""
+    from nalu_amd.problems import lowmach_problem
+    from oracle import pyoracle as O
+    P = lowmach_problem(n_cpu, seed=4321)
+    nn, E = P["coords"].shape[0], P["conn"].shape[0]
+    g1 = O.graph_build(nn, P["conn"], 1)
+    g3 = O.graph_build(nn, P["conn"], 3)
+    t0 = O.wtime()
+    md = O.compute_mdot(P["conn"], P["coords"], P["rho"], P["p"], P["u"], P["gpdx"], P["tau"])
+    v, r = O.assemble_continuity(g1, P["conn"], P["coords"], P["rho"], P["p"], P["u"], P["gpdx"], P["tau"])
+    O.assemble_momentum(g3, P["conn"], P["coords"], P["u"], P["mu"], md)
+    t_asm = O.wtime() - t0
+    b = r - r.mean()
+    t0 = O.wtime()
+    _, it, res = O.cg(g1.row_ptr, g1.col_idx, v, b, maxit=20000, tol=1e-8, precond="sgs")
+    t_solve = O.wtime() - t0
+    return {"value": E / t_asm / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+            "sample": "%d^3 low-Mach mesh (%d elements), same fields: mdot + continuity + momentum assembly "
+                      "%.2f s; CG + sequential SGS pressure solve to 1e-8: %d iters, %.2f s, resid %.1e"
+                      % (n_cpu, E, t_asm, it, t_solve, res),
+            "pressure_solve_s": t_solve, "pressure_iters": it}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the domain-decomposition design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import nalu_amd as na
+    from nalu_amd.problems import lowmach_problem
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    devc = torch.device("cuda:0")
+    n = args.n
+    print("bench: generating %d^3 low-Mach problem" % n, file=sys.stderr, flush=True)
+    P = lowmach_problem(n, seed=4321)
+    N, E = P["coords"].shape[0], P["conn"].shape[0]
+    mesh = na.Mesh(P["coords"], P["conn"])
+    cont = na.LinearSystem(1, na.SolverConfig("cg", "sgs", tolerance=1e-8, max_iterations=5000))
+    mom = na.LinearSystem(3, na.SolverConfig("gmres", "sgs"))
+    for ls in (cont, mom):
+        ls.buildElemToNodeGraph(mesh)
+        ls.finalizeLinearSystem()
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
+    rho, p, u, gp, mu = t(P["rho"]), t(P["p"]), t(P["u"]), t(P["gpdx"]), t(P["mu"])
+    mdot = torch.zeros((E, 12), dtype=torch.float64, device=devc)
+    rows, nnz1 = cont.sizes()
+    _, nnz3 = mom.sizes()
+    Pn = nnz1  # scalar node-pair nonzeros
+    x = torch.zeros(rows, dtype=torch.float64, device=devc)
+    b = torch.randn(rows, dtype=torch.float64, device=devc)
+    # the library runs on torch's current stream, so torch events bracket its kernels
+    stream = torch.cuda.current_stream().cuda_stream
+    for ls in (cont, mom):
+        ls.set_stream(stream)
+
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=devc)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+
+    def step(timers=None):
+        e = [ev() for _ in range(5)] if timers is not None else None
+        if e: e[0].record()
+        na.compute_mdot(mesh, rho, p, u, gp, mdot, tau=P["tau"], stream=stream)
+        if e: e[1].record()
+        cont.zeroSystem()
+        na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode="gather")
+        cont.loadComplete()
+        if e: e[2].record()
+        mom.zeroSystem()
+        na.assemble_momentum(mom, mesh, u, mu, mdot, mode="atomic")
+        mom.loadComplete()
+        if e: e[3].record()
+        # zero-mean projection of the continuity rhs (singular Neumann pressure system)
+        cont.removeRhsMean()
+        st = cont.solve(x)
+        it = cont.linearSolveIterations()
+        rel = cont.linearResidual() / max(cont.nonLinearResidual(), 1e-300)
+        if e: e[4].record()
+        if timers is not None:
+            timers.append((e, it, rel, st))
+        return it, rel, st
+
+    print("bench: warmup %d" % args.warmup, file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timers = []
+    print("bench: timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(timers)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    ph = {"mdot": [], "continuity_assembly": [], "momentum_assembly": [], "pressure_solve": []}
+    for (e, it, rel, st) in timers:
+        ph["mdot"].append(e[0].elapsed_time(e[1]) * 1e-3)
+        ph["continuity_assembly"].append(e[1].elapsed_time(e[2]) * 1e-3)
+        ph["momentum_assembly"].append(e[2].elapsed_time(e[3]) * 1e-3)
+        ph["pressure_solve"].append(e[3].elapsed_time(e[4]) * 1e-3)
+    med = {k: float(np.median(v)) for k, v in ph.items()}
+    iters = timers[-1][1]
+    resid = timers[-1][2]
+    t_asm = med["mdot"] + med["continuity_assembly"] + med["momentum_assembly"]
+
+    # kernel-level timing of the solver building blocks, HIP events on the stream they run on
+    def time_op(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a, bb = ev(), ev()
+        a.record()
+        for _ in range(reps):
+            fn()
+        bb.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(bb) * 1e-3 / reps
+
+    y = torch.zeros_like(x)
+    t_spmv = time_op(lambda: cont.spmv(b, y), 20)
+    t_sgs = time_op(lambda: cont.precond_apply(b, y), 10)
+    cp, _ = cont.coloring()
+    ncol = len(cp) - 1
+
+    byt = algorithmic_bytes(N, E, Pn, rows, nnz1, nnz3)
+    per_step = {
+        "mdot": (med["mdot"], byt["mdot"], 1),
+        "continuity_assembly": (med["continuity_assembly"], byt["continuity_assembly"], 1),
+        "momentum_assembly": (med["momentum_assembly"], byt["momentum_assembly"], 8),
+        "sgs_apply": (t_sgs * (iters + 1), byt["sgs_apply"], 2 * ncol),
+        "spmv": (t_spmv * (iters + 1), byt["spmv"], 1),
+    }
+    kern_name = {"mdot": "k_mdot", "continuity_assembly": "k_scalar_gather<ContinuityOp>",
+                 "momentum_assembly": "k_momentum_atomic", "sgs_apply": "k_gs_color", "spmv": "k_spmv_stream"}
+    dom = max(per_step, key=lambda k: per_step[k][0])
+    unit_time = {"mdot": med["mdot"], "continuity_assembly": med["continuity_assembly"],
+                 "momentum_assembly": med["momentum_assembly"], "sgs_apply": t_sgs, "spmv": t_spmv}[dom]
+    launches = per_step[dom][2]
+    achieved = byt[dom] / unit_time / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_%d.json" % (dom, n))
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (oracle, %d^3 sample)" % args.cpu_n, file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_n)
+
+    out = {
+        "metric": METRIC,
+        "value": E / t_asm / 1e6,
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded LCG mesh jitter and velocity noise, analytic Taylor-Green fields)",
+        "config": {"workload": "low-Mach momentum + continuity assembly and CG+SGS pressure-Poisson solve, "
+                               "%d^3 Hex8, jitter 0.1h, plain FP64" % n,
+                   "elements": E, "nodes": N, "continuity_nnz": nnz1, "momentum_nnz": nnz3,
+                   "parallelism": "single GPU"},
+        "assembly_s": t_asm,
+        "phase_s": med,
+        "pressure_solve_s": med["pressure_solve"],
+        "pressure_iters": iters,
+        "pressure_true_rel_resid": resid,
+        "sgs_colors": ncol,
+        "kernels": {
+            k: {"time_s": per_step[k][0], "unit_time_s": {"sgs_apply": t_sgs, "spmv": t_spmv}.get(k, per_step[k][0]),
+                "algorithmic_bytes": byt[k],
+                "GBps": byt[k] / {"sgs_apply": t_sgs, "spmv": t_spmv}.get(k, per_step[k][0]) / 1e9}
+            for k in per_step},
+        "roofline": {"bound": "hbm", "kernel": kern_name[dom], "unit_of_work": dom,
+                     "launches_per_unit": launches,
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

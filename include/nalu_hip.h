@@ -129,6 +129,11 @@ int nalu_linsys_apply_dirichlet(nalu_linsys *ls, int64_t num_bc_nodes,
  * become diag_value*e_i with rhs_residual in the rhs, for dofs [begin, end). */
 int nalu_linsys_reset_rows(nalu_linsys *ls, int64_t num_nodes, const int32_t *nodes_dev,
                            int begin_pos, int end_pos, double diag_value, double rhs_residual);
+/* Subtract the mean of the rhs: projects the rhs of a pure-Neumann (all-wall)
+ * pressure-Poisson system onto the range of its matrix, whose null space is
+ * the constants (the reference instead pins one node, FixPressureAtNodeAlgorithm,
+ * src/FixPressureAtNodeAlgorithm.C). */
+int nalu_linsys_remove_rhs_mean(nalu_linsys *ls);
 /* loadComplete (src/TpetraLinearSystem.C:1757-1778): serial, no export. */
 int nalu_linsys_load_complete(nalu_linsys *ls);
 /* solve (src/TpetraLinearSystem.C:1780-1847): preconditioner compute + Krylov

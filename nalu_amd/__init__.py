@@ -86,6 +86,7 @@ def lib():
         L.nalu_linsys_apply_dirichlet.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int, C.c_int]
         L.nalu_linsys_reset_rows.argtypes = [_vp, _i64, _vp, C.c_int, C.c_int, C.c_double, C.c_double]
         L.nalu_linsys_load_complete.argtypes = [_vp]
+        L.nalu_linsys_remove_rhs_mean.argtypes = [_vp]
         L.nalu_linsys_solve.argtypes = [_vp, _vp]
         L.nalu_linsys_set_l2_scaling.argtypes = [_vp, C.c_double]
         L.nalu_linsys_sizes.argtypes = [_vp, _i64p, _i64p]
@@ -281,6 +282,14 @@ class LinearSystem:
     def loadComplete(self):
         _check(lib().nalu_linsys_load_complete(self._h), "loadComplete")
 
+    def removeRhsMean(self):
+        """Project the rhs of a pure-Neumann pressure system onto the matrix range."""
+        _check(lib().nalu_linsys_remove_rhs_mean(self._h), "remove_rhs_mean")
+
+    def set_stream(self, stream_handle):
+        """Run this system's work on a HIP stream (int handle, e.g. torch's cuda_stream)."""
+        _check(lib().nalu_linsys_set_stream(self._h, stream_handle), "set_stream")
+
     # --- solve
     def solve(self, delta_out=None):
         """Returns 0 when converged, 1 when max_iterations was reached."""
@@ -366,9 +375,9 @@ def assemble_scalar_diff(ls, mesh, q, k, shifted=False, mode="gather"):
            "assemble_scalar_diff")
 
 
-def compute_mdot(mesh, rho, p, u, gpdx, mdot, tau=1.0, shift_mdot=False, shift_poisson=False):
+def compute_mdot(mesh, rho, p, u, gpdx, mdot, tau=1.0, shift_mdot=False, shift_poisson=False, stream=None):
     _check(lib().nalu_compute_mdot(mesh.handle, _ptr(rho), _ptr(p), _ptr(u), _ptr(gpdx), float(tau),
-                                   int(shift_mdot), int(shift_poisson), _ptr(mdot), None), "compute_mdot")
+                                   int(shift_mdot), int(shift_poisson), _ptr(mdot), stream), "compute_mdot")
 
 
 def assemble_continuity(ls, mesh, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False,

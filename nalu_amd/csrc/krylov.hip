@@ -586,3 +586,28 @@ extern "C" int nalu_linsys_spmv_f32(nalu_linsys* ls, const float* x, float* y) {
     return NALU_OK;
   });
 }
+
+namespace nalu {
+namespace {
+__global__ void k_sub_mean(double* __restrict__ x, int64_t n, const double* __restrict__ sum) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] -= sum[0] / (double)n;
+}
+}  // namespace
+}  // namespace nalu
+
+extern "C" int nalu_linsys_remove_rhs_mean(nalu_linsys* ls) {
+  return nalu::guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized, NALU_ERR_STATE, "remove_rhs_mean: system not finalized");
+    nalu::materialize_pending_zero(ls);
+    nalu::DevBuf<double> one(ls->nrows);
+    hipLaunchKernelGGL((nalu::k_fill<double>), dim3(nalu::nblocks(ls->nrows)), dim3(nalu::kThreads), 0, ls->stream,
+                       one.p, ls->nrows, 1.0);
+    nalu::Red red(ls);
+    red.dot_dev(ls->rhs.p, one.p, ls->nrows, ls->red_out.p + 255);
+    hipLaunchKernelGGL(nalu::k_sub_mean, dim3(nalu::nblocks(ls->nrows)), dim3(nalu::kThreads), 0, ls->stream,
+                       ls->rhs.p, ls->nrows, ls->red_out.p + 255);
+    NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
