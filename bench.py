@@ -51,7 +51,9 @@ def algorithmic_bytes(N, E, P, rows, nnz1, nnz3):
         "continuity_assembly": N * 11 * w + E * 32 + P * w + N * w,
         "momentum_assembly": N * 7 * w + E * (32 + 12 * w) + 9 * P * w + 3 * N * w,
         "spmv": nnz1 * (w + i) + (rows + 1) * i + 2 * rows * w,
-        "sgs_apply": 2 * (nnz1 * (w + i) + (rows + 1) * i) + 4 * rows * w,
+        # one SGS application = forward sweep over L + backward sweep over U: every
+        # off-diagonal entry once, L/U row pointers, and per row r, 1/d (twice), z (3 accesses)
+        "sgs_apply": (nnz1 - rows) * (w + i) + 2 * (rows + 1) * i + 6 * rows * w,
     }
 
 
@@ -130,11 +132,11 @@ def main():
         na.compute_mdot(mesh, rho, p, u, gp, mdot, tau=P["tau"], stream=stream)
         if e: e[1].record()
         cont.zeroSystem()
-        na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode="gather")
+        na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode="tile")
         cont.loadComplete()
         if e: e[2].record()
         mom.zeroSystem()
-        na.assemble_momentum(mom, mesh, u, mu, mdot, mode="atomic")
+        na.assemble_momentum(mom, mesh, u, mu, mdot, mode="tile")
         mom.loadComplete()
         if e: e[3].record()
         # zero-mean projection of the continuity rhs (singular Neumann pressure system)
@@ -192,12 +194,13 @@ def main():
     per_step = {
         "mdot": (med["mdot"], byt["mdot"], 1),
         "continuity_assembly": (med["continuity_assembly"], byt["continuity_assembly"], 1),
-        "momentum_assembly": (med["momentum_assembly"], byt["momentum_assembly"], 8),
-        "sgs_apply": (t_sgs * (iters + 1), byt["sgs_apply"], 2 * ncol),
+        "momentum_assembly": (med["momentum_assembly"], byt["momentum_assembly"], 1),
+        "sgs_apply": (t_sgs * (iters + 1), byt["sgs_apply"], 2 * ncol - 1),
         "spmv": (t_spmv * (iters + 1), byt["spmv"], 1),
     }
-    kern_name = {"mdot": "k_mdot", "continuity_assembly": "k_scalar_gather<ContinuityOp>",
-                 "momentum_assembly": "k_momentum_atomic", "sgs_apply": "k_gs_color", "spmv": "k_spmv_stream"}
+    kern_name = {"mdot": "k_mdot", "continuity_assembly": "k_scalar_tile<ContinuityOp>",
+                 "momentum_assembly": "k_momentum_tile", "sgs_apply": "k_fwd_color + k_bwd_color",
+                 "spmv": "k_split_spmv"}
     dom = max(per_step, key=lambda k: per_step[k][0])
     unit_time = {"mdot": med["mdot"], "continuity_assembly": med["continuity_assembly"],
                  "momentum_assembly": med["momentum_assembly"], "sgs_apply": t_sgs, "spmv": t_spmv}[dom]

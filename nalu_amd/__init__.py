@@ -367,10 +367,12 @@ class LinearSystem:
 
 
 # ------------------------------------------------------------------ assembly algorithms
-MODE = {"gather": 0, "atomic": 1}
+# scatter strategies: "gather" = one thread per row walks its elements (deterministic),
+# "atomic" = element-centric FP64 atomics, "tile" = node-tile LDS accumulation (fastest)
+MODE = {"gather": 0, "atomic": 1, "tile": 2}
 
 
-def assemble_scalar_diff(ls, mesh, q, k, shifted=False, mode="gather"):
+def assemble_scalar_diff(ls, mesh, q, k, shifted=False, mode="tile"):
     _check(lib().nalu_assemble_scalar_diff(ls.handle, mesh.handle, _ptr(q), _ptr(k), int(shifted), MODE[mode]),
            "assemble_scalar_diff")
 
@@ -381,12 +383,12 @@ def compute_mdot(mesh, rho, p, u, gpdx, mdot, tau=1.0, shift_mdot=False, shift_p
 
 
 def assemble_continuity(ls, mesh, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False,
-                        reduced_sens=False, mode="gather"):
+                        reduced_sens=False, mode="tile"):
     _check(lib().nalu_assemble_continuity(ls.handle, mesh.handle, _ptr(rho), _ptr(p), _ptr(u), _ptr(gpdx),
                                           float(tau), int(shift_mdot), int(shift_poisson), int(reduced_sens),
                                           MODE[mode]), "assemble_continuity")
 
 
-def assemble_momentum(ls, mesh, u, mu, mdot, include_divu=0.0, skew=False, shifted=False, mode="atomic"):
+def assemble_momentum(ls, mesh, u, mu, mdot, include_divu=0.0, skew=False, shifted=False, mode="tile"):
     _check(lib().nalu_assemble_momentum(ls.handle, mesh.handle, _ptr(u), _ptr(mu), _ptr(mdot), float(include_divu),
                                         int(skew), int(shifted), MODE[mode]), "assemble_momentum")

@@ -204,6 +204,140 @@ __global__ __launch_bounds__(256) void k_scalar_gather(Op op, const int32_t* __r
   }
 }
 
+// ---------------------------------------------------------------- node-tile gather
+// Owner-computes with one thread per (row node, local index): a block owns
+// kTileNodes consecutive rows; thread (node j, slot s) evaluates row s of every
+// element in which node j has local index s (one element on a structured
+// lattice).  The slot is uniform across a wave, so the per-LA code path never
+// diverges.  Contributions accumulate in the LDS copy of the block's CSR
+// segment (LDS FP64 atomics: the 8 slots of a node share its row), which is
+// then written to HBM once, coalesced -- no global atomics, no separate zero pass.
+constexpr int kTileNodes = 64;
+constexpr int kTileThreads = kTileNodes * 8;
+
+template <class Op>
+__global__ __launch_bounds__(kTileThreads) void k_scalar_tile(Op op, const int32_t* __restrict__ conn,
+                                                              const double* __restrict__ coords,
+                                                              const int32_t* __restrict__ row_node,
+                                                              const int32_t* __restrict__ inc_ptr,
+                                                              const int32_t* __restrict__ inc_elem,
+                                                              const int32_t* __restrict__ nrp,
+                                                              const uint64_t* __restrict__ elem_pos, int64_t nrows,
+                                                              double* __restrict__ vals, double* __restrict__ rhs,
+                                                              int accumulate) {
+  extern __shared__ double seg[];
+  const int64_t r0 = (int64_t)blockIdx.x * kTileNodes;
+  const int64_t r1 = min(r0 + (int64_t)kTileNodes, nrows);
+  const int32_t s0 = nrp[r0], s1 = nrp[r1];
+  const int nseg = s1 - s0;
+  double* rseg = seg + nseg;
+  for (int i = threadIdx.x; i < nseg + kTileNodes; i += blockDim.x) seg[i] = 0.0;
+  __syncthreads();
+  const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileNodes);  // wave-uniform local index
+  const int j = threadIdx.x % kTileNodes;
+  const int64_t r = r0 + j;
+  if (r < r1) {
+    const int32_t a = row_node ? row_node[r] : (int32_t)r;
+    double* myrow = seg + (nrp[r] - s0);
+    double acc = 0.0;
+    for (int32_t q = inc_ptr[a]; q < inc_ptr[a + 1]; ++q) {
+      const int32_t t = inc_elem[q];
+      if ((t & 7) != slot) continue;
+      const int64_t e = t >> 3;
+      int32_t nd[8];
+      load_nodes(conn, e, nd);
+      double x[8][3];
+      load_vec3(coords, nd, x);
+      const uint64_t pk = elem_pos[t];
+      acc += dispatch_row(op, slot, nd, x, [&](int c, double v) {
+        unsafeAtomicAdd(myrow + (int)((pk >> (8 * c)) & 0xff), v);
+      });
+    }
+    if (acc != 0.0) unsafeAtomicAdd(rseg + j, acc);
+  }
+  __syncthreads();
+  if (accumulate) {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
+    for (int i = threadIdx.x; i < r1 - r0; i += blockDim.x) rhs[r0 + i] += rseg[i];
+  } else {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] = seg[i];
+    for (int i = threadIdx.x; i < r1 - r0; i += blockDim.x) rhs[r0 + i] = rseg[i];
+  }
+}
+
+template <int LA, bool SKEW, bool SHIFT>
+__device__ __forceinline__ void momentum_row_lds(const int32_t (&nd)[8], const double (&x)[8][3],
+                                                 const double* __restrict__ u, const double* __restrict__ mu,
+                                                 const double* __restrict__ md, double incl_divu, double* myrow,
+                                                 int32_t L, uint64_t pk, double (&rh)[3]) {
+  const hex8::Gather gt{nd};
+  const double m3[3] = {md[hex8::adj_ip(LA, 0)], md[hex8::adj_ip(LA, 1)], md[hex8::adj_ip(LA, 2)]};
+  hex8::row_momentum<LA, SKEW, SHIFT, double>(x, gt, u, mu, m3, incl_divu, rh, [&](int c, const double (&blk)[3][3]) {
+    const int p = (int)((pk >> (8 * c)) & 0xff);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) unsafeAtomicAdd(myrow + 3 * i * L + 3 * p + jj, blk[i][jj]);
+  });
+}
+
+// Momentum (numDof 3) node-tile gather: the block's segment holds the 3 rows of
+// each of its nodes (rows 3a..3a+2 are contiguous in the CSR).
+template <bool SKEW, bool SHIFT>
+__global__ __launch_bounds__(kTileThreads) void k_momentum_tile(
+    const int32_t* __restrict__ conn, const double* __restrict__ coords, const double* __restrict__ u,
+    const double* __restrict__ mu, const double* __restrict__ mdot, double incl_divu,
+    const int32_t* __restrict__ row_node, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_elem,
+    const int32_t* __restrict__ nrp, const uint64_t* __restrict__ elem_pos, int64_t nnodes,
+    double* __restrict__ vals, double* __restrict__ rhs, int accumulate) {
+  extern __shared__ double seg[];
+  const int64_t a0 = (int64_t)blockIdx.x * kTileNodes;
+  const int64_t a1 = min(a0 + (int64_t)kTileNodes, nnodes);
+  const int64_t s0 = 9 * (int64_t)nrp[a0], s1 = 9 * (int64_t)nrp[a1];
+  const int nseg = (int)(s1 - s0);
+  double* rseg = seg + nseg;
+  for (int i = threadIdx.x; i < nseg + 3 * kTileNodes; i += blockDim.x) seg[i] = 0.0;
+  __syncthreads();
+  const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileNodes);
+  const int j = threadIdx.x % kTileNodes;
+  const int64_t r = a0 + j;
+  if (r < a1) {
+    const int32_t a = row_node ? row_node[r] : (int32_t)r;
+    const int32_t L = nrp[r + 1] - nrp[r];
+    double* myrow = seg + (9 * (int64_t)nrp[r] - s0);
+    double rh[3] = {0, 0, 0};
+    for (int32_t q = inc_ptr[a]; q < inc_ptr[a + 1]; ++q) {
+      const int32_t t = inc_elem[q];
+      if ((t & 7) != slot) continue;
+      const int64_t e = t >> 3;
+      int32_t nd[8];
+      load_nodes(conn, e, nd);
+      double x[8][3];
+      load_vec3(coords, nd, x);
+      const uint64_t pk = elem_pos[t];
+      const double* md = mdot + 12 * e;
+      switch (slot) {
+#define NALU_MT(Lx) \
+  case Lx: momentum_row_lds<Lx, SKEW, SHIFT>(nd, x, u, mu, md, incl_divu, myrow, L, pk, rh); break;
+        NALU_MT(0) NALU_MT(1) NALU_MT(2) NALU_MT(3) NALU_MT(4) NALU_MT(5) NALU_MT(6) NALU_MT(7)
+#undef NALU_MT
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (rh[i] != 0.0) unsafeAtomicAdd(rseg + 3 * j + i, rh[i]);
+  }
+  __syncthreads();
+  const int nr = (int)(3 * (a1 - a0));
+  if (accumulate) {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] += rseg[i];
+  } else {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] = seg[i];
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] = rseg[i];
+  }
+}
+
 // ---------------------------------------------------------------- momentum
 template <int LA, bool SKEW, bool SHIFT>
 __device__ __forceinline__ void momentum_row_atomic(const int32_t (&nd)[8], const double (&x)[8][3],
@@ -340,6 +474,18 @@ __global__ void k_dirichlet(const int32_t* __restrict__ nodes, int64_t nbc, int6
   rhs[row] = reset_mode ? rhs_residual : bcv[(int64_t)nd * node + d] - sol[(int64_t)nd * node + d];
 }
 
+// Opt a kernel into more than the default 64 KiB of dynamic LDS when needed.
+void set_lds_limit(const void* kern, size_t bytes) {
+  int dev = 0;
+  NALU_HIP_CHECK(hipGetDevice(&dev));
+  int maxopt = 0;
+  NALU_HIP_CHECK(hipDeviceGetAttribute(&maxopt, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+  if (maxopt <= 0) NALU_HIP_CHECK(hipDeviceGetAttribute(&maxopt, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  NALU_REQUIRE(bytes <= (size_t)maxopt, NALU_ERR_UNSUPPORTED,
+               "assembly tile needs " + std::to_string(bytes) + " B of LDS, device allows " + std::to_string(maxopt));
+  if (bytes > 64 * 1024) NALU_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 template <class Op>
 void launch_scalar(nalu_linsys* ls, const nalu_mesh* m, const Op& op, int mode) {
   hipStream_t s = ls->stream;
@@ -355,6 +501,17 @@ void launch_scalar(nalu_linsys* ls, const nalu_mesh* m, const Op& op, int mode) 
     hipLaunchKernelGGL((k_scalar_gather<Op>), dim3(blocks_for(ls->nrows, B)), dim3(B), lds, s, op, m->conn.p,
                        m->coords.p, (const int32_t*)ls->row_node.p, m->inc_ptr.p, m->inc_elem.p, ls->nrp.p,
                        ls->elem_pos.p, ls->nrows, ls->values.p, ls->rhs.p, accumulate);
+    ls->values_pending_zero = ls->rhs_pending_zero = false;
+  } else if (mode == 2) {
+    build_node_incidence(const_cast<nalu_mesh*>(m), s);
+    NALU_REQUIRE(ls->values_pending_zero == ls->rhs_pending_zero, NALU_ERR_STATE,
+                 "tile assembly: values/rhs zero state diverged");
+    const int accumulate = ls->values_pending_zero ? 0 : 1;
+    const size_t lds = ((size_t)kTileNodes * ls->max_node_row_len + kTileNodes) * sizeof(double);
+    set_lds_limit((const void*)k_scalar_tile<Op>, lds);
+    hipLaunchKernelGGL((k_scalar_tile<Op>), dim3(blocks_for(ls->nrows, kTileNodes)), dim3(kTileThreads), lds, s, op,
+                       m->conn.p, m->coords.p, (const int32_t*)ls->row_node.p, m->inc_ptr.p, m->inc_elem.p,
+                       ls->nrp.p, ls->elem_pos.p, ls->nrows, ls->values.p, ls->rhs.p, accumulate);
     ls->values_pending_zero = ls->rhs_pending_zero = false;
   } else {
     materialize_pending_zero(ls);
@@ -396,7 +553,7 @@ extern "C" int nalu_assemble_scalar_diff(nalu_linsys* ls, const nalu_mesh* m, co
   return guarded([&]() -> int {
     check_assembly_args(ls, m, 1);
     NALU_REQUIRE(q && k, NALU_ERR_ARG, "scalar_diff: null field");
-    NALU_REQUIRE(mode == 0 || mode == 1, NALU_ERR_ARG, "scalar_diff: mode must be 0 or 1");
+    NALU_REQUIRE(mode >= 0 && mode <= 2, NALU_ERR_ARG, "scalar_diff: mode must be 0, 1 or 2");
     PhaseTimer t(ls, PH_ASSEMBLE);
     if (shifted) launch_scalar(ls, m, ScalarDiffOp<true>{q, k}, mode);
     else launch_scalar(ls, m, ScalarDiffOp<false>{q, k}, mode);
@@ -411,7 +568,7 @@ extern "C" int nalu_assemble_continuity(nalu_linsys* ls, const nalu_mesh* m, con
     check_assembly_args(ls, m, 1);
     NALU_REQUIRE(rho && p && u && gp, NALU_ERR_ARG, "continuity: null field");
     NALU_REQUIRE(tau > 0.0, NALU_ERR_ARG, "continuity: projTimeScale must be positive");
-    NALU_REQUIRE(mode == 0 || mode == 1, NALU_ERR_ARG, "continuity: mode must be 0 or 1");
+    NALU_REQUIRE(mode >= 0 && mode <= 2, NALU_ERR_ARG, "continuity: mode must be 0, 1 or 2");
     PhaseTimer t(ls, PH_ASSEMBLE);
     const int key = (shift_mdot ? 4 : 0) | (shift_poisson ? 2 : 0) | (reduced_sens ? 1 : 0);
     switch (key) {
@@ -458,19 +615,40 @@ extern "C" int nalu_assemble_momentum(nalu_linsys* ls, const nalu_mesh* m, const
   return guarded([&]() -> int {
     check_assembly_args(ls, m, 3);
     NALU_REQUIRE(u && mu && mdot, NALU_ERR_ARG, "momentum: null field");
-    NALU_REQUIRE(mode == 1, NALU_ERR_UNSUPPORTED, "momentum: only the atomic scatter (mode 1) is implemented");
+    NALU_REQUIRE(mode == 1 || mode == 2, NALU_ERR_UNSUPPORTED,
+                 "momentum: mode 1 (atomic) or 2 (node tile); the per-thread row gather is scalar-only");
     PhaseTimer t(ls, PH_ASSEMBLE);
-    materialize_pending_zero(ls);
     const int64_t E = m->num_elems;
-    const dim3 g(chunk_blocks<8>(E)), b(kChunk);
-    auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, g, b, 0, ls->stream, m->conn.p, m->coords.p, u, mu, mdot, include_divu,
-                         ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, E, ls->values.p, ls->rhs.p);
-    };
-    if (skew && shifted) go(k_momentum_atomic<true, true>);
-    else if (skew) go(k_momentum_atomic<true, false>);
-    else if (shifted) go(k_momentum_atomic<false, true>);
-    else go(k_momentum_atomic<false, false>);
+    if (mode == 2) {
+      build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
+      NALU_REQUIRE(ls->values_pending_zero == ls->rhs_pending_zero, NALU_ERR_STATE,
+                   "tile assembly: values/rhs zero state diverged");
+      const int accumulate = ls->values_pending_zero ? 0 : 1;
+      const size_t lds = ((size_t)kTileNodes * ls->max_node_row_len * 9 + 3 * kTileNodes) * sizeof(double);
+      auto go = [&](auto kern) {
+        set_lds_limit((const void*)kern, lds);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(ls->num_nodes, kTileNodes)), dim3(kTileThreads), lds, ls->stream,
+                           m->conn.p, m->coords.p, u, mu, mdot, include_divu, (const int32_t*)ls->row_node.p,
+                           m->inc_ptr.p, m->inc_elem.p, ls->nrp.p, ls->elem_pos.p, ls->num_nodes, ls->values.p,
+                           ls->rhs.p, accumulate);
+      };
+      if (skew && shifted) go(k_momentum_tile<true, true>);
+      else if (skew) go(k_momentum_tile<true, false>);
+      else if (shifted) go(k_momentum_tile<false, true>);
+      else go(k_momentum_tile<false, false>);
+      ls->values_pending_zero = ls->rhs_pending_zero = false;
+    } else {
+      materialize_pending_zero(ls);
+      const dim3 g(chunk_blocks<8>(E)), b(kChunk);
+      auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, g, b, 0, ls->stream, m->conn.p, m->coords.p, u, mu, mdot, include_divu,
+                           ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, E, ls->values.p, ls->rhs.p);
+      };
+      if (skew && shifted) go(k_momentum_atomic<true, true>);
+      else if (skew) go(k_momentum_atomic<true, false>);
+      else if (shifted) go(k_momentum_atomic<false, true>);
+      else go(k_momentum_atomic<false, false>);
+    }
     NALU_HIP_CHECK(hipGetLastError());
     return NALU_OK;
   });

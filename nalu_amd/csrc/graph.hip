@@ -244,6 +244,60 @@ __global__ void k_color_resolve(const int32_t* __restrict__ rp, const int32_t* _
   else atomicAdd(remaining, 1);
 }
 
+// Node-parity coloring: on a hex mesh whose elements share consistent local
+// axes (any structured-topology mesh), every node carries a 3-bit parity
+// (i mod 2, j mod 2, k mod 2) that differs between any two nodes of an
+// element, i.e. an 8-coloring of the node graph.  It is found by propagating
+// parity across elements from a seed node; an inconsistent element means the
+// mesh has no such labelling and the speculative coloring is used instead.
+__global__ void k_parity_prop(const int32_t* __restrict__ conn, int64_t nelem, int32_t* par,
+                              int* __restrict__ changed) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int32_t* nd = conn + 8 * e;
+  int p0 = -1;
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const int32_t pn = par[nd[n]];
+    if (pn >= 0 && p0 < 0) p0 = pn ^ ((n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0) ^
+                                    ((n == 2 || n == 3 || n == 6 || n == 7) ? 2 : 0) ^ (n >= 4 ? 4 : 0);
+  }
+  if (p0 < 0) return;
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const int32_t want = p0 ^ ((n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0) ^
+                         ((n == 2 || n == 3 || n == 6 || n == 7) ? 2 : 0) ^ (n >= 4 ? 4 : 0);
+    if (atomicCAS(par + nd[n], -1, want) == -1) atomicOr(changed, 1);
+  }
+}
+
+__global__ void k_parity_check(const int32_t* __restrict__ conn, int64_t nelem, const int32_t* __restrict__ par,
+                               int* __restrict__ bad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int32_t* nd = conn + 8 * e;
+  const int p0 = par[nd[0]];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const int want = p0 ^ ((n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0) ^
+                     ((n == 2 || n == 3 || n == 6 || n == 7) ? 2 : 0) ^ (n >= 4 ? 4 : 0);
+    if (p0 < 0 || par[nd[n]] != want) atomicOr(bad, 1);
+  }
+}
+
+// row color = numDof * parity(node of row) + dof; rows indexed by lid
+__global__ void k_parity_rows(const int32_t* __restrict__ par, const int32_t* __restrict__ lid, int64_t nnodes,
+                              int nd, int32_t* __restrict__ color) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nnodes) return;
+  for (int d = 0; d < nd; ++d) color[(int64_t)nd * lid[a] + d] = nd * par[a] + d;
+}
+
+__global__ void k_first_unlabeled(const int32_t* __restrict__ par, int64_t n, int* __restrict__ first) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && par[i] < 0) atomicMin(first, (int)i);
+}
+
 __global__ void k_color_keys(const int32_t* __restrict__ color, int64_t n, uint64_t* __restrict__ keys,
                              int32_t* __restrict__ hist) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -403,13 +457,52 @@ void build_graph(nalu_linsys* ls, hipStream_t s) {
   NALU_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Parity labelling of the mesh nodes; false when the mesh admits none.
+bool parity_coloring(nalu_linsys* ls, DevBuf<int32_t>& color, hipStream_t s) {
+  const nalu_mesh* m = ls->mesh;
+  const int64_t N = m->num_nodes, E = m->num_elems;
+  DevBuf<int32_t> par(N);
+  DevBuf<int> flag(2);
+  NALU_HIP_CHECK(hipMemsetAsync(par.p, 0xff, sizeof(int32_t) * N, s));
+  int seeds = 0;
+  for (;;) {
+    // seed the lowest unlabeled node (one seed per connected component)
+    int first = 0x7fffffff;
+    k_set_i32<<<1, 1, 0, s>>>(flag.p, 0x7fffffff);
+    k_first_unlabeled<<<grid_for(N), kBlock, 0, s>>>(par.p, N, flag.p);
+    NALU_HIP_CHECK(hipMemcpyAsync(&first, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    NALU_HIP_CHECK(hipStreamSynchronize(s));
+    if (first == 0x7fffffff) break;
+    if (++seeds > 1024) return false;  // too fragmented to be worth it
+    k_set_i32<<<1, 1, 0, s>>>(par.p + first, 0);
+    for (int round = 0;; ++round) {
+      if (round > 100000) return false;
+      NALU_HIP_CHECK(hipMemsetAsync(flag.p + 1, 0, sizeof(int), s));
+      k_parity_prop<<<grid_for(E), kBlock, 0, s>>>(m->conn.p, E, par.p, flag.p + 1);
+      int ch = 0;
+      NALU_HIP_CHECK(hipMemcpyAsync(&ch, flag.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+      NALU_HIP_CHECK(hipStreamSynchronize(s));
+      if (!ch) break;
+    }
+  }
+  NALU_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+  k_parity_check<<<grid_for(E), kBlock, 0, s>>>(m->conn.p, E, par.p, flag.p);
+  int bad = 0;
+  NALU_HIP_CHECK(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  NALU_HIP_CHECK(hipStreamSynchronize(s));
+  if (bad) return false;
+  k_parity_rows<<<grid_for(N), kBlock, 0, s>>>(par.p, ls->node_lid.p, N, ls->num_dof, color.p);
+  return true;
+}
+
 void build_coloring(nalu_linsys* ls, hipStream_t s) {
   const int64_t n = ls->nrows;
   DevBuf<int32_t> color(n), prop(n);
   DevBuf<int> cnt(2);
   NALU_HIP_CHECK(hipMemsetAsync(color.p, 0xff, sizeof(int32_t) * n, s));
   int rounds = 0;
-  for (;;) {
+  const bool parity = parity_coloring(ls, color, s);
+  for (; !parity;) {
     NALU_HIP_CHECK(hipMemsetAsync(cnt.p, 0, 2 * sizeof(int), s));
     k_color_propose<<<grid_for(n), kBlock, 0, s>>>(ls->row_ptr.p, ls->col_idx.p, n, color.p, prop.p, cnt.p + 1);
     k_color_resolve<<<grid_for(n), kBlock, 0, s>>>(ls->row_ptr.p, ls->col_idx.p, n, color.p, prop.p, cnt.p);

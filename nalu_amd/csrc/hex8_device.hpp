@@ -123,6 +123,24 @@ __host__ __device__ constexpr double diff_weight(int q, int p, int n) {
   return (((mq >> n) & 1) ? 1.0 / popc8(mq) : 0.0) - (((mp >> n) & 1) ? 1.0 / popc8(mp) : 0.0);
 }
 
+// Shape-function and derivative tables at the scs ips, for loops whose column
+// index is a runtime value (kept in the constant cache; the ip stays compile-time).
+struct ScsTables {
+  double deriv[2][12][8][3];  // [shifted][ip][node][dir]
+  double shape[2][12][8];     // [shifted][ip][node]
+};
+__host__ __device__ constexpr ScsTables make_scs_tables() {
+  ScsTables t{};
+  for (int sh = 0; sh < 2; ++sh)
+    for (int ip = 0; ip < 12; ++ip)
+      for (int n = 0; n < 8; ++n) {
+        t.shape[sh][ip][n] = shape(ip, n, sh != 0);
+        for (int d = 0; d < 3; ++d) t.deriv[sh][ip][n][d] = deriv(ip, n, d, sh != 0);
+      }
+  return t;
+}
+__constant__ const ScsTables kScsTab = make_scs_tables();
+
 // area vector of scs IP: 1/2 (p2 - p0) x (p3 - p1)
 template <int IP, typename R>
 __device__ __forceinline__ void scs_area(const R (&x)[8][3], R (&A)[3]) {
@@ -373,28 +391,34 @@ __device__ __forceinline__ void row_momentum(const R (&x)[8][3], const Gather& g
 #pragma unroll
       for (int j = 0; j < 3; ++j) Cs[kk][i][j] = R(C[i][j] * inv);
   });
+  constexpr int IP0 = adj_ip(LA, 0), IP1 = adj_ip(LA, 1), IP2 = adj_ip(LA, 2);
+  const double(*dv[3])[3] = {kScsTab.deriv[SHIFT][IP0], kScsTab.deriv[SHIFT][IP1], kScsTab.deriv[SHIFT][IP2]};
+  const double* sh[3] = {kScsTab.shape[0][IP0], kScsTab.shape[0][IP1], kScsTab.shape[0][IP2]};
+  const double* sa[3] = {kScsTab.shape[SKEW][IP0], kScsTab.shape[SKEW][IP1], kScsTab.shape[SKEW][IP2]};
   // pass 1: ip values of mu, u (advection shape functions) and divU
   R muIp[3] = {R(0), R(0), R(0)}, divU[3] = {R(0), R(0), R(0)}, uIp[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) uIp[a][b] = R(0);
-  static_for<8>([&](auto cc) {
-    constexpr int c = decltype(cc)::value;
+#pragma unroll 1
+  for (int c = 0; c < 8; ++c) {
     const R muc = R(gt.s(mu, c));
     const R uc[3] = {R(gt.v(u, c, 0)), R(gt.v(u, c, 1)), R(gt.v(u, c, 2))};
-    static_for<3>([&](auto kk) {
-      constexpr int IP = adj_ip(LA, decltype(kk)::value);
-      muIp[kk] += R(shape(IP, c, false)) * muc;
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+      muIp[kk] += R(sh[kk][c]) * muc;
       R dn[3];
-      dndx_scaled<IP, c, SHIFT>(Cs[kk], dn);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        dn[i] = Cs[kk][i][0] * R(dv[kk][c][0]) + Cs[kk][i][1] * R(dv[kk][c][1]) + Cs[kk][i][2] * R(dv[kk][c][2]);
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        uIp[kk][j] += R(shape(IP, c, SKEW)) * uc[j];
+        uIp[kk][j] += R(sa[kk][c]) * uc[j];
         divU[kk] += uc[j] * dn[j];
       }
-    });
-  });
+    }
+  }
   static_for<3>([&](auto kk) {
     constexpr R S = R(adj_sign(LA, decltype(kk)::value));
 #pragma unroll
@@ -403,24 +427,26 @@ __device__ __forceinline__ void row_momentum(const R (&x)[8][3], const Gather& g
       A[kk][i] *= muIp[kk];  // A becomes mu_ip A
     }
   });
-  // the nodal velocities are re-read below rather than kept live across passes
-  asm volatile("" ::: "memory");
+  constexpr R S0 = R(adj_sign(LA, 0)), S1 = R(adj_sign(LA, 1)), S2 = R(adj_sign(LA, 2));
+  const R Sg[3] = {S0, S1, S2};
   // pass 2: one 3x3 column block at a time
-  static_for<8>([&](auto cc) {
-    constexpr int c = decltype(cc)::value;
+#pragma unroll 1
+  for (int c = 0; c < 8; ++c) {
     const R uc[3] = {R(gt.v(u, c, 0)), R(gt.v(u, c, 1)), R(gt.v(u, c, 2))};
     R blk[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) blk[i][j] = R(0);
-    static_for<3>([&](auto kk) {
-      constexpr int IP = adj_ip(LA, decltype(kk)::value);
-      constexpr R S = R(adj_sign(LA, decltype(kk)::value));
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+      const R S = Sg[kk];
       R dn[3];
-      dndx_scaled<IP, c, SHIFT>(Cs[kk], dn);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        dn[i] = Cs[kk][i][0] * R(dv[kk][c][0]) + Cs[kk][i][1] * R(dv[kk][c][1]) + Cs[kk][i][2] * R(dv[kk][c][2]);
       const R h = dn[0] * A[kk][0] + dn[1] * A[kk][1] + dn[2] * A[kk][2];
-      const R adv = R(shape(IP, c, SKEW)) * mdot3[kk];
+      const R adv = R(sa[kk][c]) * mdot3[kk];
       const R Au = uc[0] * A[kk][0] + uc[1] * A[kk][1] + uc[2] * A[kk][2];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -429,9 +455,9 @@ __device__ __forceinline__ void row_momentum(const R (&x)[8][3], const Gather& g
         for (int j = 0; j < 3; ++j) blk[i][j] -= S * (dn[i] * A[kk][j]);
         rhs[i] += S * (dn[i] * Au + h * uc[i]);
       }
-    });
+    }
     emit(c, blk);
-  });
+  }
 }
 
 }  // namespace hex8

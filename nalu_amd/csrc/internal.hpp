@@ -137,16 +137,21 @@ struct nalu_linsys {
   std::vector<int64_t> color_ptr_host;
   nalu::DevBuf<int32_t> color_rows;  // rows sorted by color
 
-  // preconditioner / solver workspace
-  nalu::DevBuf<double> inv_diag;   // [nrows]
-  nalu::DevBuf<float> values32;    // FP32 copy of values (mixed precision)
-  nalu::DevBuf<float> inv_diag32;
-  nalu::DevBuf<double> work;       // Krylov vectors
-  nalu::DevBuf<float> work32;
+  // solver workspace
   nalu::DevBuf<double> red;        // reduction partials
   nalu::DevBuf<double> red_out;    // reduction results
   std::vector<double> red_host;
   bool precond_ready = false;
+
+  // color-permuted copy of the matrix used inside the solvers: rows grouped by
+  // color, each row split into L (columns of earlier colors) and U (later
+  // colors) parts plus the diagonal; the sources index the CSR values.
+  bool perm_ready = false;
+  nalu::DevBuf<int32_t> perm, iperm, cpd;        // new->old row, old->new row, color ranges
+  nalu::DevBuf<int32_t> lrp, lci, lsrc, urp, uci, usrc, dsrc;
+  nalu::DevBuf<double> lval, uval, dval, pinvd;
+  nalu::DevBuf<float> lval32, uval32, dval32, pinvd32;
+  int max_l_len = 0, max_u_len = 0;
 
   // learned preconditioner (optional)
   void* learned = nullptr;

@@ -43,7 +43,7 @@ def make_system(coords, conn, nd=1, cfg=None, ids=None):
 
 # ------------------------------------------------------------- reference golds on one element
 
-@pytest.mark.parametrize("mode", ["gather", "atomic"])
+@pytest.mark.parametrize("mode", ["gather", "atomic", "tile"])
 def test_scalar_diff_gold_gpu(golds, mode):
     print("case: scalar_diff gold, mode", mode, flush=True)
     mesh, ls = make_system(UNIT_HEX, np.arange(8, dtype=np.int32)[None, :])
@@ -55,7 +55,7 @@ def test_scalar_diff_gold_gpu(golds, mode):
     np.testing.assert_allclose(r, 0.0, atol=1e-14)
 
 
-@pytest.mark.parametrize("mode", ["gather", "atomic"])
+@pytest.mark.parametrize("mode", ["gather", "atomic", "tile"])
 @pytest.mark.parametrize("name,shift,reduced", [("continuity_default", False, False),
                                                 ("continuity_reduced_sens", False, True),
                                                 ("continuity_reduced_sens", True, True)])
@@ -72,8 +72,9 @@ def test_continuity_gold_gpu(golds, mode, name, shift, reduced):
     np.testing.assert_allclose(r, golds[name]["rhs"], rtol=0, atol=1e-14)
 
 
-def test_momentum_gold_gpu(golds):
-    print("case: momentum gold", flush=True)
+@pytest.mark.parametrize("mode", ["atomic", "tile"])
+def test_momentum_gold_gpu(golds, mode):
+    print("case: momentum gold", mode, flush=True)
     X = UNIT_HEX
     g = golds["momentum_advdiff"]
     mesh, ls = make_system(X, np.arange(8, dtype=np.int32)[None, :], nd=3)
@@ -82,7 +83,7 @@ def test_momentum_gold_gpu(golds):
     # mdot of the unit-test producer is rho u . A: no pressure terms
     na.compute_mdot(mesh, dev(np.ones(8)), dev(np.zeros(8)), u, dev(np.zeros((8, 3))), mdot, tau=1.0)
     ls.zeroSystem()
-    na.assemble_momentum(ls, mesh, u, dev(np.full(8, g["mu"])), mdot)
+    na.assemble_momentum(ls, mesh, u, dev(np.full(8, g["mu"])), mdot, mode=mode)
     rp, ci, v, r = ls.csr()
     np.testing.assert_allclose(dense_from_csr(rp, ci, v, 24), np.array(g["lhs"]).reshape(24, 24), rtol=0,
                                atol=1e-14)
@@ -112,7 +113,7 @@ def test_tpetra_gold_sum_into_batch(golds):
 
 # ------------------------------------------------------------- whole-mesh parity
 
-@pytest.mark.parametrize("mode", ["gather", "atomic"])
+@pytest.mark.parametrize("mode", ["gather", "atomic", "tile"])
 def test_heat_assembly_parity(oracle, mode):
     n = 8
     print("case: heat assembly n=%d mode=%s" % (n, mode), flush=True)
@@ -162,7 +163,7 @@ def test_gather_accumulates_after_another_algorithm(oracle):
     np.testing.assert_allclose(r2, 2 * r1, rtol=0, atol=TOL * np.abs(r1).max())
 
 
-@pytest.mark.parametrize("mode", ["gather", "atomic"])
+@pytest.mark.parametrize("mode", ["gather", "atomic", "tile"])
 def test_continuity_and_mdot_parity(oracle, mode):
     n = 8
     print("case: continuity + mdot n=%d mode=%s" % (n, mode), flush=True)
@@ -183,16 +184,17 @@ def test_continuity_and_mdot_parity(oracle, mode):
     np.testing.assert_allclose(mdot.cpu().numpy(), om, rtol=0, atol=TOL * np.abs(om).max())
 
 
-def test_momentum_parity(oracle):
+@pytest.mark.parametrize("mode", ["atomic", "tile"])
+def test_momentum_parity(oracle, mode):
     n = 6
-    print("case: momentum n=%d" % n, flush=True)
+    print("case: momentum n=%d %s" % (n, mode), flush=True)
     P = lowmach_problem(n, seed=31, mu=0.05)
     mesh, ls = make_system(P["coords"], P["conn"], nd=3)
     rho, p, u, gp = dev(P["rho"]), dev(P["p"]), dev(P["u"]), dev(P["gpdx"])
     mdot = torch.zeros((P["conn"].shape[0], 12), dtype=torch.float64, device=DEV)
     na.compute_mdot(mesh, rho, p, u, gp, mdot, tau=P["tau"])
     ls.zeroSystem()
-    na.assemble_momentum(ls, mesh, u, dev(P["mu"]), mdot)
+    na.assemble_momentum(ls, mesh, u, dev(P["mu"]), mdot, mode=mode)
     rp, ci, v, r = ls.csr()
     g = oracle.graph_build(P["coords"].shape[0], P["conn"], 3)
     np.testing.assert_array_equal(rp, g.row_ptr)
@@ -209,7 +211,7 @@ def test_explicit_global_ids(oracle):
     ids = np.random.default_rng(4).permutation(nn).astype(np.int64) + 1
     mesh, ls = make_system(P["coords"], P["conn"], ids=ids)
     ls.zeroSystem()
-    na.assemble_scalar_diff(ls, mesh, dev(np.sin(P["coords"][:, 2])), dev(P["k"]), mode="gather")
+    na.assemble_scalar_diff(ls, mesh, dev(np.sin(P["coords"][:, 2])), dev(P["k"]), mode="tile")
     rp, ci, v, r = ls.csr()
     g = oracle.graph_build(nn, P["conn"], 1, node_ids=ids)
     np.testing.assert_array_equal(rp, g.row_ptr)
@@ -312,6 +314,29 @@ def test_iteration_cap_reports_not_converged():
     dT = torch.zeros(ls.sizes()[0], dtype=torch.float64, device=DEV)
     assert ls.solve(dT) == 1
     assert ls.linearSolveIterations() == 5
+
+
+def test_parity_coloring_on_lattice_and_fallback():
+    print("case: coloring", flush=True)
+    P = heat_problem(6, seed=2)
+    for nd, expect in ((1, 8), (3, 24)):
+        mesh, ls = make_system(P["coords"], P["conn"], nd=nd)
+        cp, cr = ls.coloring()
+        assert len(cp) - 1 == expect
+        np.testing.assert_array_equal(np.sort(cr), np.arange(ls.sizes()[0]))
+    # an element with rotated local axes admits no parity labelling: fallback coloring stays valid
+    coords, conn = lattice_1x1xN(2)
+    conn = conn.copy()
+    conn[1] = conn[1][[1, 2, 3, 0, 5, 6, 7, 4]]
+    mesh, ls = make_system(coords, conn)
+    rp, ci, _, _ = ls.csr()
+    cp, cr = ls.coloring()
+    color = np.zeros(len(rp) - 1, dtype=int)
+    for c in range(len(cp) - 1):
+        color[cr[cp[c]:cp[c + 1]]] = c
+    for i in range(len(rp) - 1):
+        nb = ci[rp[i]:rp[i + 1]]
+        assert np.all((color[nb] != color[i]) | (nb == i))
 
 
 def test_wrong_numdof_is_rejected():
