@@ -168,6 +168,12 @@ int nalu_linsys_spmv(nalu_linsys *ls, const double *x_dev, double *y_dev);
 int nalu_linsys_spmv_f32(nalu_linsys *ls, const float *x_dev, float *y_dev);
 int nalu_linsys_precond_setup(nalu_linsys *ls);
 int nalu_linsys_precond_apply(nalu_linsys *ls, const double *r_dev, double *z_dev);
+/* The solvers work on a copy of the matrix in color-contiguous row order:
+ * new row k is original row perm[k] (perm_host [rows]).  These apply M^-1 and
+ * A to vectors in that order (after nalu_linsys_precond_setup). */
+int nalu_linsys_permutation(nalu_linsys *ls, int32_t *perm_host);
+int nalu_linsys_precond_apply_permuted(nalu_linsys *ls, const double *r_dev, double *z_dev);
+int nalu_linsys_spmv_permuted(nalu_linsys *ls, const double *x_dev, double *y_dev);
 /* Solve A x = b for a caller-provided rhs (x in/out is the initial guess). */
 int nalu_linsys_solve_rhs(nalu_linsys *ls, const double *b_dev, double *x_dev, int *iters,
                           double *final_rel_resid);

@@ -98,6 +98,9 @@ def lib():
         L.nalu_linsys_spmv_f32.argtypes = [_vp, _vp, _vp]
         L.nalu_linsys_precond_setup.argtypes = [_vp]
         L.nalu_linsys_precond_apply.argtypes = [_vp, _vp, _vp]
+        L.nalu_linsys_precond_apply_permuted.argtypes = [_vp, _vp, _vp]
+        L.nalu_linsys_spmv_permuted.argtypes = [_vp, _vp, _vp]
+        L.nalu_linsys_permutation.argtypes = [_vp, _i32p]
         L.nalu_linsys_solve_rhs.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int), _dp]
         L.nalu_assemble_scalar_diff.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int]
         L.nalu_compute_mdot.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp]
@@ -361,6 +364,19 @@ class LinearSystem:
 
     def precond_apply(self, r, z):
         _check(lib().nalu_linsys_precond_apply(self._h, _ptr(r), _ptr(z)), "precond_apply")
+
+    def permutation(self):
+        """Solver row order: new row k is original row perm[k]."""
+        nr, _ = self.sizes()
+        perm = np.zeros(nr, dtype=np.int32)
+        _check(lib().nalu_linsys_permutation(self._h, perm.ctypes.data_as(_i32p)), "permutation")
+        return perm
+
+    def precond_apply_permuted(self, r, z):
+        _check(lib().nalu_linsys_precond_apply_permuted(self._h, _ptr(r), _ptr(z)), "precond_apply_permuted")
+
+    def spmv_permuted(self, x, y):
+        _check(lib().nalu_linsys_spmv_permuted(self._h, _ptr(x), _ptr(y)), "spmv_permuted")
 
     def set_l2_scaling(self, s):
         _check(lib().nalu_linsys_set_l2_scaling(self._h, float(s)), "set_l2_scaling")

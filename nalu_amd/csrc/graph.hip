@@ -298,12 +298,24 @@ __global__ void k_first_unlabeled(const int32_t* __restrict__ par, int64_t n, in
   if (i < n && par[i] < 0) atomicMin(first, (int)i);
 }
 
-__global__ void k_color_keys(const int32_t* __restrict__ color, int64_t n, uint64_t* __restrict__ keys,
-                             int32_t* __restrict__ hist) {
+__global__ void k_color_keys(const int32_t* __restrict__ color, int64_t n, uint64_t* __restrict__ keys) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   keys[i] = ((uint64_t)(uint32_t)color[i] << 32) | (uint64_t)i;
-  atomicAdd(&hist[color[i]], 1);
+}
+
+// start of each color in the color-sorted keys (lower bound of c << 32)
+__global__ void k_color_bounds(const uint64_t* __restrict__ skeys, int64_t n, int32_t* __restrict__ start) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > kMaxColors) return;
+  int64_t lo = 0, hi = n;
+  const uint64_t key = (uint64_t)c << 32;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (skeys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  start[c] = (int32_t)lo;
 }
 
 __global__ void k_low32(const uint64_t* __restrict__ keys, int64_t n, int32_t* __restrict__ out) {
@@ -513,22 +525,22 @@ void build_coloring(nalu_linsys* ls, hipStream_t s) {
     if (h[0] == 0) break;
     NALU_REQUIRE(++rounds < 10000, NALU_ERR_NUMERIC, "coloring did not terminate");
   }
-  DevBuf<int32_t> hist(kMaxColors);
+  DevBuf<int32_t> bounds(kMaxColors + 1);
   DevBuf<uint64_t> keys(n), sorted(n);
-  NALU_HIP_CHECK(hipMemsetAsync(hist.p, 0, sizeof(int32_t) * kMaxColors, s));
-  k_color_keys<<<grid_for(n), kBlock, 0, s>>>(color.p, n, keys.p, hist.p);
+  k_color_keys<<<grid_for(n), kBlock, 0, s>>>(color.p, n, keys.p);
   sort_u64(keys, sorted, n, 32 + 8, s);
   ls->color_rows.alloc(n);
   k_low32<<<grid_for(n), kBlock, 0, s>>>(sorted.p, n, ls->color_rows.p);
-  std::vector<int32_t> h(kMaxColors);
-  NALU_HIP_CHECK(hipMemcpyAsync(h.data(), hist.p, sizeof(int32_t) * kMaxColors, hipMemcpyDeviceToHost, s));
+  k_color_bounds<<<1, kMaxColors + 1, 0, s>>>(sorted.p, n, bounds.p);
+  std::vector<int32_t> h(kMaxColors + 1);
+  NALU_HIP_CHECK(hipMemcpyAsync(h.data(), bounds.p, sizeof(int32_t) * (kMaxColors + 1), hipMemcpyDeviceToHost, s));
   NALU_HIP_CHECK(hipStreamSynchronize(s));
   int nc = 0;
   for (int c = 0; c < kMaxColors; ++c)
-    if (h[c] > 0) nc = c + 1;
+    if (h[c + 1] > h[c]) nc = c + 1;
   ls->num_colors = nc;
   ls->color_ptr_host.assign(nc + 1, 0);
-  for (int c = 0; c < nc; ++c) ls->color_ptr_host[c + 1] = ls->color_ptr_host[c] + h[c];
+  for (int c = 0; c <= nc; ++c) ls->color_ptr_host[c] = h[c];
 }
 
 }  // namespace nalu

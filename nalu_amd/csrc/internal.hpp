@@ -152,6 +152,7 @@ struct nalu_linsys {
   nalu::DevBuf<double> lval, uval, dval, pinvd;
   nalu::DevBuf<float> lval32, uval32, dval32, pinvd32;
   int max_l_len = 0, max_u_len = 0;
+  nalu::DevBuf<double> pwork;  // permuted in/out vectors of precond_apply
 
   // learned preconditioner (optional)
   void* learned = nullptr;

@@ -17,6 +17,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -424,6 +426,75 @@ __global__ __launch_bounds__(kThreads) void k_bwd_color(const int32_t* __restric
   }
 }
 
+// "Vector" variants: G lanes cooperate on a row, no LDS staging and no barrier.
+template <typename T, int G>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <typename T, typename TY, int G>
+__global__ __launch_bounds__(kThreads) void k_split_spmv_vec(const int32_t* __restrict__ lrp, const int32_t* __restrict__ lci,
+                                                             const T* __restrict__ lv, const int32_t* __restrict__ urp,
+                                                             const int32_t* __restrict__ uci, const T* __restrict__ uv,
+                                                             const T* __restrict__ dv, const T* __restrict__ x,
+                                                             TY* __restrict__ y, const TY* __restrict__ b, int64_t n,
+                                                             int mode) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x % G;
+  if (r >= n) return;  // whole groups exit together
+  T s = T(0);
+  for (int32_t q = lrp[r] + lane; q < lrp[r + 1]; q += G) s += lv[q] * x[lci[q]];
+  for (int32_t q = urp[r] + lane; q < urp[r + 1]; q += G) s += uv[q] * x[uci[q]];
+  s = group_sum<T, G>(s);
+  if (lane == 0) {
+    s += dv[r] * x[r];
+    y[r] = mode ? (TY)((T)b[r] - s) : (TY)s;
+  }
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__(kThreads) void k_fwd_color_vec(const int32_t* __restrict__ lrp, const int32_t* __restrict__ lci,
+                                                            const T* __restrict__ lv, const T* __restrict__ invd,
+                                                            const T* __restrict__ r, T* __restrict__ z, int64_t c0,
+                                                            int64_t c1) {
+  const int64_t k = c0 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x % G;
+  if (k >= c1) return;
+  T s = T(0);
+  for (int32_t q = lrp[k] + lane; q < lrp[k + 1]; q += G) s += lv[q] * z[lci[q]];
+  s = group_sum<T, G>(s);
+  if (lane == 0) z[k] = (r[k] - s) * invd[k];
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__(kThreads) void k_bwd_color_vec(const int32_t* __restrict__ urp, const int32_t* __restrict__ uci,
+                                                            const T* __restrict__ uv, const T* __restrict__ invd,
+                                                            T* __restrict__ z, int64_t c0, int64_t c1) {
+  const int64_t k = c0 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x % G;
+  if (k >= c1) return;
+  T s = T(0);
+  for (int32_t q = urp[k] + lane; q < urp[k + 1]; q += G) s += uv[q] * z[uci[q]];
+  s = group_sum<T, G>(s);
+  if (lane == 0) z[k] = z[k] - s * invd[k];
+}
+
+// Sparse kernel family used by the solvers: 0 = CSR-stream (LDS staged),
+// 4 / 8 = G lanes per row.  NALU_SPARSE_KERNEL=stream|vector4|vector8 overrides.
+int sparse_kernel_choice() {
+  static int choice = -1;
+  if (choice < 0) {
+    const char* e = std::getenv("NALU_SPARSE_KERNEL");
+    choice = 8;
+    if (e && std::string(e) == "stream") choice = 0;
+    else if (e && std::string(e) == "vector4") choice = 4;
+    else if (e && std::string(e) == "vector8") choice = 8;
+  }
+  return choice;
+}
+
 int stream_rows(int maxlen, size_t elem_bytes, int parts) {
   int R = 256;
   while (R > 8 && (size_t)R * std::max(1, maxlen) * elem_bytes * parts > 56 * 1024) R >>= 1;
@@ -505,6 +576,17 @@ Split<T> split_of(nalu_linsys* ls) {
 template <typename T, typename TY>
 void pspmv(nalu_linsys* ls, const T* x, TY* y, const TY* b, int mode) {
   const Split<T> A = split_of<T>(ls);
+  const int kc = sparse_kernel_choice();
+  if (kc == 8 || kc == 4) {
+    const int64_t n = ls->nrows;
+    if (kc == 8)
+      hipLaunchKernelGGL((k_split_spmv_vec<T, TY, 8>), dim3(nblocks(n * 8)), dim3(kThreads), 0, ls->stream, A.lrp,
+                         A.lci, A.lv, A.urp, A.uci, A.uv, A.dv, x, y, b, n, mode);
+    else
+      hipLaunchKernelGGL((k_split_spmv_vec<T, TY, 4>), dim3(nblocks(n * 4)), dim3(kThreads), 0, ls->stream, A.lrp,
+                         A.lci, A.lv, A.urp, A.uci, A.uv, A.dv, x, y, b, n, mode);
+    return;
+  }
   const int R = stream_rows(std::max(ls->max_l_len, ls->max_u_len), sizeof(T), 2);
   const int lcap = R * std::max(1, ls->max_l_len);
   const size_t lds = (size_t)(lcap + R * std::max(1, ls->max_u_len)) * sizeof(T);
@@ -520,6 +602,30 @@ void psgs(nalu_linsys* ls, const T* r, T* z) {
   const size_t ldl = (size_t)Rl * std::max(1, ls->max_l_len) * sizeof(T);
   const size_t ldu = (size_t)Ru * std::max(1, ls->max_u_len) * sizeof(T);
   const auto& cp = ls->color_ptr_host;
+  const int kc = sparse_kernel_choice();
+  if (kc == 8 || kc == 4) {
+    for (int c = 0; c < ls->num_colors; ++c) {
+      const int64_t c0 = cp[c], c1 = cp[c + 1];
+      if (c1 <= c0) continue;
+      if (kc == 8)
+        hipLaunchKernelGGL((k_fwd_color_vec<T, 8>), dim3(nblocks((c1 - c0) * 8)), dim3(kThreads), 0, s, A.lrp, A.lci,
+                           A.lv, A.invd, r, z, c0, c1);
+      else
+        hipLaunchKernelGGL((k_fwd_color_vec<T, 4>), dim3(nblocks((c1 - c0) * 4)), dim3(kThreads), 0, s, A.lrp, A.lci,
+                           A.lv, A.invd, r, z, c0, c1);
+    }
+    for (int c = ls->num_colors - 2; c >= 0; --c) {
+      const int64_t c0 = cp[c], c1 = cp[c + 1];
+      if (c1 <= c0) continue;
+      if (kc == 8)
+        hipLaunchKernelGGL((k_bwd_color_vec<T, 8>), dim3(nblocks((c1 - c0) * 8)), dim3(kThreads), 0, s, A.urp, A.uci,
+                           A.uv, A.invd, z, c0, c1);
+      else
+        hipLaunchKernelGGL((k_bwd_color_vec<T, 4>), dim3(nblocks((c1 - c0) * 4)), dim3(kThreads), 0, s, A.urp, A.uci,
+                           A.uv, A.invd, z, c0, c1);
+    }
+    return;
+  }
   for (int c = 0; c < ls->num_colors; ++c) {
     const int64_t c0 = cp[c], c1 = cp[c + 1];
     if (c1 > c0)
@@ -774,17 +880,19 @@ void precond_setup(nalu_linsys* ls) {
   ls->precond_ready = true;
 }
 
-// M^-1 r in the original row order (permutes in and out)
+// M^-1 r in the original row order (permutes in and out through two
+// persistent work vectors of the handle)
 void precond_apply(nalu_linsys* ls, const double* r, double* z) {
   NALU_REQUIRE(ls->precond_ready, NALU_ERR_STATE, "preconditioner not set up");
   const int64_t n = ls->nrows;
-  DevBuf<double> rp(n), zp(n);
+  if (ls->pwork.n != (size_t)(2 * n)) ls->pwork.alloc(2 * n);
+  double* rp = ls->pwork.p;
+  double* zp = rp + n;
   hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, ls->stream, r, ls->perm.p, n,
-                     rp.p);
-  pprec<double>(ls, rp.p, zp.p);
-  hipLaunchKernelGGL((k_scatter<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, ls->stream, zp.p, ls->perm.p,
+                     rp);
+  pprec<double>(ls, rp, zp);
+  hipLaunchKernelGGL((k_scatter<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, ls->stream, zp, ls->perm.p,
                      n, z);
-  NALU_HIP_CHECK(hipStreamSynchronize(ls->stream));  // rp/zp are released on return
 }
 
 // Solve with the configured method; x holds the initial guess (original order).
@@ -876,6 +984,35 @@ extern "C" int nalu_linsys_remove_rhs_mean(nalu_linsys* ls) {
     hipLaunchKernelGGL(nalu::k_sub_mean, dim3(nalu::nblocks(ls->nrows)), dim3(nalu::kThreads), 0, ls->stream,
                        ls->rhs.p, ls->nrows, ls->red_out.p + 255);
     NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_linsys_precond_apply_permuted(nalu_linsys* ls, const double* r, double* z) {
+  return nalu::guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized && r && z, NALU_ERR_ARG, "precond_apply_permuted: bad arguments");
+    NALU_REQUIRE(ls->precond_ready, NALU_ERR_STATE, "precond_apply_permuted: preconditioner not set up");
+    nalu::pprec<double>(ls, r, z);
+    NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_linsys_spmv_permuted(nalu_linsys* ls, const double* x, double* y) {
+  return nalu::guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized && x && y, NALU_ERR_ARG, "spmv_permuted: bad arguments");
+    NALU_REQUIRE(ls->precond_ready, NALU_ERR_STATE, "spmv_permuted: solver copy not set up");
+    nalu::pspmv<double, double>(ls, x, y, (const double*)nullptr, 0);
+    NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_linsys_permutation(nalu_linsys* ls, int32_t* perm_host) {
+  return nalu::guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized && perm_host, NALU_ERR_ARG, "permutation: bad arguments");
+    nalu::build_perm(ls);
+    NALU_HIP_CHECK(hipMemcpy(perm_host, ls->perm.p, sizeof(int32_t) * ls->nrows, hipMemcpyDeviceToHost));
     return NALU_OK;
   });
 }
