@@ -203,6 +203,49 @@ int nalu_assemble_momentum(nalu_linsys *ls, const nalu_mesh *m, const double *u_
                            const double *mu_dev, const double *mdot_dev, double include_divu,
                            int skew_symmetric, int shifted_grad_op, int mode);
 
+/* ------------------------------------------------- learned preconditioner
+ * No counterpart in the reference: a message-passing network over the matrix
+ * graph (BF16 weights, FP32 accumulation, hidden width 32) used inside FGMRES
+ * when the config's precond is NALU_PREC_LEARNED, with multicolor SGS as the
+ * automatic fallback (no network attached, a stagnating cycle, or a
+ * non-finite output).  Scalar (numDof 1) systems only. */
+typedef struct nalu_gnn nalu_gnn;
+typedef struct nalu_gnn_data nalu_gnn_data;
+/* Seeded network whose correction starts at zero (its output is then SGS(r)). */
+int nalu_gnn_create(int num_layers, int hidden, uint64_t seed, nalu_gnn **out);
+int nalu_gnn_destroy(nalu_gnn *g);
+int64_t nalu_gnn_num_params(const nalu_gnn *g);
+int nalu_gnn_num_layers(const nalu_gnn *g);
+/* FP32 master parameters (the kernels read their BF16 rounding). */
+int nalu_gnn_get_params(const nalu_gnn *g, float *params_host, int64_t n);
+int nalu_gnn_set_params(nalu_gnn *g, const float *params_host, int64_t n);
+int nalu_gnn_get_grad(const nalu_gnn *g, float *grad_host, int64_t n);
+/* Attach (g != NULL) or detach the network used by a system's solve. */
+int nalu_linsys_set_learned_precond(nalu_linsys *ls, nalu_gnn *g);
+/* 1 when the last solve fell back from the network to SGS. */
+int nalu_linsys_learned_fallback(const nalu_linsys *ls);
+/* z = net(r) for one right-hand side, FP32 device arrays in row order; needs a
+ * set-up preconditioner (nalu_linsys_precond_setup or a solve). */
+int nalu_gnn_apply(nalu_gnn *g, nalu_linsys *ls, const float *r_dev, float *z_dev);
+/* Loss L = mean_b [1 - <out,r>^2 / (<out,A out> <e,r>)] and its parameter
+ * gradient for `batch` pairs r_dev/e_dev [batch][rows] (FP32, row order);
+ * optional outputs: the SGS feature zs_dev and the raw output out_dev. */
+int nalu_gnn_loss_grad(nalu_gnn *g, nalu_linsys *ls, int batch, const float *r_dev, const float *e_dev,
+                       float *zs_dev, float *out_dev, double *loss_host, float *grad_host);
+/* Harvest num_rhs * pairs_per_rhs (residual, error) pairs from plain FP64 CG +
+ * SGS solves (to relative residual tol) of the system's assembled matrix with
+ * seeded unit-variance Gaussian zero-mean right-hand sides. */
+int nalu_gnn_harvest(nalu_linsys *ls, int num_rhs, int pairs_per_rhs, uint64_t seed, double tol,
+                     int max_iterations, nalu_gnn_data **out);
+int nalu_gnn_data_destroy(nalu_gnn_data *d);
+int nalu_gnn_data_size(const nalu_gnn_data *d);
+int nalu_gnn_data_cg_iterations(const nalu_gnn_data *d, int *iters_host, int count);
+int nalu_gnn_data_pair(const nalu_gnn_data *d, int idx, float *r_dev, float *e_dev);
+/* Adam (cosine learning-rate decay to lr/10) on the pairs harvested from the
+ * same system; epoch_loss_host[epochs] receives the mean loss per epoch. */
+int nalu_gnn_train(nalu_gnn *g, nalu_linsys *ls, const nalu_gnn_data *d, int epochs, int batch, double lr,
+                   uint64_t seed, double *epoch_loss_host);
+
 #ifdef __cplusplus
 }
 #endif

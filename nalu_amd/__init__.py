@@ -26,7 +26,7 @@ import numpy as np
 __all__ = [
     "lib", "available", "NaluError", "SolverConfig", "Mesh", "LinearSystem", "lattice", "normal_field",
     "lognormal_field", "assemble_scalar_diff", "compute_mdot", "assemble_continuity", "assemble_momentum",
-    "METHOD", "PRECOND", "PRECISION",
+    "METHOD", "PRECOND", "PRECISION", "LearnedPreconditioner", "HarvestedPairs",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -107,6 +107,26 @@ def lib():
         L.nalu_assemble_continuity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int,
                                                C.c_int, C.c_int]
         L.nalu_assemble_momentum.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_int]
+        # learned preconditioner
+        _fp = C.POINTER(C.c_float)
+        L.nalu_gnn_create.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp)]
+        L.nalu_gnn_destroy.argtypes = [_vp]
+        L.nalu_gnn_num_params.argtypes = [_vp]
+        L.nalu_gnn_num_params.restype = C.c_int64
+        L.nalu_gnn_num_layers.argtypes = [_vp]
+        L.nalu_gnn_get_params.argtypes = [_vp, _fp, _i64]
+        L.nalu_gnn_set_params.argtypes = [_vp, _fp, _i64]
+        L.nalu_gnn_get_grad.argtypes = [_vp, _fp, _i64]
+        L.nalu_gnn_apply.argtypes = [_vp, _vp, _vp, _vp]
+        L.nalu_gnn_loss_grad.argtypes = [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _dp, _fp]
+        L.nalu_gnn_harvest.argtypes = [_vp, C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_int, C.POINTER(_vp)]
+        L.nalu_gnn_data_destroy.argtypes = [_vp]
+        L.nalu_gnn_data_size.argtypes = [_vp]
+        L.nalu_gnn_data_cg_iterations.argtypes = [_vp, C.POINTER(C.c_int), C.c_int]
+        L.nalu_gnn_data_pair.argtypes = [_vp, C.c_int, _vp, _vp]
+        L.nalu_gnn_train.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
+        L.nalu_linsys_set_learned_precond.argtypes = [_vp, _vp]
+        L.nalu_linsys_learned_fallback.argtypes = [_vp]
         _L = L
     return _L
 
@@ -380,6 +400,107 @@ class LinearSystem:
 
     def set_l2_scaling(self, s):
         _check(lib().nalu_linsys_set_l2_scaling(self._h, float(s)), "set_l2_scaling")
+
+    def set_learned_preconditioner(self, net):
+        """Attach a LearnedPreconditioner (used when the config's preconditioner is
+        "learned", inside FGMRES, with SGS as the fallback); None detaches."""
+        self._net = net
+        _check(lib().nalu_linsys_set_learned_precond(self._h, None if net is None else net.handle),
+               "set_learned_precond")
+
+    def learned_fallback(self):
+        """True when the last solve fell back from the network to SGS."""
+        return bool(_check(lib().nalu_linsys_learned_fallback(self._h), "learned_fallback"))
+
+
+# ------------------------------------------------------------------ learned preconditioner
+class HarvestedPairs:
+    """(residual, error) pairs of plain FP64 CG + SGS solves, held on the device."""
+
+    def __init__(self, ls, num_rhs, pairs_per_rhs, seed=1, tol=1e-10, max_iterations=5000):
+        h = _vp()
+        _check(lib().nalu_gnn_harvest(ls.handle, int(num_rhs), int(pairs_per_rhs), int(seed), float(tol),
+                                      int(max_iterations), C.byref(h)), "gnn_harvest")
+        self._h = h
+        self.ls = ls
+        self.num_rhs = num_rhs
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return _check(lib().nalu_gnn_data_size(self._h), "gnn_data_size")
+
+    def cg_iterations(self):
+        it = (C.c_int * self.num_rhs)()
+        _check(lib().nalu_gnn_data_cg_iterations(self._h, it, self.num_rhs), "gnn_data_cg_iterations")
+        return list(it)
+
+    def pair(self, idx, r, e):
+        """Copy pair idx (original row order) into float32 device tensors r, e."""
+        _check(lib().nalu_gnn_data_pair(self._h, int(idx), _ptr(r), _ptr(e)), "gnn_data_pair")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _L is not None:
+            _L.nalu_gnn_data_destroy(h)
+            self._h = None
+
+
+class LearnedPreconditioner:
+    """Message-passing network over the matrix graph (BF16 weights, FP32
+    accumulation) trained and applied by the HIP kernels of libnalu_hip.so."""
+
+    def __init__(self, num_layers=4, hidden=32, seed=0):
+        h = _vp()
+        _check(lib().nalu_gnn_create(int(num_layers), int(hidden), int(seed), C.byref(h)), "gnn_create")
+        self._h = h
+        self.num_layers = num_layers
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_params(self):
+        return int(lib().nalu_gnn_num_params(self._h))
+
+    def get_params(self):
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        _check(lib().nalu_gnn_get_params(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), out.shape[0]),
+               "gnn_get_params")
+        return out
+
+    def set_params(self, th):
+        th = _np(th, np.float32)
+        _check(lib().nalu_gnn_set_params(self._h, th.ctypes.data_as(C.POINTER(C.c_float)), th.shape[0]),
+               "gnn_set_params")
+
+    def apply(self, ls, r, z):
+        """z = network(r) for float32 device tensors in the original row order."""
+        _check(lib().nalu_gnn_apply(self._h, ls.handle, _ptr(r), _ptr(z)), "gnn_apply")
+
+    def loss_grad(self, ls, r, e, zs=None, out=None):
+        """Loss and parameter gradient of one batch (r, e: float32 device tensors [B, n])."""
+        batch = int(r.shape[0]) if r.dim() == 2 else 1
+        loss = C.c_double(0)
+        g = np.zeros(self.num_params(), dtype=np.float32)
+        _check(lib().nalu_gnn_loss_grad(self._h, ls.handle, batch, _ptr(r), _ptr(e), _ptr(zs), _ptr(out),
+                                        C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))), "gnn_loss_grad")
+        return loss.value, g
+
+    def train(self, ls, pairs, epochs=4, batch=16, lr=3e-3, seed=0):
+        """Adam on the harvested pairs; returns the mean loss of each epoch."""
+        hist = np.zeros(int(epochs), dtype=np.float64)
+        _check(lib().nalu_gnn_train(self._h, ls.handle, pairs.handle, int(epochs), int(batch), float(lr), int(seed),
+                                    hist.ctypes.data_as(_dp)), "gnn_train")
+        return hist
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _L is not None:
+            _L.nalu_gnn_destroy(h)
+            self._h = None
 
 
 # ------------------------------------------------------------------ assembly algorithms

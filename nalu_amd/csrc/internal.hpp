@@ -151,6 +151,7 @@ struct nalu_linsys {
   nalu::DevBuf<int32_t> lrp, lci, lsrc, urp, uci, usrc, dsrc;
   nalu::DevBuf<double> lval, uval, dval, pinvd;
   nalu::DevBuf<float> lval32, uval32, dval32, pinvd32;
+  bool split32_ready = false;
   int max_l_len = 0, max_u_len = 0;
   nalu::DevBuf<double> pwork;  // permuted in/out vectors of precond_apply
 
@@ -194,4 +195,17 @@ void precond_apply(nalu_linsys* ls, const double* r, double* z);
 void spmv(nalu_linsys* ls, const double* x, double* y);
 int krylov_solve(nalu_linsys* ls, const double* b, double* x, int* iters, double* final_rel_resid);
 double norm2(nalu_linsys* ls, const double* x, int64_t n);
+// solver-order (color-permuted) building blocks, krylov.hip
+void perm_sgs32(nalu_linsys* ls, const float* r, float* z);
+void perm_sgs64(nalu_linsys* ls, const double* r, double* z);
+void perm_spmv64(nalu_linsys* ls, const double* x, double* y);
+void ensure_split32(nalu_linsys* ls);
+// learned.hip
+void learned_setup(nalu_linsys* ls);
+bool learned_active(const nalu_linsys* ls);
+bool learned_fallback(const nalu_linsys* ls);
+void learned_set_fallback(nalu_linsys* ls, bool on);
+void learned_apply_f32(nalu_linsys* ls, const float* r, float* z);
+void learned_apply_f64(nalu_linsys* ls, const double* r, double* z);
+void learned_release(nalu_linsys* ls);
 }  // namespace nalu

@@ -650,8 +650,15 @@ void pprec(nalu_linsys* ls, const T* r, T* z) {
     case NALU_PREC_JACOBI:
       hipLaunchKernelGGL(k_jacobi<T>, dim3(nblocks(n)), dim3(kThreads), 0, ls->stream, split_of<T>(ls).invd, r, z, n);
       break;
-    case NALU_PREC_SGS:
     case NALU_PREC_LEARNED:
+      if (learned_active(ls)) {
+        if constexpr (std::is_same<T, float>::value) learned_apply_f32(ls, r, z);
+        else learned_apply_f64(ls, r, z);
+        break;
+      }
+      psgs<T>(ls, r, z);  // no network attached, or fallen back
+      break;
+    case NALU_PREC_SGS:
       psgs<T>(ls, r, z);
       break;
     default:
@@ -723,6 +730,9 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
   TV* tmp = Z + (size_t)n * (flexible ? m : 1);
   TV* xt = tmp + n;  // x in TV precision for the residual of the inner SpMV path
   DevBuf<double> r64(n), hdev(m + 2), h2(m + 2);
+  // learned preconditioner: FGMRES falls back to SGS for the rest of the solve
+  // when a cycle stagnates or the network output is not finite
+  const bool watch = flexible && ls->cfg.precond == NALU_PREC_LEARNED;
   Hessenberg H(m);
   std::vector<double> hcol(m + 2), y;
   PinnedDoubles hpin(m + 2);
@@ -745,6 +755,7 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
     H.reset(beta);
     int k = 0;
     bool done = false;
+    const bool learned_cycle = watch && learned_active(ls);
     for (; k < m && iters < maxit; ++k) {
       ++iters;
       TV* vk = V + (size_t)n * k;
@@ -765,6 +776,10 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
       NALU_HIP_CHECK(hipStreamSynchronize(s));
       for (int i = 0; i <= k; ++i) hcol[i] = hpin.p[i];
       hcol[k + 1] = std::sqrt(hpin.p[k + 1]);
+      if (learned_cycle && !std::isfinite(hcol[k + 1])) {
+        learned_set_fallback(ls, true);  // drop this direction, finish the cycle with k columns
+        break;
+      }
       NALU_REQUIRE(std::isfinite(hcol[k + 1]), NALU_ERR_NUMERIC, "gmres: non-finite Arnoldi vector");
       const double res = H.add_column(k, hcol.data());
       rel = res / r0;
@@ -789,6 +804,8 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
     }
     NALU_HIP_CHECK(hipStreamSynchronize(s));  // hpin is reused by the next cycle
     if (done) break;
+    // stagnation: a full cycle that did not reduce the residual by 2x
+    if (learned_cycle && k >= std::min(m, 10) && rel * r0 > 0.5 * beta) learned_set_fallback(ls, true);
   }
   if (implicit_rel) *implicit_rel = rel;
   return iters;
@@ -844,6 +861,35 @@ double norm2(nalu_linsys* ls, const double* x, int64_t n) {
   return std::sqrt(red.dot(x, x, n));
 }
 
+void perm_sgs32(nalu_linsys* ls, const float* r, float* z) { psgs<float>(ls, r, z); }
+void perm_sgs64(nalu_linsys* ls, const double* r, double* z) { psgs<double>(ls, r, z); }
+void perm_spmv64(nalu_linsys* ls, const double* x, double* y) {
+  pspmv<double, double>(ls, x, y, (const double*)nullptr, 0);
+}
+
+// FP32 copy of the permuted split values (mixed-precision inner solve, and the
+// SGS input feature of the learned preconditioner)
+void ensure_split32(nalu_linsys* ls) {
+  NALU_REQUIRE(ls->precond_ready, NALU_ERR_STATE, "FP32 split copy: preconditioner not set up");
+  if (ls->split32_ready) return;
+  hipStream_t s = ls->stream;
+  const int64_t n = ls->nrows, nl = ls->lci.n, nu = ls->uci.n;
+  if (ls->lval32.n != (size_t)nl) ls->lval32.alloc(nl);
+  if (ls->uval32.n != (size_t)nu) ls->uval32.alloc(nu);
+  if (ls->dval32.n != (size_t)n) ls->dval32.alloc(n);
+  if (ls->pinvd32.n != (size_t)n) ls->pinvd32.alloc(n);
+  DevBuf<int> bad(1);
+  NALU_HIP_CHECK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_take<float>, dim3(nblocks(nl)), dim3(kThreads), 0, s, ls->lsrc.p, ls->values.p, nl,
+                     ls->lval32.p);
+  hipLaunchKernelGGL(k_take<float>, dim3(nblocks(nu)), dim3(kThreads), 0, s, ls->usrc.p, ls->values.p, nu,
+                     ls->uval32.p);
+  hipLaunchKernelGGL(k_take_diag<float>, dim3(nblocks(n)), dim3(kThreads), 0, s, ls->dsrc.p, ls->values.p, n,
+                     ls->dval32.p, ls->pinvd32.p, bad.p);
+  NALU_HIP_CHECK(hipStreamSynchronize(s));
+  ls->split32_ready = true;
+}
+
 // Ifpack2 compute(): refresh the permuted split copy of the current values.
 void precond_setup(nalu_linsys* ls) {
   hipStream_t s = ls->stream;
@@ -861,23 +907,14 @@ void precond_setup(nalu_linsys* ls) {
   hipLaunchKernelGGL(k_take<double>, dim3(nblocks(nu)), dim3(kThreads), 0, s, ls->usrc.p, ls->values.p, nu, ls->uval.p);
   hipLaunchKernelGGL(k_take_diag<double>, dim3(nblocks(n)), dim3(kThreads), 0, s, ls->dsrc.p, ls->values.p, n,
                      ls->dval.p, ls->pinvd.p, bad.p);
-  if (mixed(ls)) {
-    if (ls->lval32.n != (size_t)nl) ls->lval32.alloc(nl);
-    if (ls->uval32.n != (size_t)nu) ls->uval32.alloc(nu);
-    if (ls->dval32.n != (size_t)n) ls->dval32.alloc(n);
-    if (ls->pinvd32.n != (size_t)n) ls->pinvd32.alloc(n);
-    hipLaunchKernelGGL(k_take<float>, dim3(nblocks(nl)), dim3(kThreads), 0, s, ls->lsrc.p, ls->values.p, nl,
-                       ls->lval32.p);
-    hipLaunchKernelGGL(k_take<float>, dim3(nblocks(nu)), dim3(kThreads), 0, s, ls->usrc.p, ls->values.p, nu,
-                       ls->uval32.p);
-    hipLaunchKernelGGL(k_take_diag<float>, dim3(nblocks(n)), dim3(kThreads), 0, s, ls->dsrc.p, ls->values.p, n,
-                       ls->dval32.p, ls->pinvd32.p, bad.p);
-  }
   int hb = 0;
   NALU_HIP_CHECK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
   NALU_HIP_CHECK(hipStreamSynchronize(s));
   NALU_REQUIRE(hb == 0, NALU_ERR_NUMERIC, "preconditioner: zero or non-finite diagonal");
   ls->precond_ready = true;
+  ls->split32_ready = false;
+  if (mixed(ls) || ls->learned) ensure_split32(ls);
+  learned_setup(ls);
 }
 
 // M^-1 r in the original row order (permutes in and out through two
@@ -909,6 +946,7 @@ int krylov_solve(nalu_linsys* ls, const double* b, double* x, int* iters_out, do
   hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, x, ls->perm.p, n, xp.p);
   const double bn = std::sqrt(red.dot(bp.p, bp.p, n));
   NALU_REQUIRE(std::isfinite(bn), NALU_ERR_NUMERIC, "solve: non-finite right-hand side");
+  learned_set_fallback(ls, false);
   int iters = 0;
   if (!mixed(ls)) {
     if (c.method == NALU_METHOD_CG) iters = cg(ls, bp.p, xp.p, maxit, tol);

@@ -127,6 +127,7 @@ extern "C" int nalu_linsys_destroy(nalu_linsys* ls) {
   if (ls->stream) (void)hipStreamSynchronize(ls->stream);
   else (void)hipDeviceSynchronize();
   flush_timers(ls);
+  learned_release(ls);
   delete ls;
   return NALU_OK;
 }
