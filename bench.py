@@ -41,6 +41,11 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--no-solver-compare", action="store_true",
+                    help="skip the FP64 GMRES vs mixed + learned FGMRES pressure-solve comparison")
+    ap.add_argument("--train-n", type=int, default=64, help="mesh of the learned preconditioner's training pairs")
+    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--solve-reps", type=int, default=3)
     return ap.parse_args()
 
 
@@ -79,6 +84,95 @@ def cpu_baseline(n_cpu):
                       "%.2f s; CG + sequential SGS pressure solve to 1e-8: %d iters, %.2f s, resid %.1e"
                       % (n_cpu, E, t_asm, it, t_solve, res),
             "pressure_solve_s": t_solve, "pressure_iters": it}
+
+
+def solver_compare(args, na, torch, mesh, P, devc):
+    """Pressure-Poisson solve to 1e-8 FP64 true relative residual on the same
+    assembled continuity system: plain FP64 GMRES(50) + SGS against mixed
+    precision FGMRES(50) with the learned preconditioner (trained on pairs
+    harvested from FP64 CG + SGS on a train_n^3 mesh with lognormal(0, 0.3)
+    density), plus the mixed FGMRES + SGS ablation."""
+    from nalu_amd.problems import lowmach_problem
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
+    # training system and pairs
+    Pt = lowmach_problem(args.train_n, rho_sigma=0.3, seed=4321)
+    mt = na.Mesh(Pt["coords"], Pt["conn"])
+    lt = na.LinearSystem(1, na.SolverConfig("cg", "sgs"))
+    lt.buildElemToNodeGraph(mt)
+    lt.finalizeLinearSystem()
+    lt.zeroSystem()
+    na.assemble_continuity(lt, mt, t(Pt["rho"]), t(Pt["p"]), t(Pt["u"]), t(Pt["gpdx"]), tau=1.0, mode="gather")
+    lt.loadComplete()
+    lt.precond_setup()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pairs = na.HarvestedPairs(lt, num_rhs=40, pairs_per_rhs=50, seed=1, tol=1e-10)
+    torch.cuda.synchronize()
+    t_harvest = time.perf_counter() - t0
+    print("bench: harvested %d pairs at %d^3 in %.2f s" % (len(pairs), args.train_n, t_harvest),
+          file=sys.stderr, flush=True)
+    net = na.LearnedPreconditioner(num_layers=4, hidden=32, seed=2)
+    t0 = time.perf_counter()
+    hist = net.train(lt, pairs, epochs=args.train_epochs, batch=16, lr=3e-3, seed=1)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    print("bench: trained %d epochs in %.2f s, losses %s" % (args.train_epochs, t_train, np.round(hist, 4).tolist()),
+          file=sys.stderr, flush=True)
+    cg_iters = pairs.cg_iterations()
+    del pairs, lt, mt
+
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    res = {}
+    sols = {}
+    for name, method, prec, precision in (("fp64_gmres_sgs", "gmres", "sgs", "fp64"),
+                                          ("mixed_fgmres_learned", "fgmres", "learned", "mixed"),
+                                          ("mixed_fgmres_sgs", "fgmres", "sgs", "mixed")):
+        ls = na.LinearSystem(1, na.SolverConfig(method, prec, tolerance=1e-8, max_iterations=5000, kspace=50,
+                                                precision=precision))
+        ls.buildElemToNodeGraph(mesh)
+        ls.finalizeLinearSystem()
+        ls.set_stream(torch.cuda.current_stream().cuda_stream)
+        ls.zeroSystem()
+        na.assemble_continuity(ls, mesh, t(P["rho"]), t(P["p"]), t(P["u"]), t(P["gpdx"]), tau=P["tau"],
+                               mode="gather")
+        ls.loadComplete()
+        ls.removeRhsMean()
+        if prec == "learned":
+            ls.set_learned_preconditioner(net)
+        x = torch.zeros(ls.sizes()[0], dtype=torch.float64, device=devc)
+        times = []
+        for k in range(1 + args.solve_reps):
+            x.zero_()
+            ls.zeroSystem()
+            na.assemble_continuity(ls, mesh, t(P["rho"]), t(P["p"]), t(P["u"]), t(P["gpdx"]), tau=P["tau"],
+                                   mode="gather")
+            ls.loadComplete()
+            ls.removeRhsMean()
+            torch.cuda.synchronize()
+            a, b = ev(), ev()
+            a.record()
+            st = ls.solve(x)
+            b.record()
+            torch.cuda.synchronize()
+            if k > 0:
+                times.append(a.elapsed_time(b) * 1e-3)
+        rel = ls.linearResidual() / max(ls.nonLinearResidual(), 1e-300)
+        res[name] = {"solve_s": float(np.median(times)), "iters": ls.linearSolveIterations(),
+                     "true_rel_resid": rel, "status": st}
+        if prec == "learned":
+            res[name]["fell_back_to_sgs"] = ls.learned_fallback()
+        sols[name] = x - x.mean()
+        print("bench: %s %.4f s, %d iters, resid %.2e" % (name, res[name]["solve_s"], res[name]["iters"], rel),
+              file=sys.stderr, flush=True)
+    ref = sols["fp64_gmres_sgs"]
+    for name in ("mixed_fgmres_learned", "mixed_fgmres_sgs"):
+        res[name]["pressure_rel_diff_vs_fp64"] = float(torch.linalg.norm(sols[name] - ref) / torch.linalg.norm(ref))
+    res["speedup_mixed_learned_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_learned"]["solve_s"]
+    res["learned_training"] = {"train_mesh": "%d^3" % args.train_n, "pairs": 2000, "harvest_s": t_harvest,
+                               "harvest_cg_iters_mean": float(np.mean(cg_iters)), "train_s": t_train,
+                               "epochs": args.train_epochs, "batch": 16, "epoch_loss": hist.tolist(),
+                               "layers": 4, "hidden": 32, "weights": "bf16", "accumulate": "fp32"}
+    return res
 
 
 def main():
@@ -213,6 +307,10 @@ def main():
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
+    compare = None
+    if not args.no_solver_compare:
+        compare = solver_compare(args, na, torch, mesh, P, devc)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle, %d^3 sample)" % args.cpu_n, file=sys.stderr, flush=True)
@@ -241,6 +339,7 @@ def main():
         "pressure_iters": iters,
         "pressure_true_rel_resid": resid,
         "sgs_colors": ncol,
+        "pressure_solve_compare": compare,
         "kernels": {
             k: {"time_s": per_step[k][0], "unit_time_s": {"sgs_apply": t_sgs, "spmv": t_spmv}.get(k, per_step[k][0]),
                 "algorithmic_bytes": byt[k],

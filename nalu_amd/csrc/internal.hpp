@@ -153,6 +153,16 @@ struct nalu_linsys {
   nalu::DevBuf<float> lval32, uval32, dval32, pinvd32;
   bool split32_ready = false;
   int max_l_len = 0, max_u_len = 0;
+  // the same split system in SELL-64 form: chunks of <= 64 consecutive rows of
+  // one color; entry k of row row0 + r of a chunk sits at off + 64 k + r
+  // (column-major in the chunk, padded with zero entries to the chunk's width)
+  bool sell_ready = false;
+  int64_t sell_nchunks = 0;
+  nalu::DevBuf<unsigned char> sell_chunks;  // [nchunks] chunk descriptors (krylov.hip)
+  std::vector<int64_t> sell_color_ptr;      // chunks of color c: [ptr[c], ptr[c+1])
+  nalu::DevBuf<int32_t> sl_col, su_col, sl_src, su_src;
+  nalu::DevBuf<double> sl_val, su_val;
+  nalu::DevBuf<float> sl_val32, su_val32;
   nalu::DevBuf<double> pwork;  // permuted in/out vectors of precond_apply
 
   // learned preconditioner (optional)

@@ -110,6 +110,54 @@ __global__ __launch_bounds__(kThreads) void k_maxpy(const TV* __restrict__ V, in
   w[i] = (TW)((double)w[i] + sign * acc);
 }
 
+// Gram-Schmidt building blocks with one launch per pass over the basis.
+// part[j * gridDim.x + blk] = V_j . w over rows [blk R, (blk+1) R), one
+// wavefront per basis vector.
+template <typename TV, typename TW>
+__global__ __launch_bounds__(kThreads) void k_mdot_rows(const TV* __restrict__ V, int64_t ldv, int cnt,
+                                                        const TW* __restrict__ w, int64_t n, int64_t R,
+                                                        double* __restrict__ part) {
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t r0 = (int64_t)blockIdx.x * R, r1 = min(n, r0 + R);
+  for (int j = wv; j < cnt; j += kThreads / 64) {
+    const TV* v = V + (int64_t)j * ldv;
+    double s = 0.0;
+    for (int64_t i = r0 + l; i < r1; i += 64) s += (double)v[i] * (double)w[i];
+    s = wave_sum(s);
+    if (l == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// w -= V h and part[blk] = partial ||w||^2 of the updated (stored) values
+template <typename TV, typename TW>
+__global__ __launch_bounds__(kThreads) void k_maxpy_norm(const TV* __restrict__ V, int64_t ldv, int cnt,
+                                                         const double* __restrict__ h, TW* __restrict__ w, int64_t n,
+                                                         double* __restrict__ part) {
+  double s2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < cnt; ++j) acc += h[j] * (double)V[(int64_t)j * ldv + i];
+    const TW wn = (TW)((double)w[i] - acc);
+    w[i] = wn;
+    s2 += (double)wn * (double)wn;
+  }
+  s2 = block_sum(s2);
+  if (threadIdx.x == 0) part[blockIdx.x] = s2;
+}
+
+// out[j] = sum of part[j][0..nb); acc[j] += the same sum
+__global__ __launch_bounds__(kThreads) void k_reduce_rows2(const double* __restrict__ part, int nb,
+                                                           double* __restrict__ out, double* __restrict__ acc) {
+  const double* p = part + (int64_t)blockIdx.x * nb;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = s;
+    acc[blockIdx.x] += s;
+  }
+}
+
 // ------------------------------------------------------------ SpMV (CSR-stream)
 // Block owns rows [r0, r0+R); products of its contiguous nnz segment are
 // formed with coalesced loads into LDS, then each row sums its slice.
@@ -224,7 +272,8 @@ struct Red {
   nalu_linsys* ls;
   hipStream_t s;
   explicit Red(nalu_linsys* l) : ls(l), s(l->stream) {
-    if (ls->red.n < (size_t)kRedBlocks * 64) ls->red.alloc((size_t)kRedBlocks * 64);
+    const size_t need = (size_t)kRedBlocks * std::max(64, ls->cfg.kspace + 2);
+    if (ls->red.n < need) ls->red.alloc(need);
     if (ls->red_out.n < 256) ls->red_out.alloc(256);
   }
   // dot into device slot out (not synchronized)
@@ -243,6 +292,19 @@ struct Red {
     return h;
   }
   // out[0..cnt) (+)= V[:, 0..cnt)^T w
+  // one-launch multi-dot: rows split in at most kRedBlocks blocks
+  int64_t rows_per_block(int64_t n) const {
+    const int64_t R = std::max<int64_t>(2048, (n + kRedBlocks - 1) / kRedBlocks);
+    return (R + 63) / 64 * 64;
+  }
+  template <typename TV, typename TW>
+  unsigned mdot_part(const TV* V, int64_t ldv, int cnt, const TW* w, int64_t n) {
+    const int64_t R = rows_per_block(n);
+    const unsigned nb = (unsigned)((n + R - 1) / R);
+    NALU_REQUIRE((size_t)cnt * nb <= ls->red.n, NALU_ERR_STATE, "multi-dot workspace too small");
+    hipLaunchKernelGGL((k_mdot_rows<TV, TW>), dim3(nb), dim3(kThreads), 0, s, V, ldv, cnt, w, n, R, ls->red.p);
+    return nb;
+  }
   template <typename TV, typename TW>
   void mdot_dev(const TV* V, int64_t ldv, int cnt, const TW* w, int64_t n, double* out, int accumulate) {
     const unsigned nb = red_blocks(n);
@@ -481,13 +543,136 @@ __global__ __launch_bounds__(kThreads) void k_bwd_color_vec(const int32_t* __res
   if (lane == 0) z[k] = z[k] - s * invd[k];
 }
 
-// Sparse kernel family used by the solvers: 0 = CSR-stream (LDS staged),
-// 4 / 8 = G lanes per row.  NALU_SPARSE_KERNEL=stream|vector4|vector8 overrides.
+// ------------------------------------------------------------ SELL-64 kernels
+// One wavefront per chunk, one lane per row: every load of column indices and
+// values is a fully coalesced 64-lane access, no reductions or LDS.
+struct SellChunk {
+  int32_t row0, nrows, lw, uw;
+  int64_t loff, uoff;
+};
+
+// chunk widths = longest L / U part of the chunk's rows
+__global__ void k_sell_widths(SellChunk* __restrict__ ch, int64_t nch, const int32_t* __restrict__ lrp,
+                              const int32_t* __restrict__ urp) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nch) return;
+  int lw = 0, uw = 0;
+  for (int r = ch[c].row0; r < ch[c].row0 + ch[c].nrows; ++r) {
+    lw = max(lw, lrp[r + 1] - lrp[r]);
+    uw = max(uw, urp[r + 1] - urp[r]);
+  }
+  ch[c].lw = lw;
+  ch[c].uw = uw;
+}
+
+// column / source tables; padding entries carry no source (value 0).  Their
+// column must hold a value that is already final when the sweep reads it, so
+// that 0 * x stays 0: row 0 (the first color, written first by the forward
+// sweep) for L, the row itself (holding the forward result) for U.
+__global__ __launch_bounds__(kThreads) void k_sell_fill(const SellChunk* __restrict__ ch, int64_t nch,
+                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                        const int32_t* __restrict__ src, int upper,
+                                                        int32_t* __restrict__ col, int32_t* __restrict__ ssrc) {
+  const int64_t c = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= nch) return;
+  const SellChunk k = ch[c];
+  const int w = upper ? k.uw : k.lw;
+  const int64_t off = upper ? k.uoff : k.loff;
+  const bool act = lane < k.nrows;
+  const int row = k.row0 + (act ? lane : 0);
+  const int len = act ? rp[row + 1] - rp[row] : 0;
+  for (int q = 0; q < w; ++q) {
+    const int64_t at = off + (int64_t)q * 64 + lane;
+    if (q < len) {
+      col[at] = ci[rp[row] + q];
+      ssrc[at] = src[rp[row] + q];
+    } else {
+      col[at] = upper ? row : 0;
+      ssrc[at] = -1;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_take_sell(const int32_t* __restrict__ src, const double* __restrict__ v, int64_t n,
+                            T* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = src[q] >= 0 ? (T)v[src[q]] : T(0);
+}
+
+// y = (D + L + U) x  (mode 0)  or  y = b - (D + L + U) x  (mode 1)
+template <typename T, typename TY>
+__global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restrict__ ch, int64_t nch,
+                                                        const int32_t* __restrict__ lcol, const T* __restrict__ lval,
+                                                        const int32_t* __restrict__ ucol, const T* __restrict__ uval,
+                                                        const T* __restrict__ dv, const T* __restrict__ x,
+                                                        TY* __restrict__ y, const TY* __restrict__ b, int mode) {
+  const int64_t c = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= nch) return;
+  const SellChunk k = ch[c];
+  const bool act = lane < k.nrows;
+  const int row = k.row0 + (act ? lane : 0);
+  T s = dv[row] * x[row];
+  const int32_t* lc = lcol + k.loff + lane;
+  const T* lv = lval + k.loff + lane;
+#pragma unroll 4
+  for (int q = 0; q < k.lw; ++q) s += lv[q * 64] * x[lc[q * 64]];
+  const int32_t* uc = ucol + k.uoff + lane;
+  const T* uv = uval + k.uoff + lane;
+#pragma unroll 4
+  for (int q = 0; q < k.uw; ++q) s += uv[q * 64] * x[uc[q * 64]];
+  if (act) y[row] = mode ? (TY)((T)b[row] - s) : (TY)s;
+}
+
+// forward sweep over the chunks [c0, c1) of one color: z = (r - L z) / d
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
+                                                       const int32_t* __restrict__ lcol, const T* __restrict__ lval,
+                                                       const T* __restrict__ invd, const T* __restrict__ r,
+                                                       T* __restrict__ z) {
+  const int64_t c = c0 + (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= c1) return;
+  const SellChunk k = ch[c];
+  if (lane >= k.nrows) return;
+  const int row = k.row0 + lane;
+  T s = T(0);
+  const int32_t* lc = lcol + k.loff + lane;
+  const T* lv = lval + k.loff + lane;
+#pragma unroll 4
+  for (int q = 0; q < k.lw; ++q) s += lv[q * 64] * z[lc[q * 64]];
+  z[row] = (r[row] - s) * invd[row];
+}
+
+// backward sweep over the chunks of one color, in place: z = y - (U z) / d
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
+                                                       const int32_t* __restrict__ ucol, const T* __restrict__ uval,
+                                                       const T* __restrict__ invd, T* __restrict__ z) {
+  const int64_t c = c0 + (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= c1) return;
+  const SellChunk k = ch[c];
+  if (lane >= k.nrows) return;
+  const int row = k.row0 + lane;
+  T s = T(0);
+  const int32_t* uc = ucol + k.uoff + lane;
+  const T* uv = uval + k.uoff + lane;
+#pragma unroll 4
+  for (int q = 0; q < k.uw; ++q) s += uv[q * 64] * z[uc[q * 64]];
+  z[row] = z[row] - s * invd[row];
+}
+
+// Sparse kernel family used by the solvers: 64 = SELL-64 (default), 0 =
+// CSR-stream (LDS staged), 4 / 8 = G lanes per CSR row.
+// NALU_SPARSE_KERNEL=sell|stream|vector4|vector8 overrides.
 int sparse_kernel_choice() {
   static int choice = -1;
   if (choice < 0) {
     const char* e = std::getenv("NALU_SPARSE_KERNEL");
-    choice = 8;
+    choice = 64;
     if (e && std::string(e) == "stream") choice = 0;
     else if (e && std::string(e) == "vector4") choice = 4;
     else if (e && std::string(e) == "vector8") choice = 8;
@@ -559,6 +744,49 @@ void build_perm(nalu_linsys* ls) {
   ls->perm_ready = true;
 }
 
+// SELL-64 tables of the split system (structure only; values are refreshed
+// with the rest of the solver copy).
+void build_sell(nalu_linsys* ls) {
+  if (ls->sell_ready) return;
+  hipStream_t s = ls->stream;
+  const int nc = ls->num_colors;
+  std::vector<SellChunk> ch;
+  ls->sell_color_ptr.assign(nc + 1, 0);
+  for (int c = 0; c < nc; ++c) {
+    ls->sell_color_ptr[c] = (int64_t)ch.size();
+    for (int64_t r = ls->color_ptr_host[c]; r < ls->color_ptr_host[c + 1]; r += 64)
+      ch.push_back({(int32_t)r, (int32_t)std::min<int64_t>(64, ls->color_ptr_host[c + 1] - r), 0, 0, 0, 0});
+  }
+  ls->sell_color_ptr[nc] = (int64_t)ch.size();
+  const int64_t nch = (int64_t)ch.size();
+  ls->sell_nchunks = nch;
+  ls->sell_chunks.alloc(std::max<int64_t>(1, nch) * sizeof(SellChunk));
+  SellChunk* dch = reinterpret_cast<SellChunk*>(ls->sell_chunks.p);
+  NALU_HIP_CHECK(hipMemcpy(dch, ch.data(), nch * sizeof(SellChunk), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_sell_widths, dim3(nblocks(nch)), dim3(kThreads), 0, s, dch, nch, ls->lrp.p, ls->urp.p);
+  NALU_HIP_CHECK(hipMemcpyAsync(ch.data(), dch, nch * sizeof(SellChunk), hipMemcpyDeviceToHost, s));
+  NALU_HIP_CHECK(hipStreamSynchronize(s));
+  int64_t lo = 0, uo = 0;
+  for (auto& k : ch) {
+    k.loff = lo;
+    k.uoff = uo;
+    lo += 64 * (int64_t)k.lw;
+    uo += 64 * (int64_t)k.uw;
+  }
+  NALU_HIP_CHECK(hipMemcpy(dch, ch.data(), nch * sizeof(SellChunk), hipMemcpyHostToDevice));
+  ls->sl_col.alloc(std::max<int64_t>(1, lo));
+  ls->sl_src.alloc(std::max<int64_t>(1, lo));
+  ls->su_col.alloc(std::max<int64_t>(1, uo));
+  ls->su_src.alloc(std::max<int64_t>(1, uo));
+  const unsigned nbw = (unsigned)std::max<int64_t>(1, (nch + 3) / 4);
+  hipLaunchKernelGGL(k_sell_fill, dim3(nbw), dim3(kThreads), 0, s, dch, nch, ls->lrp.p, ls->lci.p, ls->lsrc.p, 0,
+                     ls->sl_col.p, ls->sl_src.p);
+  hipLaunchKernelGGL(k_sell_fill, dim3(nbw), dim3(kThreads), 0, s, dch, nch, ls->urp.p, ls->uci.p, ls->usrc.p, 1,
+                     ls->su_col.p, ls->su_src.p);
+  NALU_HIP_CHECK(hipGetLastError());
+  ls->sell_ready = true;
+}
+
 template <typename T>
 struct Split {
   const int32_t *lrp, *lci, *urp, *uci;
@@ -573,10 +801,23 @@ Split<T> split_of(nalu_linsys* ls) {
     return {ls->lrp.p, ls->lci.p, ls->urp.p, ls->uci.p, ls->lval32.p, ls->uval32.p, ls->dval32.p, ls->pinvd32.p};
 }
 
+template <typename T>
+const T* sell_vals(const nalu_linsys* ls, bool upper) {
+  if constexpr (std::is_same<T, double>::value) return upper ? ls->su_val.p : ls->sl_val.p;
+  else return upper ? ls->su_val32.p : ls->sl_val32.p;
+}
+
 template <typename T, typename TY>
 void pspmv(nalu_linsys* ls, const T* x, TY* y, const TY* b, int mode) {
   const Split<T> A = split_of<T>(ls);
   const int kc = sparse_kernel_choice();
+  if (kc == 64) {
+    const int64_t nch = ls->sell_nchunks;
+    hipLaunchKernelGGL((k_sell_spmv<T, TY>), dim3((unsigned)std::max<int64_t>(1, (nch + 3) / 4)), dim3(kThreads), 0,
+                       ls->stream, reinterpret_cast<const SellChunk*>(ls->sell_chunks.p), nch, ls->sl_col.p,
+                       sell_vals<T>(ls, false), ls->su_col.p, sell_vals<T>(ls, true), A.dv, x, y, b, mode);
+    return;
+  }
   if (kc == 8 || kc == 4) {
     const int64_t n = ls->nrows;
     if (kc == 8)
@@ -603,6 +844,23 @@ void psgs(nalu_linsys* ls, const T* r, T* z) {
   const size_t ldu = (size_t)Ru * std::max(1, ls->max_u_len) * sizeof(T);
   const auto& cp = ls->color_ptr_host;
   const int kc = sparse_kernel_choice();
+  if (kc == 64) {
+    const SellChunk* ch = reinterpret_cast<const SellChunk*>(ls->sell_chunks.p);
+    const auto& ccp = ls->sell_color_ptr;
+    for (int c = 0; c < ls->num_colors; ++c) {
+      const int64_t c0 = ccp[c], c1 = ccp[c + 1];
+      if (c1 > c0)
+        hipLaunchKernelGGL(k_sell_fwd<T>, dim3((unsigned)((c1 - c0 + 3) / 4)), dim3(kThreads), 0, s, ch, c0, c1,
+                           ls->sl_col.p, sell_vals<T>(ls, false), A.invd, r, z);
+    }
+    for (int c = ls->num_colors - 2; c >= 0; --c) {  // the last color has no U entries
+      const int64_t c0 = ccp[c], c1 = ccp[c + 1];
+      if (c1 > c0)
+        hipLaunchKernelGGL(k_sell_bwd<T>, dim3((unsigned)((c1 - c0 + 3) / 4)), dim3(kThreads), 0, s, ch, c0, c1,
+                           ls->su_col.p, sell_vals<T>(ls, true), A.invd, z);
+    }
+    return;
+  }
   if (kc == 8 || kc == 4) {
     for (int c = 0; c < ls->num_colors; ++c) {
       const int64_t c0 = cp[c], c1 = cp[c + 1];
@@ -763,14 +1021,17 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
       TV* zk = flexible ? Z + (size_t)n * k : Z;
       pprec<TV>(ls, vk, zk);
       pspmv<TV, TV>(ls, zk, vk1, (const TV*)nullptr, 0);
-      // ICGS: two classical Gram-Schmidt passes, coefficients summed on device
-      red.mdot_dev(V, n, k + 1, vk1, n, hdev.p, 0);
+      // ICGS: two classical Gram-Schmidt passes, coefficients summed on device;
+      // one launch per pass over the basis plus its reduction
+      unsigned nb = red.mdot_part(V, n, k + 1, vk1, n);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(k + 1), dim3(kThreads), 0, s, ls->red.p, (int)nb, hdev.p, 0);
       hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, V, n, k + 1, hdev.p, vk1, n, -1.0);
-      red.mdot_dev(V, n, k + 1, vk1, n, h2.p, 0);
-      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, V, n, k + 1, h2.p, vk1, n, -1.0);
-      hipLaunchKernelGGL((k_axpy<double, double>), dim3(nblocks(k + 1)), dim3(kThreads), 0, s, 1.0, h2.p, hdev.p,
-                         (int64_t)(k + 1));
-      red.dot_dev(vk1, vk1, n, hdev.p + k + 1);
+      nb = red.mdot_part(V, n, k + 1, vk1, n);
+      hipLaunchKernelGGL(k_reduce_rows2, dim3(k + 1), dim3(kThreads), 0, s, ls->red.p, (int)nb, h2.p, hdev.p);
+      const unsigned nbn = red_blocks(n);
+      hipLaunchKernelGGL((k_maxpy_norm<TV, TV>), dim3(nbn), dim3(kThreads), 0, s, V, n, k + 1, h2.p, vk1, n,
+                         ls->red.p);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kThreads), 0, s, ls->red.p, (int)nbn, hdev.p + k + 1, 0);
       hipLaunchKernelGGL(k_scale_invnorm<TV>, dim3(nblocks(n)), dim3(kThreads), 0, s, vk1, n, hdev.p + k + 1);
       NALU_HIP_CHECK(hipMemcpyAsync(hpin.p, hdev.p, sizeof(double) * (k + 2), hipMemcpyDeviceToHost, s));
       NALU_HIP_CHECK(hipStreamSynchronize(s));
@@ -886,6 +1147,13 @@ void ensure_split32(nalu_linsys* ls) {
                      ls->uval32.p);
   hipLaunchKernelGGL(k_take_diag<float>, dim3(nblocks(n)), dim3(kThreads), 0, s, ls->dsrc.p, ls->values.p, n,
                      ls->dval32.p, ls->pinvd32.p, bad.p);
+  const int64_t sl = ls->sl_src.n, su = ls->su_src.n;
+  if (ls->sl_val32.n != (size_t)sl) ls->sl_val32.alloc(sl);
+  if (ls->su_val32.n != (size_t)su) ls->su_val32.alloc(su);
+  hipLaunchKernelGGL(k_take_sell<float>, dim3(nblocks(sl)), dim3(kThreads), 0, s, ls->sl_src.p, ls->values.p, sl,
+                     ls->sl_val32.p);
+  hipLaunchKernelGGL(k_take_sell<float>, dim3(nblocks(su)), dim3(kThreads), 0, s, ls->su_src.p, ls->values.p, su,
+                     ls->su_val32.p);
   NALU_HIP_CHECK(hipStreamSynchronize(s));
   ls->split32_ready = true;
 }
@@ -907,6 +1175,16 @@ void precond_setup(nalu_linsys* ls) {
   hipLaunchKernelGGL(k_take<double>, dim3(nblocks(nu)), dim3(kThreads), 0, s, ls->usrc.p, ls->values.p, nu, ls->uval.p);
   hipLaunchKernelGGL(k_take_diag<double>, dim3(nblocks(n)), dim3(kThreads), 0, s, ls->dsrc.p, ls->values.p, n,
                      ls->dval.p, ls->pinvd.p, bad.p);
+  build_sell(ls);
+  {
+    const int64_t sl = ls->sl_src.n, su = ls->su_src.n;
+    if (ls->sl_val.n != (size_t)sl) ls->sl_val.alloc(sl);
+    if (ls->su_val.n != (size_t)su) ls->su_val.alloc(su);
+    hipLaunchKernelGGL(k_take_sell<double>, dim3(nblocks(sl)), dim3(kThreads), 0, s, ls->sl_src.p, ls->values.p, sl,
+                       ls->sl_val.p);
+    hipLaunchKernelGGL(k_take_sell<double>, dim3(nblocks(su)), dim3(kThreads), 0, s, ls->su_src.p, ls->values.p, su,
+                       ls->su_val.p);
+  }
   int hb = 0;
   NALU_HIP_CHECK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
   NALU_HIP_CHECK(hipStreamSynchronize(s));

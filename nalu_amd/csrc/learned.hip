@@ -288,36 +288,51 @@ __global__ __launch_bounds__(kThreads) void k_layer(const int32_t* __restrict__ 
   for (int ks = 0; ks < 4; ++ks) wfrag[ks] = *reinterpret_cast<const u16x8*>(wl + f * 2 * kH + ks * 16 + 8 * half);
   const float bias = bl[f];
 
-  for (int p = 0; p < kTile / 2; ++p) {
-    const int nl = 2 * p + half;
-    const int64_t node = tile0 + nl;
-    uint16_t hs = 0;
-    float acc = 0.f;
-    if (node < total) {
-      const int64_t b = node / n, i = node - b * n;
-      const uint16_t* hb = hin + b * n * kH;
-      hs = hb[i * kH + f];
-      const int s0 = gptr[i], s1 = gptr[i + 1];
-      for (int c0 = s0; c0 < s1; c0 += 32) {
-        const int cnt = min(32, s1 - c0);
-        int cj = 0;
-        float cw = 0.f;
-        if (f < cnt) {
-          cj = gcol[c0 + f];
-          cw = gw[c0 + f];
-        }
-
-        for (int k = 0; k < cnt; ++k) {
-          const int j = __shfl(cj, k, 32);
-          const float w = __shfl(cw, k, 32);
-          acc = fmaf(w, bf2f(hb[(int64_t)j * kH + f]), acc);
+  // aggregation: 8 lanes per node (4 features each), 8 nodes per pass; the
+  // neighbour list is read 8 entries at a time and broadcast within the group,
+  // so each lane has 8 independent 8-byte loads in flight
+  {
+    const int sub = lane >> 3, q = lane & 7;
+    for (int p = 0; p < kTile / 8; ++p) {
+      const int nl = p * 8 + sub;
+      const int64_t node = tile0 + nl;
+      uint2 hs = make_uint2(0u, 0u);
+      float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+      if (node < total) {
+        const int64_t b = node / n, i = node - b * n;
+        const uint16_t* hb = hin + b * n * kH + 4 * q;
+        hs = *reinterpret_cast<const uint2*>(hb + i * kH);
+        const int s0 = gptr[i], s1 = gptr[i + 1];
+        for (int c0 = s0; c0 < s1; c0 += 8) {
+          int cj = 0;
+          float cw = 0.f;
+          if (c0 + q < s1) {
+            cj = gcol[c0 + q];
+            cw = gw[c0 + q];
+          }
+          uint2 hv[8];
+          float ww[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int j = __shfl(cj, u, 8);
+            ww[u] = __shfl(cw, u, 8);
+            hv[u] = *reinterpret_cast<const uint2*>(hb + (int64_t)j * kH);
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            a0 = fmaf(ww[u], __uint_as_float(hv[u].x << 16), a0);
+            a1 = fmaf(ww[u], __uint_as_float(hv[u].x & 0xFFFF0000u), a1);
+            a2 = fmaf(ww[u], __uint_as_float(hv[u].y << 16), a2);
+            a3 = fmaf(ww[u], __uint_as_float(hv[u].y & 0xFFFF0000u), a3);
+          }
         }
       }
+      const uint2 mb = make_uint2((uint32_t)f2bf(a0) | ((uint32_t)f2bf(a1) << 16),
+                                  (uint32_t)f2bf(a2) | ((uint32_t)f2bf(a3) << 16));
+      *reinterpret_cast<uint2*>(&As[wv][nl][4 * q]) = hs;
+      *reinterpret_cast<uint2*>(&As[wv][nl][kH + 4 * q]) = mb;
+      if (TRAIN && node < total) *reinterpret_cast<uint2*>(mst + node * kH + 4 * q) = mb;
     }
-    As[wv][nl][f] = hs;
-    const uint16_t mb = f2bf(acc);
-    As[wv][nl][kH + f] = mb;
-    if (TRAIN && node < total) mst[node * kH + f] = mb;
   }
   __syncthreads();
 
@@ -577,35 +592,44 @@ __global__ __launch_bounds__(kThreads) void k_layer_bwd(int64_t total, const flo
   if (half == 0) pw[2 * kH * kH + f] = db;
 }
 
-// G_j = Gp_j + sum_{i in N(j)} w_ij DM_i  (transposed aggregation; FP32)
+// G_j = Gp_j + sum_{i in N(j)} w_ij DM_i  (transposed aggregation; FP32);
+// 8 lanes per node, 4 features (16 bytes) per lane
 __global__ __launch_bounds__(kThreads) void k_agg_t(const int32_t* __restrict__ gptr, const int32_t* __restrict__ gcol,
                                                     const float* __restrict__ gwt, int64_t n, int64_t total,
                                                     const float* __restrict__ Gp, const float* __restrict__ DM,
                                                     float* __restrict__ Gout) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t node = t >> 5;
-  const int f = threadIdx.x & 31;
-  if (node >= total) return;  // the 32 lanes of a node exit together
+  const int64_t node = t >> 3;
+  const int q = threadIdx.x & 7;
+  if (node >= total) return;  // the 8 lanes of a node exit together
   const int64_t b = node / n, i = node - b * n;
-  const float* db = DM + b * n * kH;
-  float acc = Gp[node * kH + f];
+  const float* db = DM + b * n * kH + 4 * q;
+  float4 acc = *reinterpret_cast<const float4*>(Gp + node * kH + 4 * q);
   const int s0 = gptr[i], s1 = gptr[i + 1];
-  for (int c0 = s0; c0 < s1; c0 += 32) {
-    const int cnt = min(32, s1 - c0);
+  for (int c0 = s0; c0 < s1; c0 += 8) {
     int cj = 0;
     float cw = 0.f;
-    if (f < cnt) {
-      cj = gcol[c0 + f];
-      cw = gwt[c0 + f];
+    if (c0 + q < s1) {
+      cj = gcol[c0 + q];
+      cw = gwt[c0 + q];
     }
-
-    for (int k = 0; k < cnt; ++k) {
-      const int j = __shfl(cj, k, 32);
-      const float w = __shfl(cw, k, 32);
-      acc = fmaf(w, db[(int64_t)j * kH + f], acc);
+    float4 v[8];
+    float w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = __shfl(cj, u, 8);
+      w[u] = __shfl(cw, u, 8);
+      v[u] = *reinterpret_cast<const float4*>(db + (int64_t)j * kH);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      acc.x = fmaf(w[u], v[u].x, acc.x);
+      acc.y = fmaf(w[u], v[u].y, acc.y);
+      acc.z = fmaf(w[u], v[u].z, acc.z);
+      acc.w = fmaf(w[u], v[u].w, acc.w);
     }
   }
-  Gout[node * kH + f] = acc;
+  *reinterpret_cast<float4*>(Gout + node * kH + 4 * q) = acc;
 }
 
 // dW_in += G0 f^T, db_in += G0: part[blk][128] = [w_in 96 | b_in 32]
@@ -941,7 +965,7 @@ void fwd_bwd(nalu_linsys* ls, Learned* st, nalu_gnn* g, Work& w, double* loss_sl
     hipLaunchKernelGGL(k_layer_bwd, dim3(nbl), dim3(kThreads), 0, s, T, w.G.p, w.mask[l].p, w.h[l].p, w.mst[l].p,
                        g->wbf.p + lay.w(l), w.Gp.p, w.DM.p, w.fpart.p);
     col_sum(s, w.fpart.p, (int)nbl * kWaves, kLayerParams, g->grad.p + lay.w(l), w);
-    hipLaunchKernelGGL(k_agg_t, dim3(nblocks(T * kH)), dim3(kThreads), 0, s, st->gptr.p, st->gcol.p, st->gwt.p, n, T,
+    hipLaunchKernelGGL(k_agg_t, dim3(nblocks(T * 8)), dim3(kThreads), 0, s, st->gptr.p, st->gcol.p, st->gwt.p, n, T,
                        w.Gp.p, w.DM.p, w.G.p);
   }
   const unsigned nbe = std::min<unsigned>(kBwdBlocks, nblocks(T * kH));

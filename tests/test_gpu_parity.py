@@ -259,6 +259,16 @@ def test_spmv_and_preconditioners(oracle):
         else:
             oz = oracle.jacobi_apply(rp, ci, v, x)
         np.testing.assert_allclose(z.cpu().numpy(), oz, rtol=0, atol=1e-12 * np.abs(oz).max())
+        # the solver-order kernels must not read what their output buffer held before
+        xp = dev(x[ls.permutation()])
+        z0, zn = torch.zeros_like(y), torch.full_like(y, float("nan"))
+        ls.precond_apply_permuted(xp, z0)
+        ls.precond_apply_permuted(xp, zn)
+        assert torch.equal(z0, zn)
+        y0, yn = torch.zeros_like(y), torch.full_like(y, float("nan"))
+        ls.spmv_permuted(xp, y0)
+        ls.spmv_permuted(xp, yn)
+        assert torch.equal(y0, yn)
 
 
 @pytest.mark.parametrize("method,prec", [("gmres", "fp64"), ("gmres", "mixed"), ("fgmres", "fp64")])
