@@ -78,26 +78,6 @@ __global__ __launch_bounds__(kThreads) void k_reduce_rows(const double* __restri
   if (threadIdx.x == 0) out[blockIdx.x] = accumulate ? out[blockIdx.x] + s : s;
 }
 
-// multi-dot: part[j][blk] for j in [0, cnt), V column-major with leading dim ldv
-template <typename TV, typename TW, int CHUNK>
-__global__ __launch_bounds__(kThreads) void k_mdot_part(const TV* __restrict__ V, int64_t ldv, int cnt,
-                                                        const TW* __restrict__ w, int64_t n, double* __restrict__ part) {
-  double s[CHUNK];
-#pragma unroll
-  for (int j = 0; j < CHUNK; ++j) s[j] = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double wi = (double)w[i];
-#pragma unroll
-    for (int j = 0; j < CHUNK; ++j)
-      if (j < cnt) s[j] += (double)V[j * ldv + i] * wi;
-  }
-#pragma unroll
-  for (int j = 0; j < CHUNK; ++j) {
-    const double t = block_sum(s[j]);
-    if (threadIdx.x == 0 && j < cnt) part[(int64_t)j * gridDim.x + blockIdx.x] = t;
-  }
-}
-
 // w -= V h  (h on device, length cnt)
 template <typename TV, typename TW>
 __global__ __launch_bounds__(kThreads) void k_maxpy(const TV* __restrict__ V, int64_t ldv, int cnt,
@@ -110,52 +90,136 @@ __global__ __launch_bounds__(kThreads) void k_maxpy(const TV* __restrict__ V, in
   w[i] = (TW)((double)w[i] + sign * acc);
 }
 
-// Gram-Schmidt building blocks with one launch per pass over the basis.
-// part[j * gridDim.x + blk] = V_j . w over rows [blk R, (blk+1) R), one
-// wavefront per basis vector.
-template <typename TV, typename TW>
-__global__ __launch_bounds__(kThreads) void k_mdot_rows(const TV* __restrict__ V, int64_t ldv, int cnt,
-                                                        const TW* __restrict__ w, int64_t n, int64_t R,
-                                                        double* __restrict__ part) {
+// ---- vectorized Gram-Schmidt passes (basis columns padded to 64 entries, so
+// every column starts 16-byte aligned; 16-byte loads per lane)
+template <typename T>
+struct V16;
+template <>
+struct V16<float> {
+  static constexpr int N = 4;
+  __device__ static void load(const float* p, double* o) {
+    const float4 a = *reinterpret_cast<const float4*>(p);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+  }
+};
+template <>
+struct V16<double> {
+  static constexpr int N = 2;
+  __device__ static void load(const double* p, double* o) {
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    o[0] = a.x; o[1] = a.y;
+  }
+};
+
+// Coefficients part[j * gridDim.x + blk] = V_j . w over the chunks blk, blk + G,
+// ... of R rows (j < cnt), plus w . w as row cnt when with_self; w's chunk is
+// staged in LDS, one wavefront per basis vector, per-block sums kept in LDS.
+template <typename TV>
+__global__ __launch_bounds__(kThreads) void k_gs_dots(const TV* __restrict__ V, int64_t ldv, int cnt, int with_self,
+                                                      const TV* __restrict__ w, int64_t n, int R,
+                                                      double* __restrict__ part) {
+  extern __shared__ unsigned char smem_raw[];
+  TV* ws = reinterpret_cast<TV*>(smem_raw);         // [R]
+  double* acc = reinterpret_cast<double*>(ws + R);  // [cnt + 1]
+  constexpr int VN = V16<TV>::N;
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int64_t r0 = (int64_t)blockIdx.x * R, r1 = min(n, r0 + R);
-  for (int j = wv; j < cnt; j += kThreads / 64) {
-    const TV* v = V + (int64_t)j * ldv;
-    double s = 0.0;
-    for (int64_t i = r0 + l; i < r1; i += 64) s += (double)v[i] * (double)w[i];
-    s = wave_sum(s);
-    if (l == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = s;
+  const int rows = cnt + (with_self ? 1 : 0);
+  for (int j = threadIdx.x; j < rows; j += blockDim.x) acc[j] = 0.0;
+  const int64_t nch = (n + R - 1) / R;
+  for (int64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    const int64_t r0 = c * R;
+    const int m = (int)min<int64_t>(R, n - r0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < m; t += blockDim.x) ws[t] = w[r0 + t];
+    __syncthreads();
+    const int mv = m / VN * VN;
+    for (int j = wv; j < rows; j += kThreads / 64) {
+      const TV* v = (j < cnt ? V + (int64_t)j * ldv : w) + r0;
+      double sj = 0.0;
+      for (int t = l * VN; t < mv; t += 64 * VN) {
+        double a[VN];
+        V16<TV>::load(v + t, a);
+#pragma unroll
+        for (int u = 0; u < VN; ++u) sj += a[u] * (double)ws[t + u];
+      }
+      for (int t = mv + l; t < m; t += 64) sj += (double)v[t] * (double)ws[t];
+      sj = wave_sum(sj);
+      if (l == 0) acc[j] += sj;
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < rows; j += blockDim.x) part[(int64_t)j * gridDim.x + blockIdx.x] = acc[j];
+}
+
+// w -= V h (FP64 accumulation); each thread owns VN consecutive rows
+template <typename TV>
+__global__ __launch_bounds__(kThreads) void k_gs_update(const TV* __restrict__ V, int64_t ldv, int cnt,
+                                                        const double* __restrict__ h, TV* __restrict__ w, int64_t n) {
+  constexpr int VN = V16<TV>::N;
+  const int64_t ng = (n + VN - 1) / VN;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i0 = g * VN;
+    if (i0 + VN <= n) {
+      double a[VN];
+      V16<TV>::load(w + i0, a);
+      for (int j = 0; j < cnt; ++j) {
+        double v[VN];
+        V16<TV>::load(V + (int64_t)j * ldv + i0, v);
+        const double hj = h[j];
+#pragma unroll
+        for (int u = 0; u < VN; ++u) a[u] -= hj * v[u];
+      }
+#pragma unroll
+      for (int u = 0; u < VN; ++u) w[i0 + u] = (TV)a[u];
+    } else {
+      for (int64_t i = i0; i < n; ++i) {
+        double t = (double)w[i];
+        for (int j = 0; j < cnt; ++j) t -= h[j] * (double)V[(int64_t)j * ldv + i];
+        w[i] = (TV)t;
+      }
+    }
   }
 }
 
-// w -= V h and part[blk] = partial ||w||^2 of the updated (stored) values
-template <typename TV, typename TW>
-__global__ __launch_bounds__(kThreads) void k_maxpy_norm(const TV* __restrict__ V, int64_t ldv, int cnt,
-                                                         const double* __restrict__ h, TW* __restrict__ w, int64_t n,
-                                                         double* __restrict__ part) {
+// pass-2 update with the norm of the result: w -= V h, part[blk] = ||w||^2 (partial);
+// each thread owns VN consecutive rows
+template <typename TV>
+__global__ __launch_bounds__(kThreads) void k_gs_update_norm(const TV* __restrict__ V, int64_t ldv, int cnt,
+                                                             const double* __restrict__ h, TV* __restrict__ w,
+                                                             int64_t n, double* __restrict__ part) {
+  constexpr int VN = V16<TV>::N;
   double s2 = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int j = 0; j < cnt; ++j) acc += h[j] * (double)V[(int64_t)j * ldv + i];
-    const TW wn = (TW)((double)w[i] - acc);
-    w[i] = wn;
-    s2 += (double)wn * (double)wn;
+  const int64_t ng = (n + VN - 1) / VN;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i0 = g * VN;
+    double a[VN];
+    if (i0 + VN <= n) {
+      V16<TV>::load(w + i0, a);
+      for (int j = 0; j < cnt; ++j) {
+        double v[VN];
+        V16<TV>::load(V + (int64_t)j * ldv + i0, v);
+        const double hj = h[j];
+#pragma unroll
+        for (int u = 0; u < VN; ++u) a[u] -= hj * v[u];
+      }
+#pragma unroll
+      for (int u = 0; u < VN; ++u) {
+        const TV t = (TV)a[u];
+        w[i0 + u] = t;
+        s2 += (double)t * (double)t;
+      }
+    } else {
+      for (int64_t i = i0; i < n; ++i) {
+        double t = (double)w[i];
+        for (int j = 0; j < cnt; ++j) t -= h[j] * (double)V[(int64_t)j * ldv + i];
+        const TV tv = (TV)t;
+        w[i] = tv;
+        s2 += (double)tv * (double)tv;
+      }
+    }
   }
   s2 = block_sum(s2);
   if (threadIdx.x == 0) part[blockIdx.x] = s2;
-}
-
-// out[j] = sum of part[j][0..nb); acc[j] += the same sum
-__global__ __launch_bounds__(kThreads) void k_reduce_rows2(const double* __restrict__ part, int nb,
-                                                           double* __restrict__ out, double* __restrict__ acc) {
-  const double* p = part + (int64_t)blockIdx.x * nb;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
-  s = block_sum(s);
-  if (threadIdx.x == 0) {
-    out[blockIdx.x] = s;
-    acc[blockIdx.x] += s;
-  }
 }
 
 // ------------------------------------------------------------ SpMV (CSR-stream)
@@ -272,7 +336,7 @@ struct Red {
   nalu_linsys* ls;
   hipStream_t s;
   explicit Red(nalu_linsys* l) : ls(l), s(l->stream) {
-    const size_t need = (size_t)kRedBlocks * std::max(64, ls->cfg.kspace + 2);
+    const size_t need = (size_t)std::max(kRedBlocks, 2048) * std::max(64, ls->cfg.kspace + 2);
     if (ls->red.n < need) ls->red.alloc(need);
     if (ls->red_out.n < 256) ls->red_out.alloc(256);
   }
@@ -290,30 +354,6 @@ struct Red {
     NALU_HIP_CHECK(hipMemcpyAsync(&h, ls->red_out.p, sizeof(double), hipMemcpyDeviceToHost, s));
     NALU_HIP_CHECK(hipStreamSynchronize(s));
     return h;
-  }
-  // out[0..cnt) (+)= V[:, 0..cnt)^T w
-  // one-launch multi-dot: rows split in at most kRedBlocks blocks
-  int64_t rows_per_block(int64_t n) const {
-    const int64_t R = std::max<int64_t>(2048, (n + kRedBlocks - 1) / kRedBlocks);
-    return (R + 63) / 64 * 64;
-  }
-  template <typename TV, typename TW>
-  unsigned mdot_part(const TV* V, int64_t ldv, int cnt, const TW* w, int64_t n) {
-    const int64_t R = rows_per_block(n);
-    const unsigned nb = (unsigned)((n + R - 1) / R);
-    NALU_REQUIRE((size_t)cnt * nb <= ls->red.n, NALU_ERR_STATE, "multi-dot workspace too small");
-    hipLaunchKernelGGL((k_mdot_rows<TV, TW>), dim3(nb), dim3(kThreads), 0, s, V, ldv, cnt, w, n, R, ls->red.p);
-    return nb;
-  }
-  template <typename TV, typename TW>
-  void mdot_dev(const TV* V, int64_t ldv, int cnt, const TW* w, int64_t n, double* out, int accumulate) {
-    const unsigned nb = red_blocks(n);
-    for (int j0 = 0; j0 < cnt; j0 += 8) {
-      const int c = std::min(8, cnt - j0);
-      hipLaunchKernelGGL((k_mdot_part<TV, TW, 8>), dim3(nb), dim3(kThreads), 0, s, V + j0 * ldv, ldv, c, w, n,
-                         ls->red.p);
-      hipLaunchKernelGGL(k_reduce_rows, dim3(c), dim3(kThreads), 0, s, ls->red.p, (int)nb, out + j0, accumulate);
-    }
   }
 };
 
@@ -981,19 +1021,38 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
   hipStream_t s = ls->stream;
   const int64_t n = ls->nrows;
   Red red(ls);
-  const size_t need = (size_t)n * ((m + 1) + (flexible ? m : 1) + 2);
+  // basis columns padded to a multiple of 64 entries (16-byte aligned columns)
+  const int64_t ldv = (n + 63) / 64 * 64;
+  const size_t need = (size_t)ldv * ((m + 1) + (flexible ? m : 1) + 2);
   DevBuf<TV> ws(need);
   TV* V = ws.p;
-  TV* Z = V + (size_t)n * (m + 1);
-  TV* tmp = Z + (size_t)n * (flexible ? m : 1);
-  TV* xt = tmp + n;  // x in TV precision for the residual of the inner SpMV path
-  DevBuf<double> r64(n), hdev(m + 2), h2(m + 2);
+  TV* Z = V + (size_t)ldv * (m + 1);
+  TV* tmp = Z + (size_t)ldv * (flexible ? m : 1);
+  TV* xt = tmp + ldv;  // x in TV precision for the residual of the inner SpMV path
+  // Gram-Schmidt launch shapes: chunks of R rows, at most kGsBlocks blocks.
+  // The FP64 solve orthogonalizes with two classical passes (ICGS, as Belos is
+  // configured in the reference); the FP32 inner solve of the mixed path does
+  // one pass and reorthogonalizes only on cancellation, ||w'||^2 < ||w||^2 / 2
+  // (the "twice is enough" criterion of Parlett and Kahan).
+  constexpr int kGsBlocks = 2048;
+  const bool always2 = std::is_same<TV, double>::value;
+  const int R_dots = 2048;
+  const unsigned nb_dots = (unsigned)std::min<int64_t>(kGsBlocks, (n + R_dots - 1) / R_dots);
+  const size_t lds_dots = (size_t)R_dots * sizeof(TV) + (size_t)(m + 2) * 8;
+  const unsigned nb_upd = red_blocks((n + 3) / 4);
+  NALU_REQUIRE((size_t)(m + 2) * kGsBlocks <= ls->red.n, NALU_ERR_STATE, "gmres: reduction workspace too small");
+  DevBuf<double> r64(n), hdev(m + 2);
+  // hA: pass-1 coefficients (+ ||w||^2 at k+1), hB: pass-2 coefficients, nrm: ||w_new||^2
+  DevBuf<double> hbuf(2 * (m + 2) + 2);
+  double* hA = hbuf.p;
+  double* hB = hbuf.p + (m + 2);
+  double* nrm = hbuf.p + 2 * (m + 2);
   // learned preconditioner: FGMRES falls back to SGS for the rest of the solve
   // when a cycle stagnates or the network output is not finite
   const bool watch = flexible && ls->cfg.precond == NALU_PREC_LEARNED;
   Hessenberg H(m);
   std::vector<double> hcol(m + 2), y;
-  PinnedDoubles hpin(m + 2);
+  PinnedDoubles hpin(2 * (m + 2) + 2);
   (void)xt;
 
   int iters = 0;
@@ -1016,27 +1075,48 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
     const bool learned_cycle = watch && learned_active(ls);
     for (; k < m && iters < maxit; ++k) {
       ++iters;
-      TV* vk = V + (size_t)n * k;
-      TV* vk1 = V + (size_t)n * (k + 1);
-      TV* zk = flexible ? Z + (size_t)n * k : Z;
+      TV* vk = V + (size_t)ldv * k;
+      TV* vk1 = V + (size_t)ldv * (k + 1);
+      TV* zk = flexible ? Z + (size_t)ldv * k : Z;
       pprec<TV>(ls, vk, zk);
       pspmv<TV, TV>(ls, zk, vk1, (const TV*)nullptr, 0);
-      // ICGS: two classical Gram-Schmidt passes, coefficients summed on device;
-      // one launch per pass over the basis plus its reduction
-      unsigned nb = red.mdot_part(V, n, k + 1, vk1, n);
-      hipLaunchKernelGGL(k_reduce_rows, dim3(k + 1), dim3(kThreads), 0, s, ls->red.p, (int)nb, hdev.p, 0);
-      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, V, n, k + 1, hdev.p, vk1, n, -1.0);
-      nb = red.mdot_part(V, n, k + 1, vk1, n);
-      hipLaunchKernelGGL(k_reduce_rows2, dim3(k + 1), dim3(kThreads), 0, s, ls->red.p, (int)nb, h2.p, hdev.p);
-      const unsigned nbn = red_blocks(n);
-      hipLaunchKernelGGL((k_maxpy_norm<TV, TV>), dim3(nbn), dim3(kThreads), 0, s, V, n, k + 1, h2.p, vk1, n,
-                         ls->red.p);
-      hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kThreads), 0, s, ls->red.p, (int)nbn, hdev.p + k + 1, 0);
-      hipLaunchKernelGGL(k_scale_invnorm<TV>, dim3(nblocks(n)), dim3(kThreads), 0, s, vk1, n, hdev.p + k + 1);
-      NALU_HIP_CHECK(hipMemcpyAsync(hpin.p, hdev.p, sizeof(double) * (k + 2), hipMemcpyDeviceToHost, s));
+      const int cnt = k + 1;
+      const size_t hbytes = sizeof(double) * (2 * (m + 2) + 1);
+      hipLaunchKernelGGL(k_gs_dots<TV>, dim3(nb_dots), dim3(kThreads), lds_dots, s, V, ldv, cnt, always2 ? 0 : 1, vk1,
+                         n, R_dots, ls->red.p);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(always2 ? cnt : cnt + 1), dim3(kThreads), 0, s, ls->red.p, (int)nb_dots,
+                         hA, 0);
+      auto second_pass = [&]() {
+        hipLaunchKernelGGL(k_gs_dots<TV>, dim3(nb_dots), dim3(kThreads), lds_dots, s, V, ldv, cnt, 0, vk1, n, R_dots,
+                           ls->red.p);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(cnt), dim3(kThreads), 0, s, ls->red.p, (int)nb_dots, hB, 0);
+        hipLaunchKernelGGL(k_gs_update_norm<TV>, dim3(nb_upd), dim3(kThreads), 0, s, V, ldv, cnt, hB, vk1, n,
+                           ls->red.p);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kThreads), 0, s, ls->red.p, (int)nb_upd, nrm, 0);
+      };
+      if (always2) {
+        hipLaunchKernelGGL(k_gs_update<TV>, dim3(nb_upd), dim3(kThreads), 0, s, V, ldv, cnt, hA, vk1, n);
+        second_pass();
+      } else {
+        hipLaunchKernelGGL(k_gs_update_norm<TV>, dim3(nb_upd), dim3(kThreads), 0, s, V, ldv, cnt, hA, vk1, n,
+                           ls->red.p);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kThreads), 0, s, ls->red.p, (int)nb_upd, nrm, 0);
+      }
+      NALU_HIP_CHECK(hipMemcpyAsync(hpin.p, hbuf.p, hbytes, hipMemcpyDeviceToHost, s));
       NALU_HIP_CHECK(hipStreamSynchronize(s));
-      for (int i = 0; i <= k; ++i) hcol[i] = hpin.p[i];
-      hcol[k + 1] = std::sqrt(hpin.p[k + 1]);
+      bool two = always2;
+      if (!always2) {
+        const double w2 = hpin.p[cnt], n2 = hpin.p[2 * (m + 2)];
+        if (!(n2 >= 0.5 * w2)) {  // cancellation (or NaN): reorthogonalize once
+          second_pass();
+          NALU_HIP_CHECK(hipMemcpyAsync(hpin.p, hbuf.p, hbytes, hipMemcpyDeviceToHost, s));
+          NALU_HIP_CHECK(hipStreamSynchronize(s));
+          two = true;
+        }
+      }
+      for (int i = 0; i <= k; ++i) hcol[i] = hpin.p[i] + (two ? hpin.p[(m + 2) + i] : 0.0);
+      hcol[k + 1] = std::sqrt(hpin.p[2 * (m + 2)]);
+      hipLaunchKernelGGL(k_scale_invnorm<TV>, dim3(nblocks(n)), dim3(kThreads), 0, s, vk1, n, nrm);
       if (learned_cycle && !std::isfinite(hcol[k + 1])) {
         learned_set_fallback(ls, true);  // drop this direction, finish the cycle with k columns
         break;
@@ -1056,10 +1136,10 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
     NALU_HIP_CHECK(hipMemcpyAsync(hdev.p, hpin.p, sizeof(double) * k, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((k_fill<TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, tmp, n, TV(0));
     if (flexible) {
-      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, Z, n, k, hdev.p, tmp, n, 1.0);
+      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, Z, ldv, k, hdev.p, tmp, n, 1.0);
       hipLaunchKernelGGL((k_axpy<TV, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, 1.0, tmp, x, n);
     } else {
-      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, V, n, k, hdev.p, tmp, n, 1.0);
+      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, V, ldv, k, hdev.p, tmp, n, 1.0);
       pprec<TV>(ls, tmp, Z);
       hipLaunchKernelGGL((k_axpy<TV, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, 1.0, Z, x, n);
     }
