@@ -160,6 +160,10 @@ struct nalu_linsys {
   int64_t sell_nchunks = 0;
   nalu::DevBuf<unsigned char> sell_chunks;  // [nchunks] chunk descriptors (krylov.hip)
   std::vector<int64_t> sell_color_ptr;      // chunks of color c: [ptr[c], ptr[c+1])
+  // all-color kernels visit the chunks in this order: fine spatial slabs, each
+  // with its chunks of every color, so that neighbouring rows are touched close
+  // together in time; the 8 XCDs take contiguous eighths of the order
+  nalu::DevBuf<int32_t> sell_sched;
   nalu::DevBuf<int32_t> sl_col, su_col, sl_src, su_src;
   nalu::DevBuf<double> sl_val, su_val;
   nalu::DevBuf<float> sl_val32, su_val32;
@@ -192,6 +196,24 @@ struct PhaseTimer {
   ~PhaseTimer();
 };
 void flush_timers(nalu_linsys* ls);
+
+// One chunk of the SELL-64 solver copy: rows [row0, row0 + nrows) of one color;
+// L entry k of row row0 + r at loff + 64 k + r (lw entries), U likewise.
+struct SellChunk {
+  int32_t row0, nrows, lw, uw;
+  int64_t loff, uoff;
+};
+
+// XCD-aware block mapping: hardware dispatches block b to XCD b % 8; this
+// returns the position of block b in a work order split into 8 contiguous
+// ranges of nper blocks (one per XCD).  Blocks past nb return -1.
+constexpr int kNumXcd = 8;
+__host__ __device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t nper = (nb + kNumXcd - 1) / kNumXcd;
+  const int64_t pos = (b % kNumXcd) * nper + b / kNumXcd;
+  return pos < nb ? pos : -1;
+}
+inline int64_t xcd_grid(int64_t nb) { return (nb + kNumXcd - 1) / kNumXcd * kNumXcd; }
 
 // graph.hip
 void build_node_incidence(nalu_mesh* m, hipStream_t s);

@@ -586,11 +586,6 @@ __global__ __launch_bounds__(kThreads) void k_bwd_color_vec(const int32_t* __res
 // ------------------------------------------------------------ SELL-64 kernels
 // One wavefront per chunk, one lane per row: every load of column indices and
 // values is a fully coalesced 64-lane access, no reductions or LDS.
-struct SellChunk {
-  int32_t row0, nrows, lw, uw;
-  int64_t loff, uoff;
-};
-
 // chunk widths = longest L / U part of the chunk's rows
 __global__ void k_sell_widths(SellChunk* __restrict__ ch, int64_t nch, const int32_t* __restrict__ lrp,
                               const int32_t* __restrict__ urp) {
@@ -800,6 +795,20 @@ void build_sell(nalu_linsys* ls) {
   ls->sell_color_ptr[nc] = (int64_t)ch.size();
   const int64_t nch = (int64_t)ch.size();
   ls->sell_nchunks = nch;
+  {
+    // schedule: S fine slabs; slab k holds the k-th S-th of every color's chunks
+    const int64_t S = std::max<int64_t>(1, nch / (4 * std::max(1, nc)));
+    std::vector<int32_t> sched;
+    sched.reserve(nch);
+    for (int64_t k = 0; k < S; ++k)
+      for (int c = 0; c < nc; ++c) {
+        const int64_t a = ls->sell_color_ptr[c], len = ls->sell_color_ptr[c + 1] - a;
+        for (int64_t q = a + k * len / S; q < a + (k + 1) * len / S; ++q) sched.push_back((int32_t)q);
+      }
+    NALU_REQUIRE((int64_t)sched.size() == nch, NALU_ERR_STATE, "SELL schedule does not cover the chunks");
+    ls->sell_sched.alloc(std::max<int64_t>(1, nch));
+    NALU_HIP_CHECK(hipMemcpy(ls->sell_sched.p, sched.data(), sizeof(int32_t) * nch, hipMemcpyHostToDevice));
+  }
   ls->sell_chunks.alloc(std::max<int64_t>(1, nch) * sizeof(SellChunk));
   SellChunk* dch = reinterpret_cast<SellChunk*>(ls->sell_chunks.p);
   NALU_HIP_CHECK(hipMemcpy(dch, ch.data(), nch * sizeof(SellChunk), hipMemcpyHostToDevice));

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -266,6 +267,158 @@ __global__ __launch_bounds__(kThreads) void k_encode(const float* __restrict__ r
   h = fmaf(bf2f(wbf[f * kIn + 2]), dl, h);
   h0[node * kH + f] = f2bf(h);
   if (f == 0) F[node] = make_float2(f0, f1);
+}
+
+// Encoder for inference: one thread per node, 64-byte row written as 4 x 16 B.
+__global__ __launch_bounds__(kThreads) void k_encode_rows(const float* __restrict__ r, const float* __restrict__ zs,
+                                                          const float* __restrict__ delta,
+                                                          const double* __restrict__ dsum,
+                                                          const float* __restrict__ scal, int64_t n,
+                                                          const uint16_t* __restrict__ wbf,
+                                                          const float* __restrict__ th, uint16_t* __restrict__ h0,
+                                                          float2* __restrict__ F) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float inv_s = scal[0];
+  const float dbar = (float)(dsum[0] / (double)n);
+  const float dl = delta[i];
+  const float f0 = r[i] * inv_s * dl;
+  const float f1 = zs[i] * inv_s * dbar;
+  F[i] = make_float2(f0, f1);
+  uint32_t pk[16];
+#pragma unroll
+  for (int f = 0; f < kH; f += 2) {
+    float a = th[kH * kIn + f], b = th[kH * kIn + f + 1];
+    a = fmaf(bf2f(wbf[f * kIn + 0]), f0, a);
+    a = fmaf(bf2f(wbf[f * kIn + 1]), f1, a);
+    a = fmaf(bf2f(wbf[f * kIn + 2]), dl, a);
+    b = fmaf(bf2f(wbf[(f + 1) * kIn + 0]), f0, b);
+    b = fmaf(bf2f(wbf[(f + 1) * kIn + 1]), f1, b);
+    b = fmaf(bf2f(wbf[(f + 1) * kIn + 2]), dl, b);
+    pk[f / 2] = (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+  }
+  uint4* o = reinterpret_cast<uint4*>(h0 + i * kH);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) o[q] = make_uint4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
+}
+
+// Inference layer on the SELL-64 chunks of the solver copy: a wavefront owns
+// one chunk (<= 64 rows of one color), a lane one row.  Column indices and
+// FP32 matrix values are read with coalesced 64-lane loads (w_ij = -a_ij / a_ii
+// formed on the fly), each neighbour's 64-byte hidden row with 4 x 16 B loads;
+// the [h ; m] tile goes through LDS into two 32x32x16 BF16 MFMA M-tiles.  The
+// last layer writes z = (w_out . h + c0 f0 + c1 f1) s / dbar instead of h.
+template <bool LAST, int G>
+__global__ __launch_bounds__(kThreads) void k_layer_sell(
+    const SellChunk* __restrict__ ch, const int32_t* __restrict__ sched, int64_t nch, int64_t nbw,
+    const int32_t* __restrict__ lcol, const float* __restrict__ lval,
+    const int32_t* __restrict__ ucol, const float* __restrict__ uval, const float* __restrict__ invd,
+    const uint16_t* __restrict__ hin, uint16_t* __restrict__ hout, const uint16_t* __restrict__ wl,
+    const float* __restrict__ bl, const uint16_t* __restrict__ wo, const float2* __restrict__ F,
+    const float* __restrict__ scal, float* __restrict__ z) {
+  __shared__ __align__(16) uint16_t As[kWaves][64][kLdA];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, f = lane & 31, half = lane >> 5;
+  const int64_t pb = xcd_block(blockIdx.x, nbw);
+  const int64_t pos = pb * kWaves + wv;
+  const bool live = pb >= 0 && pos < nch;
+  SellChunk k{0, 0, 0, 0, 0, 0};
+  if (live) k = ch[sched[pos]];
+  const bool act = lane < k.nrows;
+  const int row = k.row0 + (act ? lane : 0);
+  // aggregation (lanes of a dead wave or past the chunk's rows carry zeros)
+  float acc[kH];
+#pragma unroll
+  for (int t = 0; t < kH; ++t) acc[t] = 0.f;
+  uint4 own[4] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+  float mscale = 0.f;
+  if (live) {
+    const uint4* hp = reinterpret_cast<const uint4*>(hin + (int64_t)row * kH);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) own[q] = hp[q];
+    mscale = -invd[row];
+    // neighbours in groups of G: indices and weights first, then 4 G independent
+    // 16-byte row loads in flight per lane
+    for (int part = 0; part < 2; ++part) {
+      const int w = part ? k.uw : k.lw;
+      const int32_t* cc = (part ? ucol + k.uoff : lcol + k.loff) + lane;
+      const float* vv = (part ? uval + k.uoff : lval + k.loff) + lane;
+      for (int q = 0; q < w; q += G) {
+        int col[G];
+        float wt[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const bool ok = q + u < w;
+          col[u] = ok ? cc[(q + u) * 64] : row;
+          wt[u] = ok ? vv[(q + u) * 64] : 0.f;
+        }
+        uint4 hv[G][4];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const uint4* p = reinterpret_cast<const uint4*>(hin + (int64_t)col[u] * kH);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) hv[u][t] = p[t];
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const uint32_t e[4] = {hv[u][t].x, hv[u][t].y, hv[u][t].z, hv[u][t].w};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              acc[8 * t + 2 * v] = fmaf(wt[u], __uint_as_float(e[v] << 16), acc[8 * t + 2 * v]);
+              acc[8 * t + 2 * v + 1] = fmaf(wt[u], __uint_as_float(e[v] & 0xFFFF0000u), acc[8 * t + 2 * v + 1]);
+            }
+          }
+      }
+    }
+  }
+  {
+    uint4* dst = reinterpret_cast<uint4*>(&As[wv][lane][0]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = own[q];
+    uint32_t pk[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      pk[t] = (uint32_t)f2bf(mscale * acc[2 * t]) | ((uint32_t)f2bf(mscale * acc[2 * t + 1]) << 16);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[4 + q] = make_uint4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
+  }
+  __syncthreads();
+  u16x8 wfrag[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) wfrag[ks] = *reinterpret_cast<const u16x8*>(wl + f * 2 * kH + ks * 16 + 8 * half);
+  const float bias = bl[f];
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt) {
+    f32x16 d = {};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const u16x8 a = *reinterpret_cast<const u16x8*>(&As[wv][mt * 32 + f][ks * 16 + 8 * half]);
+      d = mfma_bf16(a, wfrag[ks], d);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int nrow = mt * 32 + acc_row(r, half);
+      const float hv = bf2f(As[wv][nrow][f]);
+      As[wv][nrow][f] = f2bf(hv + fmaxf(d[r] + bias, 0.f));
+    }
+  }
+  __syncthreads();
+  if (!live || !act) return;
+  if (!LAST) {
+    const uint4* src = reinterpret_cast<const uint4*>(&As[wv][lane][0]);
+    uint4* o = reinterpret_cast<uint4*>(hout + (int64_t)row * kH);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = src[q];
+  } else {
+    float out = 0.f;
+#pragma unroll
+    for (int t = 0; t < kH; ++t) out = fmaf(bf2f(wo[t]), bf2f(As[wv][lane][t]), out);
+    const float2 ff = F[row];
+    out = fmaf(bf2f(wo[kH]), ff.x, out);
+    out = fmaf(bf2f(wo[kH + 1]), ff.y, out);
+    z[row] = out * scal[1];
+  }
 }
 
 // One message-passing layer.  A wavefront owns 32 consecutive nodes: it
@@ -854,17 +1007,27 @@ void apply_perm(nalu_linsys* ls, Learned* st, const nalu_gnn* g, const float* r,
   hipLaunchKernelGGL((k_bdot<float, float>), dim3(nb, 1), dim3(kThreads), 0, s, r, r, n, st->part.p);
   hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, st->part.p, (int)nb, st->sumsq.p, 1, 0);
   hipLaunchKernelGGL(k_scale, dim3(1), dim3(64), 0, s, st->sumsq.p, 1, n, st->dsum.p, st->scal.p);
-  hipLaunchKernelGGL(k_encode, dim3(nblocks(n * kH)), dim3(kThreads), 0, s, r, st->zs.p, st->delta.p, st->dsum.p,
-                     st->scal.p, n, n, g->wbf.p, g->theta.p, st->h0.p, st->F.p);
+  hipLaunchKernelGGL(k_encode_rows, dim3(nblocks(n)), dim3(kThreads), 0, s, r, st->zs.p, st->delta.p, st->dsum.p,
+                     st->scal.p, n, g->wbf.p, g->theta.p, st->h0.p, st->F.p);
+  // layers on the SELL-64 chunks of the solver copy (FP32 values, w = -a / d)
+  const SellChunk* ch = reinterpret_cast<const SellChunk*>(ls->sell_chunks.p);
+  const int64_t nch = ls->sell_nchunks;
+  const int64_t nbl = std::max<int64_t>(1, (nch + kWaves - 1) / kWaves);
   uint16_t *hin = st->h0.p, *hout = st->h1.p;
   for (int l = 0; l < g->L; ++l) {
-    hipLaunchKernelGGL(k_layer<false>, dim3(nblocks(n, kTile * kWaves)), dim3(kThreads), 0, s, st->gptr.p,
-                       st->gcol.p, st->gw.p, n, n, hin, hout, g->wbf.p + lay.w(l), g->theta.p + lay.b(l),
-                       (uint16_t*)nullptr, (uint32_t*)nullptr);
+    const bool last = l == g->L - 1;
+    static const int group = [] {
+      const char* e = std::getenv("NALU_GNN_GATHER");
+      return (e && std::atoi(e) == 2) ? 2 : 4;
+    }();
+    auto kern = group == 2 ? (last ? k_layer_sell<true, 2> : k_layer_sell<false, 2>)
+                           : (last ? k_layer_sell<true, 4> : k_layer_sell<false, 4>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)xcd_grid(nbl)),
+                       dim3(kThreads), 0, s, ch, ls->sell_sched.p, nch, nbl,
+                       ls->sl_col.p, ls->sl_val32.p, ls->su_col.p, ls->su_val32.p, ls->pinvd32.p, hin, hout,
+                       g->wbf.p + lay.w(l), g->theta.p + lay.b(l), g->wbf.p + lay.w_out(), st->F.p, st->scal.p, z);
     std::swap(hin, hout);
   }
-  hipLaunchKernelGGL(k_output<float>, dim3(nblocks(n)), dim3(kThreads), 0, s, hin, st->F.p, g->wbf.p + lay.w_out(),
-                     st->scal.p, n, n, 0, z, (int*)nullptr);
 }
 
 // ---------------------------------------------------------------- batch work
@@ -1145,12 +1308,11 @@ extern "C" int nalu_gnn_apply(nalu_gnn* g, nalu_linsys* ls, const float* r_dev, 
     const int64_t n = ls->nrows;
     hipStream_t s = ls->stream;
     nalu::ensure_one(st, n);
-    DevBuf<float> zp(n);
     hipLaunchKernelGGL((nalu::k_gather_perm<float, float>), dim3(nalu::nblocks(n)), dim3(nalu::kThreads), 0, s, r_dev,
                        ls->perm.p, n, 1, st->r32.p);
-    nalu::apply_perm(ls, st, g, st->r32.p, zp.p);
+    nalu::apply_perm(ls, st, g, st->r32.p, st->z32.p);
     hipLaunchKernelGGL((nalu::k_scatter_perm<float, float>), dim3(nalu::nblocks(n)), dim3(nalu::kThreads), 0, s,
-                       zp.p, ls->perm.p, n, 1, z_dev);
+                       st->z32.p, ls->perm.p, n, 1, z_dev);
     NALU_HIP_CHECK(hipStreamSynchronize(s));
     return NALU_OK;
   });

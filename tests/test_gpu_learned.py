@@ -94,6 +94,35 @@ def test_forward_loss_grad_match_oracle(oracle):
         assert err <= 2e-2, name
 
 
+def test_inference_path_matches_oracle(oracle):
+    """The solver's inference kernels (SELL-64 layers, fused output) against the
+    oracle's forward pass with the same weights."""
+    print("case: learned inference vs oracle, n=7", flush=True)
+    _, _, ls = poisson_system(7)
+    rp, ci, v, _ = ls.csr()
+    n = len(rp) - 1
+    gr = G.Graph(rp, ci, v)
+    net = na.LearnedPreconditioner(num_layers=4, seed=9)
+    o = G.layout(4)
+    rng = np.random.default_rng(8)
+    th = net.get_params().astype(np.float64)
+    th[o["w_out"]:o["c"]] = rng.normal(0, 0.3, 32)
+    th[o["c"]] = 0.2
+    for l in range(4):
+        th[o[f"b{l}"]:o[f"b{l}"] + 32] = rng.normal(0, 0.1, 32)
+    th = th.astype(np.float32)
+    net.set_params(th)
+    r = rng.standard_normal(n).astype(np.float32)
+    z = torch.zeros(n, dtype=torch.float32, device=DEV)
+    net.apply(ls, dev(r, torch.float32), z)
+    zs = torch.zeros(1, n, dtype=torch.float32, device=DEV)
+    net.loss_grad(ls, dev(r[None], torch.float32), dev(r[None], torch.float32), zs=zs)
+    want = G.apply(th.astype(np.float64), gr, r[None].astype(np.float64), zs.cpu().numpy().astype(np.float64))[0]
+    err = rel(z.cpu().numpy().astype(np.float64), want)
+    print("  z rel %.2e" % err, flush=True)
+    assert err <= 5e-3
+
+
 def test_untrained_network_is_sgs():
     print("case: untrained network == SGS, n=8", flush=True)
     _, _, ls = poisson_system(8)
