@@ -193,7 +193,16 @@ int nalu_assemble_scalar_diff(nalu_linsys *ls, const nalu_mesh *m, const double 
 int nalu_compute_mdot(const nalu_mesh *m, const double *rho_dev, const double *p_dev,
                       const double *u_dev, const double *gpdx_dev, double proj_time_scale,
                       int shift_mdot, int shift_poisson, double *mdot_dev, void *stream);
-/* ContinuityAdvElemKernel (src/kernel/ContinuityAdvElemKernel.C:88-148). */
+/* The same with FP32 arithmetic (Jacobians and determinants in FP64, nodal
+ * coordinates taken relative to each element's first node); the mixed-precision
+ * path.  No counterpart in the reference. */
+int nalu_compute_mdot_f32(const nalu_mesh *m, const double *rho_dev, const double *p_dev,
+                          const double *u_dev, const double *gpdx_dev, double proj_time_scale,
+                          int shift_mdot, int shift_poisson, double *mdot_dev, void *stream);
+/* ContinuityAdvElemKernel (src/kernel/ContinuityAdvElemKernel.C:88-148).
+ * Assembly kernels evaluate with FP32 arithmetic (FP64 Jacobians, FP64
+ * accumulation into the matrix) when the system was created with precision
+ * NALU_PRECISION_MIXED, in FP64 otherwise. */
 int nalu_assemble_continuity(nalu_linsys *ls, const nalu_mesh *m, const double *rho_dev,
                              const double *p_dev, const double *u_dev, const double *gpdx_dev,
                              double proj_time_scale, int shift_mdot, int shift_poisson,
@@ -234,7 +243,8 @@ int nalu_gnn_loss_grad(nalu_gnn *g, nalu_linsys *ls, int batch, const float *r_d
                        float *zs_dev, float *out_dev, double *loss_host, float *grad_host);
 /* Harvest num_rhs * pairs_per_rhs (residual, error) pairs from plain FP64 CG +
  * SGS solves (to relative residual tol) of the system's assembled matrix with
- * seeded unit-variance Gaussian zero-mean right-hand sides. */
+ * seeded unit-variance Gaussian zero-mean right-hand sides (pure-Neumann systems:
+ * each error is stored with its mean removed). */
 int nalu_gnn_harvest(nalu_linsys *ls, int num_rhs, int pairs_per_rhs, uint64_t seed, double tol,
                      int max_iterations, nalu_gnn_data **out);
 int nalu_gnn_data_destroy(nalu_gnn_data *d);

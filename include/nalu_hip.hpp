@@ -135,9 +135,10 @@ inline void assembleScalarDiff(LinearSystem& ls, const Mesh& m, const double* qD
 }
 inline void computeMdot(const Mesh& m, const double* rhoDev, const double* pDev, const double* uDev,
                         const double* gpdxDev, double projTimeScale, double* mdotDev, bool shiftMdot = false,
-                        bool shiftPoisson = false) {
-  check(nalu_compute_mdot(m.get(), rhoDev, pDev, uDev, gpdxDev, projTimeScale, shiftMdot, shiftPoisson, mdotDev,
-                          nullptr), "ComputeMdotElemAlgorithm");
+                        bool shiftPoisson = false, bool fp32Arithmetic = false) {
+  auto fn = fp32Arithmetic ? nalu_compute_mdot_f32 : nalu_compute_mdot;
+  check(fn(m.get(), rhoDev, pDev, uDev, gpdxDev, projTimeScale, shiftMdot, shiftPoisson, mdotDev, nullptr),
+        "ComputeMdotElemAlgorithm");
 }
 inline void assembleContinuity(LinearSystem& ls, const Mesh& m, const double* rhoDev, const double* pDev,
                                const double* uDev, const double* gpdxDev, double projTimeScale,

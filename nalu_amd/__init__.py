@@ -104,6 +104,7 @@ def lib():
         L.nalu_linsys_solve_rhs.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int), _dp]
         L.nalu_assemble_scalar_diff.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int]
         L.nalu_compute_mdot.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp]
+        L.nalu_compute_mdot_f32.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp]
         L.nalu_assemble_continuity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int,
                                                C.c_int, C.c_int]
         L.nalu_assemble_momentum.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_int]
@@ -514,9 +515,13 @@ def assemble_scalar_diff(ls, mesh, q, k, shifted=False, mode="tile"):
            "assemble_scalar_diff")
 
 
-def compute_mdot(mesh, rho, p, u, gpdx, mdot, tau=1.0, shift_mdot=False, shift_poisson=False, stream=None):
-    _check(lib().nalu_compute_mdot(mesh.handle, _ptr(rho), _ptr(p), _ptr(u), _ptr(gpdx), float(tau),
-                                   int(shift_mdot), int(shift_poisson), _ptr(mdot), stream), "compute_mdot")
+def compute_mdot(mesh, rho, p, u, gpdx, mdot, tau=1.0, shift_mdot=False, shift_poisson=False, stream=None,
+                 precision="fp64"):
+    """mdot at the 12 scs of every element; precision "fp32" evaluates with FP32
+    arithmetic (FP64 Jacobians), as the mixed-precision path does."""
+    fn = lib().nalu_compute_mdot_f32 if precision == "fp32" else lib().nalu_compute_mdot
+    _check(fn(mesh.handle, _ptr(rho), _ptr(p), _ptr(u), _ptr(gpdx), float(tau), int(shift_mdot), int(shift_poisson),
+              _ptr(mdot), stream), "compute_mdot")
 
 
 def assemble_continuity(ls, mesh, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False,

@@ -37,6 +37,22 @@ __device__ __forceinline__ void load_vec3(const double* __restrict__ f, const in
     for (int d = 0; d < 3; ++d) v[n][d] = R(f[3 * (int64_t)nd[n] + d]);
 }
 
+// Element coordinates in the arithmetic type R.  The FP32 path works on
+// coordinates relative to the element's first node, the differences formed in
+// FP64, so that short edges of stretched elements keep their relative accuracy.
+template <typename R>
+__device__ __forceinline__ void load_coords(const double* __restrict__ f, const int32_t (&nd)[8], R (&x)[8][3]) {
+  if constexpr (std::is_same<R, double>::value) {
+    load_vec3(f, nd, x);
+  } else {
+    const double o[3] = {f[3 * (int64_t)nd[0]], f[3 * (int64_t)nd[0] + 1], f[3 * (int64_t)nd[0] + 2]};
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) x[n][d] = R(f[3 * (int64_t)nd[n] + d] - o[d]);
+  }
+}
+
 template <typename R>
 __device__ __forceinline__ void load_scalar(const double* __restrict__ f, const int32_t (&nd)[8], R (&v)[8]) {
 #pragma unroll
@@ -45,37 +61,41 @@ __device__ __forceinline__ void load_scalar(const double* __restrict__ f, const 
 
 // ---------------------------------------------------------------- physics ops
 // Each op evaluates row LA of its element matrix, calling emit(c, value) once
-// per column node c, and returns the row's rhs contribution.
-template <bool SHIFT>
+// per column node c, and returns the row's rhs contribution.  R is the
+// arithmetic type (double, or float for the mixed-precision path); Jacobians
+// and their determinants are always formed in FP64.
+template <bool SHIFT, typename R = double>
 struct ScalarDiffOp {
+  using real = R;
   const double* q;
   const double* k;
   template <int LA, class Emit>
-  __device__ __forceinline__ double row(const int32_t (&nd)[8], const double (&x)[8][3], Emit&& emit) const {
+  __device__ __forceinline__ R row(const int32_t (&nd)[8], const R (&x)[8][3], Emit&& emit) const {
     const hex8::Gather gt{nd};
     return hex8::row_scalar_diff<LA, SHIFT, double>(x, gt, q, k, emit);
   }
 };
 
-template <bool SHIFT_MDOT, bool SHIFT_POISSON, bool REDUCED>
+template <bool SHIFT_MDOT, bool SHIFT_POISSON, bool REDUCED, typename R = double>
 struct ContinuityOp {
+  using real = R;
   const double* rho;
   const double* p;
   const double* u;
   const double* gp;
   double tau;
   template <int LA, class Emit>
-  __device__ __forceinline__ double row(const int32_t (&nd)[8], const double (&x)[8][3], Emit&& emit) const {
+  __device__ __forceinline__ R row(const int32_t (&nd)[8], const R (&x)[8][3], Emit&& emit) const {
     const hex8::Gather gt{nd};
-    return hex8::row_continuity<LA, SHIFT_MDOT, SHIFT_POISSON, REDUCED, double>(x, gt, rho, p, u, gp, tau, emit);
+    return hex8::row_continuity<LA, SHIFT_MDOT, SHIFT_POISSON, REDUCED, double>(x, gt, rho, p, u, gp, R(tau), emit);
   }
 };
 
 // dispatch a wave-uniform (or mostly uniform) runtime row index to the
 // compile-time specialisation
 template <class Op, class Emit>
-__device__ __forceinline__ double dispatch_row(const Op& op, int la, const int32_t (&nd)[8], const double (&x)[8][3],
-                                               Emit&& emit) {
+__device__ __forceinline__ typename Op::real dispatch_row(const Op& op, int la, const int32_t (&nd)[8],
+                                                          const typename Op::real (&x)[8][3], Emit&& emit) {
   switch (la) {
     case 0: return op.template row<0>(nd, x, emit);
     case 1: return op.template row<1>(nd, x, emit);
@@ -129,13 +149,13 @@ __device__ __forceinline__ void scalar_atomic_body(const Op& op, const int32_t* 
                                                    double* __restrict__ vals, double* __restrict__ rhs) {
   int32_t nd[8];
   load_nodes(conn, e, nd);
-  double x[8][3];
-  load_vec3(coords, nd, x);
+  typename Op::real x[8][3];
+  load_coords(coords, nd, x);
   const int32_t row = lid[nd[LA]];
   double* rowp = vals + nrp[row];
   const uint64_t pk = elem_pos[8 * e + LA];
-  const double rr = op.template row<LA>(nd, x, [&](int c, double v) {
-    unsafeAtomicAdd(rowp + (int)((pk >> (8 * c)) & 0xff), v);
+  const double rr = (double)op.template row<LA>(nd, x, [&](int c, auto v) {
+    unsafeAtomicAdd(rowp + (int)((pk >> (8 * c)) & 0xff), (double)v);
   });
   unsafeAtomicAdd(rhs + row, rr);
 }
@@ -188,10 +208,11 @@ __global__ __launch_bounds__(256) void k_scalar_gather(Op op, const int32_t* __r
       const int la = t & 7;
       int32_t nd[8];
       load_nodes(conn, e, nd);
-      double x[8][3];
-      load_vec3(coords, nd, x);
+      typename Op::real x[8][3];
+      load_coords(coords, nd, x);
       const uint64_t pk = elem_pos[t];
-      rr += dispatch_row(op, la, nd, x, [&](int c, double v) { myrow[(int)((pk >> (8 * c)) & 0xff)] += v; });
+      rr += (double)dispatch_row(op, la, nd, x,
+                                 [&](int c, auto v) { myrow[(int)((pk >> (8 * c)) & 0xff)] += (double)v; });
     }
   }
   __syncthreads();
@@ -246,11 +267,11 @@ __global__ __launch_bounds__(kTileThreads) void k_scalar_tile(Op op, const int32
       const int64_t e = t >> 3;
       int32_t nd[8];
       load_nodes(conn, e, nd);
-      double x[8][3];
-      load_vec3(coords, nd, x);
+      typename Op::real x[8][3];
+      load_coords(coords, nd, x);
       const uint64_t pk = elem_pos[t];
-      acc += dispatch_row(op, slot, nd, x, [&](int c, double v) {
-        unsafeAtomicAdd(myrow + (int)((pk >> (8 * c)) & 0xff), v);
+      acc += (double)dispatch_row(op, slot, nd, x, [&](int c, auto v) {
+        unsafeAtomicAdd(myrow + (int)((pk >> (8 * c)) & 0xff), (double)v);
       });
     }
     if (acc != 0.0) unsafeAtomicAdd(rseg + j, acc);
@@ -265,25 +286,28 @@ __global__ __launch_bounds__(kTileThreads) void k_scalar_tile(Op op, const int32
   }
 }
 
-template <int LA, bool SKEW, bool SHIFT>
-__device__ __forceinline__ void momentum_row_lds(const int32_t (&nd)[8], const double (&x)[8][3],
+template <int LA, bool SKEW, bool SHIFT, typename R>
+__device__ __forceinline__ void momentum_row_lds(const int32_t (&nd)[8], const R (&x)[8][3],
                                                  const double* __restrict__ u, const double* __restrict__ mu,
                                                  const double* __restrict__ md, double incl_divu, double* myrow,
                                                  int32_t L, uint64_t pk, double (&rh)[3]) {
   const hex8::Gather gt{nd};
-  const double m3[3] = {md[hex8::adj_ip(LA, 0)], md[hex8::adj_ip(LA, 1)], md[hex8::adj_ip(LA, 2)]};
-  hex8::row_momentum<LA, SKEW, SHIFT, double>(x, gt, u, mu, m3, incl_divu, rh, [&](int c, const double (&blk)[3][3]) {
+  const R m3[3] = {R(md[hex8::adj_ip(LA, 0)]), R(md[hex8::adj_ip(LA, 1)]), R(md[hex8::adj_ip(LA, 2)])};
+  R rr[3] = {R(0), R(0), R(0)};
+  hex8::row_momentum<LA, SKEW, SHIFT, double>(x, gt, u, mu, m3, R(incl_divu), rr, [&](int c, const R (&blk)[3][3]) {
     const int p = (int)((pk >> (8 * c)) & 0xff);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int jj = 0; jj < 3; ++jj) unsafeAtomicAdd(myrow + 3 * i * L + 3 * p + jj, blk[i][jj]);
+      for (int jj = 0; jj < 3; ++jj) unsafeAtomicAdd(myrow + 3 * i * L + 3 * p + jj, (double)blk[i][jj]);
   });
+#pragma unroll
+  for (int i = 0; i < 3; ++i) rh[i] += (double)rr[i];
 }
 
 // Momentum (numDof 3) node-tile gather: the block's segment holds the 3 rows of
 // each of its nodes (rows 3a..3a+2 are contiguous in the CSR).
-template <bool SKEW, bool SHIFT>
+template <bool SKEW, bool SHIFT, typename R>
 __global__ __launch_bounds__(kTileThreads) void k_momentum_tile(
     const int32_t* __restrict__ conn, const double* __restrict__ coords, const double* __restrict__ u,
     const double* __restrict__ mu, const double* __restrict__ mdot, double incl_divu,
@@ -312,13 +336,13 @@ __global__ __launch_bounds__(kTileThreads) void k_momentum_tile(
       const int64_t e = t >> 3;
       int32_t nd[8];
       load_nodes(conn, e, nd);
-      double x[8][3];
-      load_vec3(coords, nd, x);
+      R x[8][3];
+      load_coords(coords, nd, x);
       const uint64_t pk = elem_pos[t];
       const double* md = mdot + 12 * e;
       switch (slot) {
 #define NALU_MT(Lx) \
-  case Lx: momentum_row_lds<Lx, SKEW, SHIFT>(nd, x, u, mu, md, incl_divu, myrow, L, pk, rh); break;
+  case Lx: momentum_row_lds<Lx, SKEW, SHIFT, R>(nd, x, u, mu, md, incl_divu, myrow, L, pk, rh); break;
         NALU_MT(0) NALU_MT(1) NALU_MT(2) NALU_MT(3) NALU_MT(4) NALU_MT(5) NALU_MT(6) NALU_MT(7)
 #undef NALU_MT
       }
@@ -339,24 +363,27 @@ __global__ __launch_bounds__(kTileThreads) void k_momentum_tile(
 }
 
 // ---------------------------------------------------------------- momentum
-template <int LA, bool SKEW, bool SHIFT>
-__device__ __forceinline__ void momentum_row_atomic(const int32_t (&nd)[8], const double (&x)[8][3],
+template <int LA, bool SKEW, bool SHIFT, typename R>
+__device__ __forceinline__ void momentum_row_atomic(const int32_t (&nd)[8], const R (&x)[8][3],
                                                     const double* __restrict__ u, const double* __restrict__ mu,
                                                     const double* __restrict__ md, double incl_divu,
                                                     double* __restrict__ rowp, int32_t L, uint64_t pk,
                                                     double (&rh)[3]) {
   const hex8::Gather gt{nd};
-  const double m3[3] = {md[hex8::adj_ip(LA, 0)], md[hex8::adj_ip(LA, 1)], md[hex8::adj_ip(LA, 2)]};
-  hex8::row_momentum<LA, SKEW, SHIFT, double>(x, gt, u, mu, m3, incl_divu, rh, [&](int c, const double (&blk)[3][3]) {
+  const R m3[3] = {R(md[hex8::adj_ip(LA, 0)]), R(md[hex8::adj_ip(LA, 1)]), R(md[hex8::adj_ip(LA, 2)])};
+  R rr[3] = {R(0), R(0), R(0)};
+  hex8::row_momentum<LA, SKEW, SHIFT, double>(x, gt, u, mu, m3, R(incl_divu), rr, [&](int c, const R (&blk)[3][3]) {
     const int p = (int)((pk >> (8 * c)) & 0xff);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) unsafeAtomicAdd(rowp + 3 * i * L + 3 * p + j, blk[i][j]);
+      for (int j = 0; j < 3; ++j) unsafeAtomicAdd(rowp + 3 * i * L + 3 * p + j, (double)blk[i][j]);
   });
+#pragma unroll
+  for (int i = 0; i < 3; ++i) rh[i] += (double)rr[i];
 }
 
-template <int LA, bool SKEW, bool SHIFT>
+template <int LA, bool SKEW, bool SHIFT, typename R>
 __device__ __forceinline__ void momentum_body(const int32_t* __restrict__ conn, const double* __restrict__ coords,
                                               const double* __restrict__ u, const double* __restrict__ mu,
                                               const double* __restrict__ mdot, double incl_divu,
@@ -365,20 +392,20 @@ __device__ __forceinline__ void momentum_body(const int32_t* __restrict__ conn, 
                                               double* __restrict__ vals, double* __restrict__ rhs) {
   int32_t nd[8];
   load_nodes(conn, e, nd);
-  double x[8][3];
-  load_vec3(coords, nd, x);
+  R x[8][3];
+  load_coords(coords, nd, x);
   const int32_t nr = lid[nd[LA]];
   const int32_t nb = nrp[nr];
   const int32_t L = nrp[nr + 1] - nb;
   double* rowp = vals + 9 * (int64_t)nb;
   const uint64_t pk = elem_pos[8 * e + LA];
   double rh[3] = {0, 0, 0};
-  momentum_row_atomic<LA, SKEW, SHIFT>(nd, x, u, mu, mdot + 12 * e, incl_divu, rowp, L, pk, rh);
+  momentum_row_atomic<LA, SKEW, SHIFT, R>(nd, x, u, mu, mdot + 12 * e, incl_divu, rowp, L, pk, rh);
 #pragma unroll
   for (int i = 0; i < 3; ++i) unsafeAtomicAdd(rhs + 3 * (int64_t)nr + i, rh[i]);
 }
 
-template <bool SKEW, bool SHIFT>
+template <bool SKEW, bool SHIFT, typename R>
 __global__ __launch_bounds__(kChunk) void k_momentum_atomic(
     const int32_t* __restrict__ conn, const double* __restrict__ coords, const double* __restrict__ u,
     const double* __restrict__ mu, const double* __restrict__ mdot, double incl_divu,
@@ -389,7 +416,7 @@ __global__ __launch_bounds__(kChunk) void k_momentum_atomic(
   switch (cm.la) {
 #define NALU_MA(L)                                                                                          \
   case L:                                                                                                   \
-    momentum_body<L, SKEW, SHIFT>(conn, coords, u, mu, mdot, incl_divu, lid, nrp, elem_pos, cm.e, vals, rhs); \
+    momentum_body<L, SKEW, SHIFT, R>(conn, coords, u, mu, mdot, incl_divu, lid, nrp, elem_pos, cm.e, vals, rhs); \
     break;
     NALU_MA(0) NALU_MA(1) NALU_MA(2) NALU_MA(3) NALU_MA(4) NALU_MA(5) NALU_MA(6) NALU_MA(7)
 #undef NALU_MA
@@ -397,15 +424,15 @@ __global__ __launch_bounds__(kChunk) void k_momentum_atomic(
 }
 
 // ---------------------------------------------------------------- mdot
-template <int IP, bool SHIFT_MDOT, bool SHIFT_POISSON>
-__device__ __forceinline__ double mdot_one(const int32_t (&nd)[8], const double (&x)[8][3], const double* rho,
+template <int IP, bool SHIFT_MDOT, bool SHIFT_POISSON, typename R>
+__device__ __forceinline__ double mdot_one(const int32_t (&nd)[8], const R (&x)[8][3], const double* rho,
                                            const double* p, const double* u, const double* gp, double tau) {
-  hex8::IpGeo<double, double> G;
+  hex8::IpGeo<double, R> G;
   hex8::ip_geo<IP, SHIFT_POISSON>(x, G);
-  return hex8::ip_mdot_stream<IP, SHIFT_MDOT, SHIFT_POISSON>(G, hex8::Gather{nd}, rho, p, u, gp, tau);
+  return (double)hex8::ip_mdot_stream<IP, SHIFT_MDOT, SHIFT_POISSON>(G, hex8::Gather{nd}, rho, p, u, gp, R(tau));
 }
 
-template <bool SHIFT_MDOT, bool SHIFT_POISSON>
+template <bool SHIFT_MDOT, bool SHIFT_POISSON, typename R>
 __global__ __launch_bounds__(kChunk) void k_mdot(const int32_t* __restrict__ conn, const double* __restrict__ coords,
                                                  const double* __restrict__ rho, const double* __restrict__ p,
                                                  const double* __restrict__ u, const double* __restrict__ gp,
@@ -414,12 +441,12 @@ __global__ __launch_bounds__(kChunk) void k_mdot(const int32_t* __restrict__ con
   if (!cm.valid) return;
   int32_t nd[8];
   load_nodes(conn, cm.e, nd);
-  double x[8][3];
-  load_vec3(coords, nd, x);
+  R x[8][3];
+  load_coords(coords, nd, x);
   double m = 0.0;
   switch (cm.la) {
 #define NALU_MD(IP) \
-  case IP: m = mdot_one<IP, SHIFT_MDOT, SHIFT_POISSON>(nd, x, rho, p, u, gp, tau); break;
+  case IP: m = mdot_one<IP, SHIFT_MDOT, SHIFT_POISSON, R>(nd, x, rho, p, u, gp, tau); break;
     NALU_MD(0) NALU_MD(1) NALU_MD(2) NALU_MD(3) NALU_MD(4) NALU_MD(5)
     NALU_MD(6) NALU_MD(7) NALU_MD(8) NALU_MD(9) NALU_MD(10) NALU_MD(11)
 #undef NALU_MD
@@ -521,6 +548,25 @@ void launch_scalar(nalu_linsys* ls, const nalu_mesh* m, const Op& op, int mode) 
   NALU_HIP_CHECK(hipGetLastError());
 }
 
+// Systems created for the mixed-precision path assemble with FP32 arithmetic.
+bool fp32_assembly(const nalu_linsys* ls) { return ls->cfg.precision == NALU_PRECISION_MIXED; }
+
+template <typename R>
+void launch_mdot(const nalu_mesh* m, const double* rho, const double* p, const double* u, const double* gp,
+                 double tau, int shift_mdot, int shift_poisson, double* mdot, hipStream_t s) {
+  const int64_t E = m->num_elems;
+  const dim3 g(chunk_blocks<12>(E)), b(kChunk);
+  if (shift_mdot && shift_poisson)
+    hipLaunchKernelGGL((k_mdot<true, true, R>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
+  else if (shift_mdot)
+    hipLaunchKernelGGL((k_mdot<true, false, R>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
+  else if (shift_poisson)
+    hipLaunchKernelGGL((k_mdot<false, true, R>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
+  else
+    hipLaunchKernelGGL((k_mdot<false, false, R>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
+  NALU_HIP_CHECK(hipGetLastError());
+}
+
 void check_flag(DevBuf<int>& flag, hipStream_t s, const char* msg) {
   int h = 0;
   NALU_HIP_CHECK(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -555,8 +601,13 @@ extern "C" int nalu_assemble_scalar_diff(nalu_linsys* ls, const nalu_mesh* m, co
     NALU_REQUIRE(q && k, NALU_ERR_ARG, "scalar_diff: null field");
     NALU_REQUIRE(mode >= 0 && mode <= 2, NALU_ERR_ARG, "scalar_diff: mode must be 0, 1 or 2");
     PhaseTimer t(ls, PH_ASSEMBLE);
-    if (shifted) launch_scalar(ls, m, ScalarDiffOp<true>{q, k}, mode);
-    else launch_scalar(ls, m, ScalarDiffOp<false>{q, k}, mode);
+    if (fp32_assembly(ls)) {
+      if (shifted) launch_scalar(ls, m, ScalarDiffOp<true, float>{q, k}, mode);
+      else launch_scalar(ls, m, ScalarDiffOp<false, float>{q, k}, mode);
+    } else {
+      if (shifted) launch_scalar(ls, m, ScalarDiffOp<true>{q, k}, mode);
+      else launch_scalar(ls, m, ScalarDiffOp<false>{q, k}, mode);
+    }
     return NALU_OK;
   });
 }
@@ -572,8 +623,11 @@ extern "C" int nalu_assemble_continuity(nalu_linsys* ls, const nalu_mesh* m, con
     PhaseTimer t(ls, PH_ASSEMBLE);
     const int key = (shift_mdot ? 4 : 0) | (shift_poisson ? 2 : 0) | (reduced_sens ? 1 : 0);
     switch (key) {
-#define NALU_CT(K, A, B, C) \
-  case K: launch_scalar(ls, m, ContinuityOp<A, B, C>{rho, p, u, gp, tau}, mode); break;
+#define NALU_CT(K, A, B, C)                                                          \
+  case K:                                                                            \
+    if (fp32_assembly(ls)) launch_scalar(ls, m, ContinuityOp<A, B, C, float>{rho, p, u, gp, tau}, mode); \
+    else launch_scalar(ls, m, ContinuityOp<A, B, C>{rho, p, u, gp, tau}, mode);       \
+    break;
       NALU_CT(0, false, false, false)
       NALU_CT(1, false, false, true)
       NALU_CT(2, false, true, false)
@@ -594,18 +648,18 @@ extern "C" int nalu_compute_mdot(const nalu_mesh* m, const double* rho, const do
   return guarded([&]() -> int {
     NALU_REQUIRE(m && rho && p && u && gp && mdot, NALU_ERR_ARG, "mdot: null argument");
     NALU_REQUIRE(tau > 0.0, NALU_ERR_ARG, "mdot: projTimeScale must be positive");
-    hipStream_t s = (hipStream_t)stream;
-    const int64_t E = m->num_elems;
-    const dim3 g(chunk_blocks<12>(E)), b(kChunk);
-    if (shift_mdot && shift_poisson)
-      hipLaunchKernelGGL((k_mdot<true, true>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
-    else if (shift_mdot)
-      hipLaunchKernelGGL((k_mdot<true, false>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
-    else if (shift_poisson)
-      hipLaunchKernelGGL((k_mdot<false, true>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
-    else
-      hipLaunchKernelGGL((k_mdot<false, false>), g, b, 0, s, m->conn.p, m->coords.p, rho, p, u, gp, tau, E, mdot);
-    NALU_HIP_CHECK(hipGetLastError());
+    launch_mdot<double>(m, rho, p, u, gp, tau, shift_mdot, shift_poisson, mdot, (hipStream_t)stream);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_compute_mdot_f32(const nalu_mesh* m, const double* rho, const double* p, const double* u,
+                                     const double* gp, double tau, int shift_mdot, int shift_poisson, double* mdot,
+                                     void* stream) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(m && rho && p && u && gp && mdot, NALU_ERR_ARG, "mdot: null argument");
+    NALU_REQUIRE(tau > 0.0, NALU_ERR_ARG, "mdot: projTimeScale must be positive");
+    launch_mdot<float>(m, rho, p, u, gp, tau, shift_mdot, shift_poisson, mdot, (hipStream_t)stream);
     return NALU_OK;
   });
 }
@@ -632,10 +686,11 @@ extern "C" int nalu_assemble_momentum(nalu_linsys* ls, const nalu_mesh* m, const
                            m->inc_ptr.p, m->inc_elem.p, ls->nrp.p, ls->elem_pos.p, ls->num_nodes, ls->values.p,
                            ls->rhs.p, accumulate);
       };
-      if (skew && shifted) go(k_momentum_tile<true, true>);
-      else if (skew) go(k_momentum_tile<true, false>);
-      else if (shifted) go(k_momentum_tile<false, true>);
-      else go(k_momentum_tile<false, false>);
+      const bool f32 = fp32_assembly(ls);
+      if (skew && shifted) f32 ? go(k_momentum_tile<true, true, float>) : go(k_momentum_tile<true, true, double>);
+      else if (skew) f32 ? go(k_momentum_tile<true, false, float>) : go(k_momentum_tile<true, false, double>);
+      else if (shifted) f32 ? go(k_momentum_tile<false, true, float>) : go(k_momentum_tile<false, true, double>);
+      else f32 ? go(k_momentum_tile<false, false, float>) : go(k_momentum_tile<false, false, double>);
       ls->values_pending_zero = ls->rhs_pending_zero = false;
     } else {
       materialize_pending_zero(ls);
@@ -644,10 +699,11 @@ extern "C" int nalu_assemble_momentum(nalu_linsys* ls, const nalu_mesh* m, const
         hipLaunchKernelGGL(kern, g, b, 0, ls->stream, m->conn.p, m->coords.p, u, mu, mdot, include_divu,
                            ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, E, ls->values.p, ls->rhs.p);
       };
-      if (skew && shifted) go(k_momentum_atomic<true, true>);
-      else if (skew) go(k_momentum_atomic<true, false>);
-      else if (shifted) go(k_momentum_atomic<false, true>);
-      else go(k_momentum_atomic<false, false>);
+      const bool f32 = fp32_assembly(ls);
+      if (skew && shifted) f32 ? go(k_momentum_atomic<true, true, float>) : go(k_momentum_atomic<true, true, double>);
+      else if (skew) f32 ? go(k_momentum_atomic<true, false, float>) : go(k_momentum_atomic<true, false, double>);
+      else if (shifted) f32 ? go(k_momentum_atomic<false, true, float>) : go(k_momentum_atomic<false, true, double>);
+      else f32 ? go(k_momentum_atomic<false, false, float>) : go(k_momentum_atomic<false, false, double>);
     }
     NALU_HIP_CHECK(hipGetLastError());
     return NALU_OK;

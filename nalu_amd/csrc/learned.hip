@@ -867,14 +867,18 @@ __global__ void k_cg_dir(double* __restrict__ p, const double* __restrict__ z, c
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = z[k] + (rz_new[0] / rz_old[0]) * p[k];
 }
-// R = r / nr, E = (xs - xk) / nr in FP32
-__global__ void k_pair(const double* __restrict__ r, const double* __restrict__ xs, const double* __restrict__ xk,
+__global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b, int64_t n, double* __restrict__ c) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) c[k] = a[k] - b[k];
+}
+// R = r / nr, E = (e - mean e) / nr in FP32 (sum[0] = sum of e)
+__global__ void k_pair(const double* __restrict__ r, const double* __restrict__ e, const double* __restrict__ sum,
                        const double* __restrict__ nr2, int64_t n, float* __restrict__ R, float* __restrict__ E) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const double inv = 1.0 / std::sqrt(nr2[0]);
   R[k] = (float)(r[k] * inv);
-  E[k] = (float)((xs[k] - xk[k]) * inv);
+  E[k] = (float)((e[k] - sum[0] / (double)n) * inv);
 }
 __global__ void k_gather_rows(const float* __restrict__ src, const int32_t* __restrict__ idx, int64_t n, int nb,
                               float* __restrict__ dst) {
@@ -1445,8 +1449,14 @@ extern "C" int nalu_gnn_harvest(nalu_linsys* ls, int num_rhs, int pairs_per_rhs,
       for (int k = 0; k < pairs_per_rhs; ++k, ++pair) {
         const int64_t j = (int64_t)k * usable / pairs_per_rhs;  // evenly spread over the usable iterates
         dot_into(Rr.p + j * n, Rr.p + j * n, sc.p + 5);
-        hipLaunchKernelGGL(k_pair, dim3(nblocks(n)), dim3(kThreads), 0, s, Rr.p + j * n, x.p, X.p + j * n, sc.p + 5,
-                           n, d->R.p + (int64_t)pair * n, d->E.p + (int64_t)pair * n);
+        // e = x* - x_j with its constant part removed: the right-hand sides are
+        // zero-mean (pure-Neumann pressure systems), whose solutions are fixed
+        // up to a constant, and a drifting constant would swamp e in FP32
+        hipLaunchKernelGGL(k_sub, dim3(nblocks(n)), dim3(kThreads), 0, s, x.p, X.p + j * n, n, q.p);
+        hipLaunchKernelGGL(k_bsum<double>, dim3(nb, 1), dim3(kThreads), 0, s, q.p, n, part.p);
+        hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, part.p, (int)nb, sc.p + 6, 1, 0);
+        hipLaunchKernelGGL(k_pair, dim3(nblocks(n)), dim3(kThreads), 0, s, Rr.p + j * n, q.p, sc.p + 6, sc.p + 5, n,
+                           d->R.p + (int64_t)pair * n, d->E.p + (int64_t)pair * n);
         perm_sgs32(ls, d->R.p + (int64_t)pair * n, d->ZS.p + (int64_t)pair * n);
         hipLaunchKernelGGL((k_bdot<float, float>), dim3(nb, 1), dim3(kThreads), 0, s, d->E.p + (int64_t)pair * n,
                            d->R.p + (int64_t)pair * n, n, part.p);

@@ -138,7 +138,7 @@ def test_untrained_network_is_sgs():
 
 def test_harvest_pairs_are_consistent(oracle):
     print("case: harvest n=8", flush=True)
-    _, _, ls = poisson_system(8)
+    _, _, ls = poisson_system(8, na.SolverConfig("cg", "sgs"))  # plain FP64 system, as the pairs are defined
     rp, ci, v, _ = ls.csr()
     n = len(rp) - 1
     data = na.HarvestedPairs(ls, num_rhs=3, pairs_per_rhs=5, seed=7, tol=1e-10)
