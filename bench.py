@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--train-n", type=int, default=64, help="mesh of the learned preconditioner's training pairs")
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--solve-reps", type=int, default=3)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-assembly / mixed-precision block")
+    ap.add_argument("--stretched-n", type=int, default=0,
+                    help="run the stretched-mesh pressure comparison on an n^3 mesh (256: the stretched "
+                         "configuration, about 80 s of solves; 0 skips it)")
     return ap.parse_args()
 
 
@@ -170,12 +174,136 @@ def solver_compare(args, na, torch, mesh, P, devc):
     for name in ("mixed_fgmres_learned", "mixed_fgmres_sgs"):
         res[name]["pressure_rel_diff_vs_fp64"] = float(torch.linalg.norm(sols[name] - ref) / torch.linalg.norm(ref))
     res["speedup_mixed_learned_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_learned"]["solve_s"]
+    res["speedup_mixed_sgs_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_sgs"]["solve_s"]
     res["learned_training"] = {"train_mesh": "%d^3" % args.train_n, "pairs": 2000, "rhs": 200, "cg_tol": 1e-6,
                                "harvest_s": t_harvest,
                                "harvest_cg_iters_mean": float(np.mean(cg_iters)), "train_s": t_train,
                                "epochs": args.train_epochs, "batch": 16, "epoch_loss": hist.tolist(),
                                "layers": 4, "hidden": 32, "weights": "bf16", "accumulate": "fp32"}
+    return res, net
+
+
+def timed(torch, fn, reps=1):
+    """Median wall time of fn() with HIP events on the current stream."""
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    out = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        a, b = ev(), ev()
+        a.record()
+        fn()
+        b.record()
+        torch.cuda.synchronize()
+        out.append(a.elapsed_time(b) * 1e-3)
+    return float(np.median(out))
+
+
+def mixed_block(args, na, torch, mesh, P, devc, stream):
+    """FP32-arithmetic assembly (FP64 Jacobians) and mixed-precision solves on
+    the same 128^3 low-Mach case, with the error of every field against the
+    FP64 path (itself pinned to the FP64 oracle within 1e-12 by the tests)."""
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
+    rho, p, u, gp, mu = t(P["rho"]), t(P["p"]), t(P["u"]), t(P["gpdx"]), t(P["mu"])
+    E = P["conn"].shape[0]
+    res = {"errors_vs_fp64": {}, "solutions_vs_fp64": {}}
+    md = {}
+    sys_ = {}
+    for prec in ("fp64", "mixed"):
+        m = torch.zeros((E, 12), dtype=torch.float64, device=devc)
+        na.compute_mdot(mesh, rho, p, u, gp, m, tau=P["tau"], stream=stream,
+                        precision="fp32" if prec == "mixed" else "fp64")
+        md[prec] = m
+        cont = na.LinearSystem(1, na.SolverConfig("fgmres" if prec == "mixed" else "gmres", "sgs", tolerance=1e-8,
+                                                  max_iterations=5000, precision=prec))
+        mom = na.LinearSystem(3, na.SolverConfig("fgmres" if prec == "mixed" else "gmres", "sgs", tolerance=1e-8,
+                                                 max_iterations=5000, precision=prec))
+        for ls in (cont, mom):
+            ls.buildElemToNodeGraph(mesh)
+            ls.finalizeLinearSystem()
+            ls.set_stream(stream)
+        sys_[prec] = (cont, mom)
+
+    def assemble(prec):
+        cont, mom = sys_[prec]
+        na.compute_mdot(mesh, rho, p, u, gp, md[prec], tau=P["tau"], stream=stream,
+                        precision="fp32" if prec == "mixed" else "fp64")
+        cont.zeroSystem()
+        na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode="gather")
+        cont.loadComplete()
+        mom.zeroSystem()
+        na.assemble_momentum(mom, mesh, u, mu, md[prec], mode="tile")
+        mom.loadComplete()
+
+    for prec in ("fp64", "mixed"):
+        assemble(prec)
+        res["assembly_s_" + prec] = timed(torch, lambda: assemble(prec), reps=5)
+    res["assembly_melem_s_mixed"] = E / res["assembly_s_mixed"] / 1e6
+    res["assembly_melem_s_fp64"] = E / res["assembly_s_fp64"] / 1e6
+    # per-field errors of the FP32 assembly
+    err = res["errors_vs_fp64"]
+    err["mdot_max_rel"] = float((md["mixed"] - md["fp64"]).abs().max() / md["fp64"].abs().max())
+    for name, k in (("continuity", 0), ("momentum", 1)):
+        a64, b64 = sys_["fp64"][k], sys_["mixed"][k]
+        # host copies of both CSR systems (the graph is identical)
+        _, _, vA, rA = a64.csr()
+        _, _, vB, rB = b64.csr()
+        err[name + "_matrix_max_rel"] = float(np.abs(vB - vA).max() / np.abs(vA).max())
+        err[name + "_rhs_max_rel"] = float(np.abs(rB - rA).max() / np.abs(rA).max())
+    # pressure solve (scalar, zero-mean); the steady momentum system without its
+    # time (mass) term is an advection-dominated operator that GMRES(50) + SGS
+    # needs thousands of iterations for, so only its assembly is compared here
+    sol = {}
+    for prec in ("fp64", "mixed"):
+        cont, _ = sys_[prec]
+        cont.removeRhsMean()
+        xp = torch.zeros(cont.sizes()[0], dtype=torch.float64, device=devc)
+        res["pressure_solve_s_" + prec] = timed(torch, lambda: cont.solve(xp))
+        res["pressure_iters_" + prec] = cont.linearSolveIterations()
+        res["pressure_true_rel_resid_" + prec] = cont.linearResidual() / max(cont.nonLinearResidual(), 1e-300)
+        sol[prec] = xp - xp.mean()
+    res["solutions_vs_fp64"]["pressure_rel_l2"] = float(torch.linalg.norm(sol["mixed"] - sol["fp64"]) /
+                                                        torch.linalg.norm(sol["fp64"]))
     return res
+
+
+def stretched_compare(args, na, torch, net, devc):
+    """Stretched-mesh pressure-Poisson: geometric ratio 1.05 from all faces
+    (aspect ratio capped at 100), jitter 0.05h, density lognormal(0, 0.3).
+    Plain FP64 GMRES(50) + SGS against mixed FGMRES(50) + learned preconditioner
+    to the same 1e-8 FP64 relative residual (one timed solve each)."""
+    from nalu_amd.problems import lowmach_problem
+    n = args.stretched_n
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
+    P = lowmach_problem(n, jitter=0.05, rho_sigma=0.3, seed=777, stretch=1.05, max_aspect=100.0)
+    hx = np.diff(P["coords"][:n + 1, 0])  # the boundary line j = k = 0 carries the unjittered axis positions
+    mesh = na.Mesh(P["coords"], P["conn"])
+    out = {"mesh": "%d^3" % n, "elements": int(P["conn"].shape[0]), "stretch_ratio": 1.05,
+           "max_to_min_spacing": float(hx.max() / hx.min())}
+    for name, method, prec, precision in (("fp64_gmres_sgs", "gmres", "sgs", "fp64"),
+                                          ("mixed_fgmres_learned", "fgmres", "learned", "mixed")):
+        ls = na.LinearSystem(1, na.SolverConfig(method, prec, tolerance=1e-8, max_iterations=20000, kspace=50,
+                                                precision=precision))
+        ls.buildElemToNodeGraph(mesh)
+        ls.finalizeLinearSystem()
+        ls.set_stream(torch.cuda.current_stream().cuda_stream)
+        ls.zeroSystem()
+        na.assemble_continuity(ls, mesh, t(P["rho"]), t(P["p"]), t(P["u"]), t(P["gpdx"]), tau=1.0, mode="gather")
+        ls.loadComplete()
+        ls.removeRhsMean()
+        if prec == "learned":
+            ls.set_learned_preconditioner(net)
+        x = torch.zeros(ls.sizes()[0], dtype=torch.float64, device=devc)
+        dt = timed(torch, lambda: ls.solve(x))
+        rel = ls.linearResidual() / max(ls.nonLinearResidual(), 1e-300)
+        out[name] = {"solve_s": dt, "iters": ls.linearSolveIterations(), "true_rel_resid": rel}
+        if prec == "learned":
+            out[name]["fell_back_to_sgs"] = ls.learned_fallback()
+        print("bench: stretched %s %.3f s, %d iters, resid %.2e" % (name, dt, out[name]["iters"], rel),
+              file=sys.stderr, flush=True)
+        del ls, x
+        torch.cuda.empty_cache()
+    out["speedup_mixed_learned_vs_fp64"] = out["fp64_gmres_sgs"]["solve_s"] / out["mixed_fgmres_learned"]["solve_s"]
+    return out
 
 
 def main():
@@ -310,9 +438,21 @@ def main():
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    compare = None
+    compare = net = None
     if not args.no_solver_compare:
-        compare = solver_compare(args, na, torch, mesh, P, devc)
+        compare, net = solver_compare(args, na, torch, mesh, P, devc)
+
+    mixed = None
+    if not args.no_mixed:
+        print("bench: mixed-precision assembly and solves", file=sys.stderr, flush=True)
+        mixed = mixed_block(args, na, torch, mesh, P, devc, stream)
+
+    stretched = None
+    if args.stretched_n > 0 and net is not None:
+        del cont, mom
+        torch.cuda.empty_cache()
+        print("bench: stretched %d^3 pressure comparison" % args.stretched_n, file=sys.stderr, flush=True)
+        stretched = stretched_compare(args, na, torch, net, devc)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -343,6 +483,8 @@ def main():
         "pressure_true_rel_resid": resid,
         "sgs_colors": ncol,
         "pressure_solve_compare": compare,
+        "mixed_precision": mixed,
+        "stretched_pressure_compare": stretched,
         "kernels": {
             k: {"time_s": per_step[k][0], "unit_time_s": {"sgs_apply": t_sgs, "spmv": t_spmv}.get(k, per_step[k][0]),
                 "algorithmic_bytes": byt[k],
