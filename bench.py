@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-assembly / mixed-precision block")
+    ap.add_argument("--learned-apply-only", type=int, default=0, metavar="K",
+                    help="only time K applications of an untrained learned preconditioner on the n^3 "
+                         "continuity system (profiling aid), print one JSON line and exit")
     ap.add_argument("--stretched-n", type=int, default=0,
                     help="run the stretched-mesh pressure comparison on an n^3 mesh (256: the stretched "
                          "configuration, about 80 s of solves; 0 skips it)")
@@ -170,6 +173,7 @@ def solver_compare(args, na, torch, mesh, P, devc):
         sols[name] = x - x.mean()
         print("bench: %s %.4f s, %d iters, resid %.2e" % (name, res[name]["solve_s"], res[name]["iters"], rel),
               file=sys.stderr, flush=True)
+    res["learned_apply"] = learned_apply_timing(args, na, torch, devc, net=net, mesh=mesh, P=P)
     ref = sols["fp64_gmres_sgs"]
     for name in ("mixed_fgmres_learned", "mixed_fgmres_sgs"):
         res[name]["pressure_rel_diff_vs_fp64"] = float(torch.linalg.norm(sols[name] - ref) / torch.linalg.norm(ref))
@@ -196,6 +200,37 @@ def timed(torch, fn, reps=1):
         torch.cuda.synchronize()
         out.append(a.elapsed_time(b) * 1e-3)
     return float(np.median(out))
+
+
+def learned_apply_timing(args, na, torch, devc, net=None, mesh=None, P=None):
+    """Time one application of the learned preconditioner (FP32, solver order)
+    against one FP32 multicolor SGS application on the n^3 continuity system."""
+    from nalu_amd.problems import lowmach_problem
+    if mesh is None:
+        P = lowmach_problem(args.n, seed=4321)
+        mesh = na.Mesh(P["coords"], P["conn"])
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
+    ls = na.LinearSystem(1, na.SolverConfig("fgmres", "learned", precision="mixed"))
+    ls.buildElemToNodeGraph(mesh)
+    ls.finalizeLinearSystem()
+    ls.set_stream(torch.cuda.current_stream().cuda_stream)
+    ls.zeroSystem()
+    na.assemble_continuity(ls, mesh, t(P["rho"]), t(P["p"]), t(P["u"]), t(P["gpdx"]), tau=P["tau"], mode="gather")
+    ls.loadComplete()
+    ls.precond_setup()
+    net = net or na.LearnedPreconditioner(num_layers=4, hidden=32, seed=2)
+    nr = ls.sizes()[0]
+    r = torch.randn(nr, dtype=torch.float32, device=devc)
+    z = torch.zeros_like(r)
+    reps = max(1, args.learned_apply_only or 20)
+    net.apply(ls, r, z)
+    t_net = timed(torch, lambda: [net.apply(ls, r, z) for _ in range(reps)]) / reps
+    ls.spmv_f32(r, z)  # FP32 solver copy exists (mixed system)
+    rd = r.double()
+    zd = torch.zeros_like(rd)
+    t_sgs = timed(torch, lambda: [ls.precond_apply_permuted(rd, zd) for _ in range(reps)]) / reps
+    return {"learned_apply_ms": t_net * 1e3, "fp64_sgs_apply_ms": t_sgs * 1e3, "reps": reps,
+            "note": "learned apply includes its FP32 SGS feature and the permutation in/out of nalu_gnn_apply"}
 
 
 def mixed_block(args, na, torch, mesh, P, devc, stream):
@@ -320,6 +355,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     devc = torch.device("cuda:0")
     n = args.n
+    if args.learned_apply_only > 0:
+        print(json.dumps(learned_apply_timing(args, na, torch, devc)))
+        return
     print("bench: generating %d^3 low-Mach problem" % n, file=sys.stderr, flush=True)
     P = lowmach_problem(n, seed=4321)
     N, E = P["coords"].shape[0], P["conn"].shape[0]
