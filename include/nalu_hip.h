@@ -212,6 +212,14 @@ int nalu_assemble_momentum(nalu_linsys *ls, const nalu_mesh *m, const double *u_
                            const double *mu_dev, const double *mdot_dev, double include_divu,
                            int skew_symmetric, int shifted_grad_op, int mode);
 
+/* LinearSolverConfig "write_matrix_files" (src/LinearSolverConfig.C:102): when
+ * base_name is not NULL, every solve writes <base>-O-<k>.mm.1 (MatrixMarket
+ * coordinate, 1-based Tpetra GIDs), <base>-O-<k>.rhs.1 and, after the solve,
+ * <base>-O-<k>.sln.1 (dense arrays), k counting solves
+ * (TpetraLinearSystem::writeToFile / writeSolutionToFile,
+ * src/TpetraLinearSystem.C:1794-1817,1969-2090).  NULL turns it off. */
+int nalu_linsys_set_write_matrix_files(nalu_linsys *ls, const char *base_name);
+
 /* ------------------------------------------------- learned preconditioner
  * No counterpart in the reference: a message-passing network over the matrix
  * graph (BF16 weights, FP32 accumulation, hidden width 32) used inside FGMRES

@@ -113,6 +113,11 @@ class LinearSystem {
           "resetRows");
   }
   void loadComplete() { check(nalu_linsys_load_complete(ls_), "loadComplete"); }
+  // write_matrix_files (src/LinearSolverConfig.C:102); empty name turns it off
+  void setWriteMatrixFiles(const std::string& baseName) {
+    check(nalu_linsys_set_write_matrix_files(ls_, baseName.empty() ? nullptr : baseName.c_str()),
+          "setWriteMatrixFiles");
+  }
 
   // solve; writes the delta solution (device array of numRows doubles)
   int solve(double* deltaDev) { return check(nalu_linsys_solve(ls_, deltaDev), "solve"); }

@@ -127,6 +127,7 @@ def lib():
         L.nalu_gnn_data_pair.argtypes = [_vp, C.c_int, _vp, _vp]
         L.nalu_gnn_train.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
         L.nalu_linsys_set_learned_precond.argtypes = [_vp, _vp]
+        L.nalu_linsys_set_write_matrix_files.argtypes = [_vp, C.c_char_p]
         L.nalu_linsys_learned_fallback.argtypes = [_vp]
         _L = L
     return _L
@@ -401,6 +402,11 @@ class LinearSystem:
 
     def set_l2_scaling(self, s):
         _check(lib().nalu_linsys_set_l2_scaling(self._h, float(s)), "set_l2_scaling")
+
+    def set_write_matrix_files(self, base_name):
+        """write_matrix_files: every solve writes <base>-O-<k>.mm.1 / .rhs.1 / .sln.1 (None turns it off)."""
+        _check(lib().nalu_linsys_set_write_matrix_files(self._h, None if base_name is None else base_name.encode()),
+               "set_write_matrix_files")
 
     def set_learned_preconditioner(self, net):
         """Attach a LearnedPreconditioner (used when the config's preconditioner is

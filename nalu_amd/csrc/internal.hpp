@@ -172,6 +172,10 @@ struct nalu_linsys {
   // learned preconditioner (optional)
   void* learned = nullptr;
 
+  // matrix-file output around solve() (LinearSolverConfig "write_matrix_files")
+  std::string write_base;  // empty: off
+  int write_counter = 0;
+
   // results
   int iters = 0;
   double linear_residual = 0.0;

@@ -4,8 +4,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <limits>
 #include <string>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -230,10 +232,91 @@ extern "C" int nalu_linsys_load_complete(nalu_linsys* ls) {
   });
 }
 
+namespace nalu {
+namespace {
+
+// Global row index of every row, 1-based, as Tpetra numbers them:
+// GID = numDof (nodeId - 1) + dof + 1 (src/TpetraLinearSystem.C:72).
+std::vector<int64_t> row_gids(nalu_linsys* ls) {
+  const int64_t N = ls->num_nodes, nd = ls->num_dof;
+  std::vector<int64_t> id(N);
+  if (ls->mesh->ids.p) {
+    std::vector<int64_t> ids(N);
+    std::vector<int32_t> rn(N);
+    NALU_HIP_CHECK(hipMemcpy(ids.data(), ls->mesh->ids.p, sizeof(int64_t) * N, hipMemcpyDeviceToHost));
+    NALU_HIP_CHECK(hipMemcpy(rn.data(), ls->row_node.p, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    for (int64_t l = 0; l < N; ++l) id[l] = ids[rn[l]];
+  } else {
+    for (int64_t l = 0; l < N; ++l) id[l] = l + 1;
+  }
+  std::vector<int64_t> g(N * nd);
+  for (int64_t l = 0; l < N; ++l)
+    for (int64_t d = 0; d < nd; ++d) g[l * nd + d] = nd * (id[l] - 1) + d + 1;
+  return g;
+}
+
+std::string file_name(const nalu_linsys* ls, const char* kind) {
+  return ls->write_base + "-O-" + std::to_string(ls->write_counter) + "." + kind + ".1";
+}
+
+void write_dense(const std::string& path, const std::vector<double>& v) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  NALU_REQUIRE(f, NALU_ERR_ARG, "cannot open " + path);
+  std::fprintf(f, "%%%%MatrixMarket matrix array real general\n%lld 1\n", (long long)v.size());
+  for (double x : v) std::fprintf(f, "%.17g\n", x);
+  std::fclose(f);
+}
+
+// TpetraLinearSystem::writeToFile (src/TpetraLinearSystem.C:1969-1994): the owned
+// matrix as a MatrixMarket coordinate file and the rhs as a dense array file.
+void write_matrix_files(nalu_linsys* ls) {
+  materialize_pending_zero(ls);
+  NALU_HIP_CHECK(hipStreamSynchronize(ls->stream));
+  const int64_t n = ls->nrows, nnz = ls->nnz;
+  std::vector<int32_t> rp(n + 1), ci(nnz);
+  std::vector<double> v(nnz), r(n);
+  NALU_HIP_CHECK(hipMemcpy(rp.data(), ls->row_ptr.p, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+  NALU_HIP_CHECK(hipMemcpy(ci.data(), ls->col_idx.p, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+  NALU_HIP_CHECK(hipMemcpy(v.data(), ls->values.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+  NALU_HIP_CHECK(hipMemcpy(r.data(), ls->rhs.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  const std::vector<int64_t> gid = row_gids(ls);
+  const std::string path = file_name(ls, "mm");
+  FILE* f = std::fopen(path.c_str(), "w");
+  NALU_REQUIRE(f, NALU_ERR_ARG, "cannot open " + path);
+  std::fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n%% %s\n%% matrix for: %s\n",
+               ls->write_base.c_str(), ls->write_base.c_str());
+  std::fprintf(f, "%lld %lld %lld\n", (long long)n, (long long)n, (long long)nnz);
+  for (int64_t i = 0; i < n; ++i)
+    for (int32_t q = rp[i]; q < rp[i + 1]; ++q)
+      std::fprintf(f, "%lld %lld %.17g\n", (long long)gid[i], (long long)gid[ci[q]], v[q]);
+  std::fclose(f);
+  write_dense(file_name(ls, "rhs"), r);
+}
+
+// TpetraLinearSystem::writeSolutionToFile (src/TpetraLinearSystem.C:2070-2090)
+void write_solution_file(nalu_linsys* ls) {
+  NALU_HIP_CHECK(hipStreamSynchronize(ls->stream));
+  std::vector<double> x(ls->nrows);
+  NALU_HIP_CHECK(hipMemcpy(x.data(), ls->sln.p, sizeof(double) * ls->nrows, hipMemcpyDeviceToHost));
+  write_dense(file_name(ls, "sln"), x);
+}
+
+}  // namespace
+}  // namespace nalu
+
+extern "C" int nalu_linsys_set_write_matrix_files(nalu_linsys* ls, const char* base_name) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls, NALU_ERR_ARG, "set_write_matrix_files: null handle");
+    ls->write_base = base_name ? std::string(base_name) : std::string();
+    return NALU_OK;
+  });
+}
+
 extern "C" int nalu_linsys_solve(nalu_linsys* ls, double* delta_out) {
   return guarded([&]() -> int {
     NALU_REQUIRE(ls && ls->finalized, NALU_ERR_STATE, "solve: system not finalized");
     materialize_pending_zero(ls);
+    if (!ls->write_base.empty()) write_matrix_files(ls);
     {
       PhaseTimer t(ls, PH_PRECOND);
       precond_setup(ls);
@@ -246,6 +329,10 @@ extern "C" int nalu_linsys_solve(nalu_linsys* ls, double* delta_out) {
       double rel = 0.0;
       status = krylov_solve(ls, ls->rhs.p, ls->sln.p, &it, &rel);
       ls->iters = it;
+    }
+    if (!ls->write_base.empty()) {
+      write_solution_file(ls);
+      ++ls->write_counter;
     }
     if (delta_out)
       NALU_HIP_CHECK(hipMemcpyAsync(delta_out, ls->sln.p, ls->sln.bytes(), hipMemcpyDeviceToDevice, ls->stream));

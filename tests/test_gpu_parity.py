@@ -354,3 +354,39 @@ def test_wrong_numdof_is_rejected():
     mesh, ls = make_system(P["coords"], P["conn"], nd=3)
     with pytest.raises(na.NaluError, match="numDof"):
         na.assemble_scalar_diff(ls, mesh, dev(P["T"]), dev(P["k"]))
+
+
+def test_matrix_market_files(tmp_path, oracle):
+    """write_matrix_files: the MatrixMarket matrix uses 1-based Tpetra GIDs
+    (numDof (id - 1) + dof + 1) and round-trips the CSR values exactly."""
+    import scipy.io
+    print("case: matrix market dump", flush=True)
+    P = heat_problem(3, seed=9)
+    nn = P["coords"].shape[0]
+    ids = np.random.default_rng(4).permutation(nn).astype(np.int64) + 1
+    mesh, ls = make_system(P["coords"], P["conn"], nd=3, ids=ids, cfg=na.SolverConfig("gmres", "sgs"))
+    ls.zeroSystem()
+    rng = np.random.default_rng(2)
+    E = P["conn"].shape[0]
+    lhs = rng.standard_normal((E, 24, 24))
+    lhs += 48 * np.eye(24)[None]
+    rhs = rng.standard_normal((E, 24))
+    ls.sumIntoBatch(dev(lhs), dev(rhs))
+    ls.loadComplete()
+    base = str(tmp_path / "momentum")
+    ls.set_write_matrix_files(base)
+    x = torch.zeros(ls.sizes()[0], dtype=torch.float64, device=DEV)
+    ls.solve(x)
+    rp, ci, v, r = ls.csr()
+    A = scipy.io.mmread(base + "-O-0.mm.1").tocsr()
+    b = scipy.io.mmread(base + "-O-0.rhs.1").ravel()
+    s = scipy.io.mmread(base + "-O-0.sln.1").ravel()
+    # GID of each row: rows are sorted by node id, so row l belongs to the l-th smallest id
+    sid = np.sort(ids)
+    gid = (3 * (np.repeat(sid, 3) - 1) + np.tile(np.arange(3), nn)).astype(np.int64)  # 0-based
+    for row in range(len(r)):
+        cols = ci[rp[row]:rp[row + 1]]
+        np.testing.assert_array_equal(A[gid[row], gid[cols]].toarray().ravel(), v[rp[row]:rp[row + 1]])
+    assert A.nnz == len(v)
+    np.testing.assert_array_equal(b, r)
+    np.testing.assert_array_equal(s, x.cpu().numpy())
