@@ -8,7 +8,7 @@
 // delta_i = dbar / d_i, s = rms(r)):
 //   features  f = (r_i delta_i / s, zs_i dbar / s, delta_i), zs = SGS(r)
 //   encoder   h0 = W_in f + b_in                               3 -> 32
-//   layer l   m_i = sum_{j != i} w_ij h_j,  w_ij = -a_ij / a_ii
+//   layer l   m_i = sum_{j != i} w_ij h_j,  w_ij = -a_ij / a_ii (BF16)
 //             h' = h + relu(W_l [h ; m] + b_l)                 64 -> 32
 //   output    out = w_out . h + c0 f0 + c1 f1,  z = out s / dbar
 // Weights are BF16, hidden states are stored as BF16, every accumulation is
@@ -269,155 +269,278 @@ __global__ __launch_bounds__(kThreads) void k_encode(const float* __restrict__ r
   if (f == 0) F[node] = make_float2(f0, f1);
 }
 
-// Encoder for inference: one thread per node, 64-byte row written as 4 x 16 B.
-__global__ __launch_bounds__(kThreads) void k_encode_rows(const float* __restrict__ r, const float* __restrict__ zs,
-                                                          const float* __restrict__ delta,
-                                                          const double* __restrict__ dsum,
-                                                          const float* __restrict__ scal, int64_t n,
-                                                          const uint16_t* __restrict__ wbf,
-                                                          const float* __restrict__ th, uint16_t* __restrict__ h0,
-                                                          float2* __restrict__ F) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float inv_s = scal[0];
-  const float dbar = (float)(dsum[0] / (double)n);
-  const float dl = delta[i];
-  const float f0 = r[i] * inv_s * dl;
-  const float f1 = zs[i] * inv_s * dbar;
-  F[i] = make_float2(f0, f1);
-  uint32_t pk[16];
-#pragma unroll
-  for (int f = 0; f < kH; f += 2) {
-    float a = th[kH * kIn + f], b = th[kH * kIn + f + 1];
-    a = fmaf(bf2f(wbf[f * kIn + 0]), f0, a);
-    a = fmaf(bf2f(wbf[f * kIn + 1]), f1, a);
-    a = fmaf(bf2f(wbf[f * kIn + 2]), dl, a);
-    b = fmaf(bf2f(wbf[(f + 1) * kIn + 0]), f0, b);
-    b = fmaf(bf2f(wbf[(f + 1) * kIn + 1]), f1, b);
-    b = fmaf(bf2f(wbf[(f + 1) * kIn + 2]), dl, b);
-    pk[f / 2] = (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
-  }
-  uint4* o = reinterpret_cast<uint4*>(h0 + i * kH);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) o[q] = make_uint4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
+// ------------------------------------------------------------------ inference on message tiles
+// The inference layers run on spatial tiles of the matrix graph.  Rows are put
+// in the order of a Morton curve through the node coordinates and cut greedily
+// into tiles of at most kTileRows rows whose footprint -- the rows plus their
+// neighbours outside the tile (the halo) -- fits kTileFoot hidden rows of LDS.
+// Hidden states live in this tile order.  A block loads its tile's footprint
+// once (64 B per row) and gathers the neighbour rows of each node from LDS.
+constexpr int kTileRows = 256;  // rows per tile, one thread each
+constexpr int kTileFoot = 640;  // footprint rows held in LDS (50 KB: three blocks per CU)
+constexpr int kFootIter = kTileFoot * 4 / kTileRows;  // 16-byte footprint pieces per thread
+static_assert(kTileFoot * 4 % kTileRows == 0, "footprint pieces per thread");
+
+struct MsgTile {
+  int32_t start, nint, nfoot, width;  // first row (tile order), rows, footprint rows, edges per row (even)
+  int64_t eoff, foff;                 // entry k of row t at eoff + k nint + t; footprint rows at foff
+};
+
+// A footprint row occupies kFootStride 16-byte slots (64 B of data, 16 B of
+// padding): with an odd stride any 8 (16) consecutive rows put a given quarter
+// on distinct four-byte LDS banks, and the four quarters of a row are one base
+// address plus immediate offsets.
+constexpr int kFootStride = 5;
+static_assert(kTileFoot * kFootStride * 16 < 65536, "footprint byte offsets fit 16 bits");
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> packed BF16 pair, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pk_bf16(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
 }
 
-// Inference layer on the SELL-64 chunks of the solver copy: a wavefront owns
-// one chunk (<= 64 rows of one color), a lane one row.  Column indices and
-// FP32 matrix values are read with coalesced 64-lane loads (w_ij = -a_ij / a_ii
-// formed on the fly), each neighbour's 64-byte hidden row with 4 x 16 B loads;
-// the [h ; m] tile goes through LDS into two 32x32x16 BF16 MFMA M-tiles.  The
-// last layer writes z = (w_out . h + c0 f0 + c1 f1) s / dbar instead of h.
-template <bool LAST, int G>
-__global__ __launch_bounds__(kThreads) void k_layer_sell(
-    const SellChunk* __restrict__ ch, const int32_t* __restrict__ sched, int64_t nch, int64_t nbw,
-    const int32_t* __restrict__ lcol, const float* __restrict__ lval,
-    const int32_t* __restrict__ ucol, const float* __restrict__ uval, const float* __restrict__ invd,
-    const uint16_t* __restrict__ hin, uint16_t* __restrict__ hout, const uint16_t* __restrict__ wl,
-    const float* __restrict__ bl, const uint16_t* __restrict__ wo, const float2* __restrict__ F,
-    const float* __restrict__ scal, float* __restrict__ z) {
-  __shared__ __align__(16) uint16_t As[kWaves][64][kLdA];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, f = lane & 31, half = lane >> 5;
-  const int64_t pb = xcd_block(blockIdx.x, nbw);
-  const int64_t pos = pb * kWaves + wv;
-  const bool live = pb >= 0 && pos < nch;
-  SellChunk k{0, 0, 0, 0, 0, 0};
-  if (live) k = ch[sched[pos]];
-  const bool act = lane < k.nrows;
-  const int row = k.row0 + (act ? lane : 0);
-  // aggregation (lanes of a dead wave or past the chunk's rows carry zeros)
+// a.lo b.lo + a.hi b.hi + c (BF16 pairs, FP32 accumulate)
+__device__ __forceinline__ float dot2_bf16(uint32_t a, uint32_t b, float c) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a), __builtin_bit_cast(bf16x2, b), c, false);
+}
+
+// Edge words: low 16 bits the LDS byte offset of the neighbour's footprint row, high 16 bits the
+// BF16 message weight w_ij = -a_ij / a_ii (rounded from FP32), refreshed from
+// the FP64 split system at every preconditioner set-up.  One block per tile.
+__global__ __launch_bounds__(kTileRows) void k_tile_weights(const MsgTile* __restrict__ tiles,
+                                                            const int32_t* __restrict__ esrc,
+                                                            const int32_t* __restrict__ trow,
+                                                            const double* __restrict__ lval,
+                                                            const double* __restrict__ uval, int64_t nl,
+                                                            const double* __restrict__ dval,
+                                                            uint32_t* __restrict__ edge) {
+  const MsgTile T = tiles[blockIdx.x];
+  const int t = threadIdx.x;
+  if (t >= T.nint) return;
+  const double d = dval[trow[T.start + t]];
+  for (int k = 0; k < T.width; ++k) {
+    const int64_t e = T.eoff + (int64_t)k * T.nint + t;
+    const int32_t src = esrc[e];
+    uint32_t w = 0;
+    if (src >= 0) {
+      const double a = src < nl ? lval[src] : uval[src - nl];
+      w = f2bf((float)(-a / d));
+    }
+    edge[e] = (edge[e] & 0xFFFFu) | (w << 16);
+  }
+}
+
+constexpr int kFixedWidth = 26;  // the 27-point stencil: tiles padded to this width use the unrolled kernel
+
+// acc[2p + h] += w_a h_a,(2p+h) + w_b h_b,(2p+h) for the two edges (ea, eb) (low
+// 16 bits: byte offset of the neighbour's footprint row): the two weights and
+// each pair of BF16 features go into one register each (v_perm_b32) and are
+// summed with v_dot2_f32_bf16
+__device__ __forceinline__ void pair_step(const uint4* __restrict__ foot, uint32_t ea, uint32_t eb, float* acc) {
+  const uint4* pa = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(foot) + (ea & 0xFFFFu));
+  const uint4* pb = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(foot) + (eb & 0xFFFFu));
+  uint4 x[4], y[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    x[q] = pa[q];
+    y[q] = pb[q];
+  }
+  const uint32_t wp = __builtin_amdgcn_perm(eb, ea, 0x07060302u);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t a4[4] = {x[q].x, x[q].y, x[q].z, x[q].w}, b4[4] = {y[q].x, y[q].y, y[q].z, y[q].w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int p = 4 * q + i;  // features 2p (low halves) and 2p + 1 (high halves)
+      acc[2 * p] = dot2_bf16(__builtin_amdgcn_perm(b4[i], a4[i], 0x05040100u), wp, acc[2 * p]);
+      acc[2 * p + 1] = dot2_bf16(__builtin_amdgcn_perm(b4[i], a4[i], 0x07060302u), wp, acc[2 * p + 1]);
+    }
+  }
+}
+
+// One inference layer on one tile per block (kTileRows threads, a thread per row).
+//  1. the footprint's hidden rows are copied to LDS (interior rows, then halo);
+//  2. each thread forms m_i = sum_j w_ij h_j over its edges two at a time: the
+//     two weights and each pair of BF16 features (h_a,t, h_b,t) are packed into
+//     one register and summed with v_dot2_f32_bf16 (FP32 accumulation);
+//  3. each wavefront computes W_l [h ; m] for its 64 rows as the transposed
+//     product (W_l [h ; m]^T, M = 32 features, N = 32 rows, K = 64) on two
+//     32x32x16 BF16 MFMA tiles; the operand halves and the outputs move between
+//     lanes l and l + 32 with one exchange each, so every lane ends up with the
+//     32 outputs of its own row and writes h' = h + relu(. + b) as one 64-byte
+//     row -- or, in the last layer, z = (w_out . h' + c0 f0 + c1 f1) s / dbar
+//     at the row's place in the solver order.
+// The first layer encodes its footprint itself: h0 = W_in f + b_in from the
+// residual r, the SGS feature zs and delta of each footprint row (solver order,
+// gathered through foot_srows), and writes f0, f1 of its own rows to F.
+struct EncodeArgs {
+  const int32_t* foot_srows;  // footprint rows in the solver order
+  const float *r, *zs, *delta;
+  const double* dsum;
+  int64_t n;
+  const uint16_t* win;  // W_in (BF16, [32][3])
+  const float* bin;     // b_in (FP32)
+  float2* F;            // f0, f1 per row of the tile order
+};
+
+template <bool FIRST, bool LAST, int KW>
+__global__ __launch_bounds__(kTileRows) __attribute__((amdgpu_waves_per_eu(3))) void k_layer_tile(
+    const MsgTile* __restrict__ tiles, int64_t ntiles, const int32_t* __restrict__ foot_rows,
+    const uint32_t* __restrict__ edge, const uint16_t* __restrict__ hin, uint16_t* __restrict__ hout,
+    const uint16_t* __restrict__ wl, const float* __restrict__ bl, const uint16_t* __restrict__ wo,
+    const float2* __restrict__ F, const float* __restrict__ scal, const int32_t* __restrict__ trow,
+    float* __restrict__ z, EncodeArgs enc) {
+  __shared__ uint4 foot[kTileFoot * kFootStride];
+  const int64_t tb = xcd_block(blockIdx.x, ntiles);
+  if (tb < 0) return;  // the whole block
+  const MsgTile T = tiles[tb];
+  // Round trip 1: the footprint row indices of this thread's pieces and the
+  // row's edge words (fixed width); round trip 2: the footprint rows.  Loads are
+  // unconditional (clamped to valid addresses) so that each round trip is one
+  // batch of requests with one wait.
+  const int t = threadIdx.x;
+  const bool act = t < T.nint;
+  const int q = t & 3, fr0 = t >> 2;
+  int32_t grow[kFootIter];
+  const int32_t* frows = FIRST ? enc.foot_srows : foot_rows;
+#pragma unroll
+  for (int u = 0; u < kFootIter; ++u) grow[u] = frows[T.foff + min(fr0 + u * (kTileRows / 4), T.nfoot - 1)];
+  const uint32_t* ep = edge + T.eoff + (act ? t : 0);
+  const int64_t stride = T.nint;
+  uint32_t ew[KW > 0 ? KW : 2];
+  if (KW > 0) {
+#pragma unroll
+    for (int k = 0; k < KW; ++k) ew[k] = ep[k * stride];
+  }
+  if (FIRST) {
+    float rv[kFootIter], zv[kFootIter], dv[kFootIter];
+#pragma unroll
+    for (int u = 0; u < kFootIter; ++u) {
+      rv[u] = enc.r[grow[u]];
+      zv[u] = enc.zs[grow[u]];
+      dv[u] = enc.delta[grow[u]];
+    }
+    const float inv_s = scal[0];
+    const float dbar = (float)(enc.dsum[0] / (double)enc.n);
+    float w0[8], w1[8], w2[8], b[8];  // encoder rows of features 8q .. 8q + 7
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int fi = 8 * q + i;
+      w0[i] = bf2f(enc.win[fi * kIn + 0]);
+      w1[i] = bf2f(enc.win[fi * kIn + 1]);
+      w2[i] = bf2f(enc.win[fi * kIn + 2]);
+      b[i] = enc.bin[fi];
+    }
+#pragma unroll
+    for (int u = 0; u < kFootIter; ++u) {
+      const int fr = fr0 + u * (kTileRows / 4);
+      const float f0 = rv[u] * inv_s * dv[u], f1 = zv[u] * inv_s * dbar;
+      float h[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) h[i] = fmaf(w2[i], dv[u], fmaf(w1[i], f1, fmaf(w0[i], f0, b[i])));
+      foot[fr * kFootStride + q] = make_uint4(pk_bf16(h[0], h[1]), pk_bf16(h[2], h[3]), pk_bf16(h[4], h[5]),
+                                              pk_bf16(h[6], h[7]));
+      if (q == 0 && fr < T.nint) enc.F[(int64_t)T.start + fr] = make_float2(f0, f1);
+    }
+  } else {
+    const uint4* hq = reinterpret_cast<const uint4*>(hin);
+    uint4 v[kFootIter];
+#pragma unroll
+    for (int u = 0; u < kFootIter; ++u) v[u] = hq[(int64_t)grow[u] * 4 + q];
+    // rows past nfoot (< kTileFoot) get a copy of the last row: never read,
+    // stored unconditionally so that no load is sunk into a branch
+#pragma unroll
+    for (int u = 0; u < kFootIter; ++u) foot[(fr0 + u * (kTileRows / 4)) * kFootStride + q] = v[u];
+  }
+  __syncthreads();
+  const int lane = t & 63, half = lane >> 5, f = lane & 31;
+  if ((t & ~63) >= T.nint) return;  // a wavefront without rows (no barrier follows)
+  uint32_t own[16], mp[16];
   float acc[kH];
 #pragma unroll
-  for (int t = 0; t < kH; ++t) acc[t] = 0.f;
-  uint4 own[4] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
-  float mscale = 0.f;
-  if (live) {
-    const uint4* hp = reinterpret_cast<const uint4*>(hin + (int64_t)row * kH);
+  for (int p = 0; p < 16; ++p) own[p] = 0u;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) own[q] = hp[q];
-    mscale = -invd[row];
-    // neighbours in groups of G: indices and weights first, then 4 G independent
-    // 16-byte row loads in flight per lane
-    for (int part = 0; part < 2; ++part) {
-      const int w = part ? k.uw : k.lw;
-      const int32_t* cc = (part ? ucol + k.uoff : lcol + k.loff) + lane;
-      const float* vv = (part ? uval + k.uoff : lval + k.loff) + lane;
-      for (int q = 0; q < w; q += G) {
-        int col[G];
-        float wt[G];
+  for (int p = 0; p < kH; ++p) acc[p] = 0.f;
+  if (act) {
 #pragma unroll
-        for (int u = 0; u < G; ++u) {
-          const bool ok = q + u < w;
-          col[u] = ok ? cc[(q + u) * 64] : row;
-          wt[u] = ok ? vv[(q + u) * 64] : 0.f;
-        }
-        uint4 hv[G][4];
+    for (int q = 0; q < 4; ++q) {
+      const uint4 v = foot[t * kFootStride + q];
+      own[4 * q] = v.x, own[4 * q + 1] = v.y, own[4 * q + 2] = v.z, own[4 * q + 3] = v.w;
+    }
+    if (KW > 0) {
 #pragma unroll
-        for (int u = 0; u < G; ++u) {
-          const uint4* p = reinterpret_cast<const uint4*>(hin + (int64_t)col[u] * kH);
-#pragma unroll
-          for (int t = 0; t < 4; ++t) hv[u][t] = p[t];
-        }
-#pragma unroll
-        for (int u = 0; u < G; ++u)
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const uint32_t e[4] = {hv[u][t].x, hv[u][t].y, hv[u][t].z, hv[u][t].w};
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              acc[8 * t + 2 * v] = fmaf(wt[u], __uint_as_float(e[v] << 16), acc[8 * t + 2 * v]);
-              acc[8 * t + 2 * v + 1] = fmaf(wt[u], __uint_as_float(e[v] & 0xFFFF0000u), acc[8 * t + 2 * v + 1]);
-            }
-          }
-      }
+      for (int k = 0; k < KW; k += 2) pair_step(foot, ew[k], ew[k + 1], acc);
+    } else {
+      for (int k = 0; k < T.width; k += 2) pair_step(foot, ep[k * stride], ep[(k + 1) * stride], acc);
     }
   }
-  {
-    uint4* dst = reinterpret_cast<uint4*>(&As[wv][lane][0]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) dst[q] = own[q];
-    uint32_t pk[16];
+  for (int p = 0; p < 16; ++p) mp[p] = pk_bf16(acc[2 * p], acc[2 * p + 1]);
+  // A operand: W_l[o = f][k = 16 ks + 8 half + 0..7].  B operand of tile 0: rows
+  // 0..31 of the wavefront (lane l < 32 gives k = 16 ks + 0..7 of its own row,
+  // lane l + 32 gives k + 8 of row l); tile 1: rows 32..63 likewise.  Each lane
+  // holds both halves (P: k + 0..7, Q: k + 8..15) of its own row; one
+  // v_permlane32_swap per register pair forms both operands.
+  f32x16 d0 = {}, d1 = {};
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
-      pk[t] = (uint32_t)f2bf(mscale * acc[2 * t]) | ((uint32_t)f2bf(mscale * acc[2 * t + 1]) << 16);
+  for (int ks = 0; ks < 4; ++ks) {
+    const u16x8 a = *reinterpret_cast<const u16x8*>(wl + f * 2 * kH + ks * 16 + 8 * half);
+    const uint32_t* src = ks < 2 ? own + 8 * ks : mp + 8 * (ks - 2);
+    u32x4 b0, b1;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) dst[4 + q] = make_uint4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
-  }
-  __syncthreads();
-  u16x8 wfrag[4];
-#pragma unroll
-  for (int ks = 0; ks < 4; ++ks) wfrag[ks] = *reinterpret_cast<const u16x8*>(wl + f * 2 * kH + ks * 16 + 8 * half);
-  const float bias = bl[f];
-#pragma unroll
-  for (int mt = 0; mt < 2; ++mt) {
-    f32x16 d = {};
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const u16x8 a = *reinterpret_cast<const u16x8*>(&As[wv][mt * 32 + f][ks * 16 + 8 * half]);
-      d = mfma_bf16(a, wfrag[ks], d);
+    for (int i = 0; i < 4; ++i) {
+      // lanes 32..63 of P <-> lanes 0..31 of Q: P becomes the tile-0 operand, Q the tile-1 one
+      const auto pq = __builtin_amdgcn_permlane32_swap(src[i], src[4 + i], false, false);
+      b0[i] = pq[0];
+      b1[i] = pq[1];
     }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int nrow = mt * 32 + acc_row(r, half);
-      const float hv = bf2f(As[wv][nrow][f]);
-      As[wv][nrow][f] = f2bf(hv + fmaxf(d[r] + bias, 0.f));
-    }
+    d0 = mfma_bf16(a, __builtin_bit_cast(u16x8, b0), d0);
+    d1 = mfma_bf16(a, __builtin_bit_cast(u16x8, b1), d1);
   }
-  __syncthreads();
-  if (!live || !act) return;
+  // lane l < 32 holds outputs 8g+4h'+s (h' = 0) of row l in d0, lane l + 32 the
+  // h' = 1 outputs of the same row; tile 1 mirrored.  One swap per register pair
+  // gives every lane lo[4g+s] = output 8g+s and hi[4g+s] = output 8g+4+s of its
+  // own row.
+  float lo[16], hi[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const auto x = __builtin_amdgcn_permlane32_swap(__float_as_uint(d0[r]), __float_as_uint(d1[r]), false, false);
+    lo[r] = __uint_as_float(x[0]);
+    hi[r] = __uint_as_float(x[1]);
+  }
+  if (!act) return;
+  float hv[kH];  // h' = h + relu(W [h ; m] + b) of the lane's row
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int o0 = 8 * g + s, o1 = o0 + 4;
+      const float h0v = __uint_as_float(((o0 & 1) ? own[o0 >> 1] : own[o0 >> 1] << 16) & 0xFFFF0000u);
+      const float h1v = __uint_as_float(((o1 & 1) ? own[o1 >> 1] : own[o1 >> 1] << 16) & 0xFFFF0000u);
+      hv[o0] = h0v + fmaxf(lo[4 * g + s] + bl[o0], 0.f);
+      hv[o1] = h1v + fmaxf(hi[4 * g + s] + bl[o1], 0.f);
+    }
+  uint32_t hn[16];
+#pragma unroll
+  for (int p = 0; p < 16; ++p) hn[p] = pk_bf16(hv[2 * p], hv[2 * p + 1]);
+  const int64_t row = (int64_t)T.start + t;
   if (!LAST) {
-    const uint4* src = reinterpret_cast<const uint4*>(&As[wv][lane][0]);
-    uint4* o = reinterpret_cast<uint4*>(hout + (int64_t)row * kH);
+    uint4* o = reinterpret_cast<uint4*>(hout + row * kH);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) o[q] = src[q];
+    for (int p = 0; p < 4; ++p) o[p] = make_uint4(hn[4 * p], hn[4 * p + 1], hn[4 * p + 2], hn[4 * p + 3]);
   } else {
     float out = 0.f;
 #pragma unroll
-    for (int t = 0; t < kH; ++t) out = fmaf(bf2f(wo[t]), bf2f(As[wv][lane][t]), out);
+    for (int o = 0; o < kH; ++o)
+      out = fmaf(bf2f(wo[o]), __uint_as_float((o & 1) ? hn[o >> 1] & 0xFFFF0000u : hn[o >> 1] << 16), out);
     const float2 ff = F[row];
     out = fmaf(bf2f(wo[kH]), ff.x, out);
     out = fmaf(bf2f(wo[kH + 1]), ff.y, out);
-    z[row] = out * scal[1];
+    z[trow[row]] = out * scal[1];
   }
 }
 
@@ -461,7 +584,7 @@ __global__ __launch_bounds__(kThreads) void k_layer(const int32_t* __restrict__ 
           float cw = 0.f;
           if (c0 + q < s1) {
             cj = gcol[c0 + q];
-            cw = gw[c0 + q];
+            cw = bf2f(f2bf(gw[c0 + q]));  // message weights are BF16
           }
           uint2 hv[8];
           float ww[8];
@@ -764,7 +887,7 @@ __global__ __launch_bounds__(kThreads) void k_agg_t(const int32_t* __restrict__ 
     float cw = 0.f;
     if (c0 + q < s1) {
       cj = gcol[c0 + q];
-      cw = gwt[c0 + q];
+      cw = bf2f(f2bf(gwt[c0 + q]));  // the forward's BF16 message weights
     }
     float4 v[8];
     float w[8];
@@ -919,10 +1042,19 @@ namespace nalu {
 struct Learned {
   nalu_gnn* net = nullptr;
   int64_t n = 0;
+  // training graph (CSR of the solver order, FP32 weights) for the current matrix
   bool graph_ready = false, wt_ready = false;
   DevBuf<int32_t> gptr, gcol;
   DevBuf<float> gw, gwt, dF, delta;
   DevBuf<double> dsum;  // [1] sum of the diagonal
+  // inference: message tiles (built once per system) and their weights
+  bool infer_ready = false;
+  int64_t tiles_n = 0, ntiles = 0, tnl = 0;
+  bool tile_fixed = false;  // every tile has kFixedWidth edges per row
+  DevBuf<unsigned char> tiles;  // MsgTile[ntiles]
+  DevBuf<int32_t> tfoot, tfoot_s, tesrc, trow;  // footprint rows (tile order, solver order), edge sources,
+                                                // tile order -> solver row
+  DevBuf<uint32_t> tedge;
   // one-vector workspace
   DevBuf<uint16_t> h0, h1;
   DevBuf<float> zs, r32, z32, scal;
@@ -1000,36 +1132,246 @@ void ensure_one(Learned* st, int64_t n) {
   if (st->part.n < (size_t)kRedBlocks * 64) st->part.alloc((size_t)kRedBlocks * 64);
 }
 
+// ---------------------------------------------------------------- message tiles
+uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+  v &= 0x1FFFFFu;
+  v = (v | v << 32) & 0x1F00000000FFFFull;
+  v = (v | v << 16) & 0x1F0000FF0000FFull;
+  v = (v | v << 8) & 0x100F00F00F00F00Full;
+  v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
+  return v;
+}
+
+// Tile order and tiles of the solver-order graph (host, once per system): rows
+// sorted along a Morton curve of their node coordinates (cells of the mean node
+// spacing), then cut greedily -- a row joins the current tile while the tile
+// keeps <= kTileRows rows and <= kTileFoot footprint rows.  Each tile lists its
+// footprint rows (its own, then the halo in ascending tile-order position) and,
+// per row, its off-diagonal
+// entries as (footprint index, source in [lval ; uval]) in the column-major
+// layout of the layer kernel, padded to an even width with zero-weight self
+// entries.
+void build_tiles(nalu_linsys* ls, Learned* st) {
+  const int64_t n = ls->nrows;
+  NALU_REQUIRE(ls->mesh, NALU_ERR_STATE, "learned preconditioner: the system has no mesh");
+  NALU_REQUIRE(n >= 1 && n < ((int64_t)1 << 31), NALU_ERR_UNSUPPORTED, "learned preconditioner: bad row count");
+  std::vector<int32_t> lrp(n + 1), urp(n + 1), perm(n);
+  NALU_HIP_CHECK(hipMemcpy(lrp.data(), ls->lrp.p, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+  NALU_HIP_CHECK(hipMemcpy(urp.data(), ls->urp.p, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+  NALU_HIP_CHECK(hipMemcpy(perm.data(), ls->perm.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  const int64_t nl = lrp[n], nu = urp[n];
+  NALU_REQUIRE(nl + nu < ((int64_t)1 << 31), NALU_ERR_UNSUPPORTED, "learned preconditioner: nnz >= 2^31");
+  std::vector<int32_t> lci(std::max<int64_t>(nl, 1)), uci(std::max<int64_t>(nu, 1));
+  if (nl) NALU_HIP_CHECK(hipMemcpy(lci.data(), ls->lci.p, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
+  if (nu) NALU_HIP_CHECK(hipMemcpy(uci.data(), ls->uci.p, sizeof(int32_t) * nu, hipMemcpyDeviceToHost));
+  const nalu_mesh* m = ls->mesh;
+  std::vector<double> X(3 * m->num_nodes);
+  NALU_HIP_CHECK(hipMemcpy(X.data(), m->coords.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost));
+  std::vector<int32_t> rn;
+  if (m->ids.p) {
+    rn.resize(n);
+    NALU_HIP_CHECK(hipMemcpy(rn.data(), ls->row_node.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  }
+  auto node_of = [&](int64_t p) -> int64_t {
+    const int64_t old = perm[p];
+    return rn.empty() ? old : rn[old];
+  };
+  // Morton keys on cells of the mean node spacing
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t p = 0; p < n; ++p)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], X[3 * node_of(p) + d]);
+      hi[d] = std::max(hi[d], X[3 * node_of(p) + d]);
+    }
+  const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+  double vol = 1.0;
+  int nd = 0;
+  for (int d = 0; d < 3; ++d)
+    if (hi[d] - lo[d] > 1e-12 * ext) vol *= hi[d] - lo[d], ++nd;
+  const double h = std::max(nd ? std::pow(vol / (double)n, 1.0 / nd) : ext, ext / 2097151.0);
+  std::vector<std::pair<uint64_t, int32_t>> key(n);
+  for (int64_t p = 0; p < n; ++p) {
+    uint64_t k = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double c = std::floor((X[3 * node_of(p) + d] - lo[d]) / h);
+      k |= spread3((uint64_t)std::min(std::max(c, 0.0), 2097151.0)) << d;
+    }
+    key[p] = {k, (int32_t)p};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> order(n), pos(n);
+  for (int64_t q = 0; q < n; ++q) {
+    order[q] = key[q].second;
+    pos[key[q].second] = (int32_t)q;
+  }
+  std::vector<std::pair<uint64_t, int32_t>>().swap(key);
+  // greedy cut
+  std::vector<int32_t> stamp(n, -1);
+  std::vector<int64_t> starts{0};
+  int cur = 0, nint = 0, nfoot = 0;
+  auto fresh = [&](int64_t p, int tile) {
+    int c = stamp[p] != tile;
+    for (int32_t e = lrp[p]; e < lrp[p + 1]; ++e) c += stamp[lci[e]] != tile;
+    for (int32_t e = urp[p]; e < urp[p + 1]; ++e) c += stamp[uci[e]] != tile;
+    return c;
+  };
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t p = order[q];
+    int add = fresh(p, cur);
+    if (nint > 0 && (nint + 1 > kTileRows || nfoot + add > kTileFoot)) {
+      ++cur;
+      starts.push_back(q);
+      nint = nfoot = 0;
+      add = fresh(p, cur);
+    }
+    NALU_REQUIRE(add <= kTileFoot, NALU_ERR_UNSUPPORTED,
+                 "learned preconditioner: a row has more neighbours than a message tile holds");
+    stamp[p] = cur;
+    for (int32_t e = lrp[p]; e < lrp[p + 1]; ++e) stamp[lci[e]] = cur;
+    for (int32_t e = urp[p]; e < urp[p + 1]; ++e) stamp[uci[e]] = cur;
+    nfoot += add;
+    ++nint;
+  }
+  starts.push_back(n);
+  const int64_t nt = (int64_t)starts.size() - 1;
+  // per tile: halo, local indices, edges (all tiles padded to kFixedWidth when
+  // no row has more off-diagonal entries)
+  int maxdeg = 0;
+  for (int64_t p = 0; p < n; ++p) maxdeg = std::max(maxdeg, (lrp[p + 1] - lrp[p]) + (urp[p + 1] - urp[p]));
+  // NALU_GNN_RUNTIME_WIDTH=1 selects the runtime-width kernel for any graph (tests)
+  const char* rw = std::getenv("NALU_GNN_RUNTIME_WIDTH");
+  const bool fixed = maxdeg <= kFixedWidth && !(rw && std::atoi(rw) != 0);
+  std::vector<MsgTile> tiles(nt);
+  std::vector<int32_t> flist, esrc, loc(n, -1), mark(n, -1);
+  std::vector<uint32_t> edge;
+  std::vector<int32_t> hl;
+  std::vector<std::pair<int32_t, int32_t>> ent;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t s0 = starts[t], s1 = starts[t + 1];
+    const int ni = (int)(s1 - s0);
+    hl.clear();
+    int width = 0;
+    for (int64_t q = s0; q < s1; ++q) {
+      const int64_t p = order[q];
+      width = std::max(width, (lrp[p + 1] - lrp[p]) + (urp[p + 1] - urp[p]));
+      auto visit = [&](int32_t j) {
+        const int32_t g = pos[j];
+        if ((g < s0 || g >= s1) && mark[g] != (int32_t)t) {
+          mark[g] = (int32_t)t;
+          hl.push_back(g);
+        }
+      };
+      for (int32_t e = lrp[p]; e < lrp[p + 1]; ++e) visit(lci[e]);
+      for (int32_t e = urp[p]; e < urp[p + 1]; ++e) visit(uci[e]);
+    }
+    std::sort(hl.begin(), hl.end());
+    NALU_REQUIRE(ni + (int64_t)hl.size() <= kTileFoot, NALU_ERR_STATE, "learned preconditioner: tile footprint");
+    for (size_t k = 0; k < hl.size(); ++k) loc[hl[k]] = ni + (int32_t)k;
+    for (int64_t q = s0; q < s1; ++q) loc[q] = (int32_t)(q - s0);
+    width = fixed ? kFixedWidth : width + (width & 1);
+    MsgTile& T = tiles[t];
+    T.start = (int32_t)s0;
+    T.nint = ni;
+    T.nfoot = ni + (int32_t)hl.size();
+    T.width = width;
+    T.eoff = (int64_t)edge.size();
+    T.foff = (int64_t)flist.size();
+    for (int64_t q = s0; q < s1; ++q) flist.push_back((int32_t)q);  // footprint: the rows, then the halo
+    flist.insert(flist.end(), hl.begin(), hl.end());
+    edge.resize(edge.size() + (size_t)ni * width);
+    esrc.resize(edge.size());
+    for (int64_t q = s0; q < s1; ++q) {
+      const int64_t p = order[q];
+      const int tl = (int)(q - s0);
+      ent.clear();
+      for (int32_t e = lrp[p]; e < lrp[p + 1]; ++e) ent.push_back({loc[pos[lci[e]]], e});
+      for (int32_t e = urp[p]; e < urp[p + 1]; ++e) ent.push_back({loc[pos[uci[e]]], (int32_t)(nl + e)});
+      std::sort(ent.begin(), ent.end());
+      for (int k = 0; k < width; ++k) {
+        const int64_t slot = T.eoff + (int64_t)k * ni + tl;
+        const bool real = k < (int)ent.size();
+        edge[slot] = (uint32_t)((real ? ent[k].first : tl) * kFootStride * 16);
+        esrc[slot] = real ? ent[k].second : -1;
+      }
+    }
+  }
+  st->ntiles = nt;
+  st->tiles.alloc(sizeof(MsgTile) * nt);
+  st->tfoot.alloc(flist.size());
+  st->tfoot_s.alloc(flist.size());
+  st->tedge.alloc(edge.size());
+  st->tesrc.alloc(esrc.size());
+  st->trow.alloc(n);
+  NALU_HIP_CHECK(hipMemcpy(st->tiles.p, tiles.data(), sizeof(MsgTile) * nt, hipMemcpyHostToDevice));
+  NALU_HIP_CHECK(hipMemcpy(st->tfoot.p, flist.data(), sizeof(int32_t) * flist.size(), hipMemcpyHostToDevice));
+  for (auto& g : flist) g = order[g];
+  NALU_HIP_CHECK(hipMemcpy(st->tfoot_s.p, flist.data(), sizeof(int32_t) * flist.size(), hipMemcpyHostToDevice));
+  NALU_HIP_CHECK(hipMemcpy(st->tedge.p, edge.data(), sizeof(uint32_t) * edge.size(), hipMemcpyHostToDevice));
+  NALU_HIP_CHECK(hipMemcpy(st->tesrc.p, esrc.data(), sizeof(int32_t) * esrc.size(), hipMemcpyHostToDevice));
+  NALU_HIP_CHECK(hipMemcpy(st->trow.p, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  st->tnl = nl;
+  st->tile_fixed = fixed;
+  st->tiles_n = n;
+}
+
+// diagonal sum and delta_i = dbar / a_ii of the current matrix (solver order)
+void refresh_diag(nalu_linsys* ls, Learned* st) {
+  hipStream_t s = ls->stream;
+  const int64_t n = ls->nrows;
+  if (st->delta.n != (size_t)n) st->delta.alloc(n);
+  if (st->dsum.n < 1) st->dsum.alloc(1);
+  if (st->part.n < (size_t)kRedBlocks * 64) st->part.alloc((size_t)kRedBlocks * 64);
+  const unsigned nb = std::min<unsigned>(kRedBlocks, nblocks(n));
+  hipLaunchKernelGGL(k_bsum<double>, dim3(nb, 1), dim3(kThreads), 0, s, ls->dval.p, n, st->part.p);
+  hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, st->part.p, (int)nb, st->dsum.p, 1, 0);
+  hipLaunchKernelGGL(k_graph_delta, dim3(nblocks(n)), dim3(kThreads), 0, s, ls->dval.p, n, st->dsum.p, st->delta.p);
+}
+
+// Inference state for the current matrix: tiles (built once per system), message
+// weights and diagonal scaling (every set-up).
+void refresh_inference(nalu_linsys* ls, Learned* st) {
+  NALU_REQUIRE(ls->num_dof == 1, NALU_ERR_UNSUPPORTED, "learned preconditioner: scalar systems only (numDof 1)");
+  NALU_REQUIRE(ls->precond_ready && ls->perm_ready, NALU_ERR_STATE,
+               "learned preconditioner: the preconditioner of the system is not set up");
+  if (st->tiles_n != ls->nrows) build_tiles(ls, st);
+  refresh_diag(ls, st);
+  hipLaunchKernelGGL(k_tile_weights, dim3((unsigned)st->ntiles), dim3(kTileRows), 0, ls->stream,
+                     reinterpret_cast<const MsgTile*>(st->tiles.p), st->tesrc.p, st->trow.p, ls->lval.p, ls->uval.p,
+                     st->tnl, ls->dval.p, st->tedge.p);
+  st->infer_ready = true;
+}
+
 // z = net(r) for one vector in the permuted order (FP32)
 void apply_perm(nalu_linsys* ls, Learned* st, const nalu_gnn* g, const float* r, float* z) {
   hipStream_t s = ls->stream;
   const int64_t n = ls->nrows;
   const Layout lay{g->L};
+  NALU_REQUIRE(st->infer_ready, NALU_ERR_STATE, "learned preconditioner: inference state not set up");
   ensure_one(st, n);
   perm_sgs32(ls, r, st->zs.p);
   const unsigned nb = std::min<unsigned>(kRedBlocks, nblocks(n));
   hipLaunchKernelGGL((k_bdot<float, float>), dim3(nb, 1), dim3(kThreads), 0, s, r, r, n, st->part.p);
   hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, st->part.p, (int)nb, st->sumsq.p, 1, 0);
   hipLaunchKernelGGL(k_scale, dim3(1), dim3(64), 0, s, st->sumsq.p, 1, n, st->dsum.p, st->scal.p);
-  hipLaunchKernelGGL(k_encode_rows, dim3(nblocks(n)), dim3(kThreads), 0, s, r, st->zs.p, st->delta.p, st->dsum.p,
-                     st->scal.p, n, g->wbf.p, g->theta.p, st->h0.p, st->F.p);
-  // layers on the SELL-64 chunks of the solver copy (FP32 values, w = -a / d)
-  const SellChunk* ch = reinterpret_cast<const SellChunk*>(ls->sell_chunks.p);
-  const int64_t nch = ls->sell_nchunks;
-  const int64_t nbl = std::max<int64_t>(1, (nch + kWaves - 1) / kWaves);
+  const int32_t* trow = st->trow.p;
+  const MsgTile* tl = reinterpret_cast<const MsgTile*>(st->tiles.p);
+  const EncodeArgs enc{st->tfoot_s.p, r, st->zs.p, st->delta.p, st->dsum.p, n, g->wbf.p + lay.w_in(),
+                       g->theta.p + lay.b_in(), st->F.p};
   uint16_t *hin = st->h0.p, *hout = st->h1.p;
   for (int l = 0; l < g->L; ++l) {
-    const bool last = l == g->L - 1;
-    static const int group = [] {
-      const char* e = std::getenv("NALU_GNN_GATHER");
-      return (e && std::atoi(e) == 2) ? 2 : 4;
-    }();
-    auto kern = group == 2 ? (last ? k_layer_sell<true, 2> : k_layer_sell<false, 2>)
-                           : (last ? k_layer_sell<true, 4> : k_layer_sell<false, 4>);
-    hipLaunchKernelGGL(kern, dim3((unsigned)xcd_grid(nbl)),
-                       dim3(kThreads), 0, s, ch, ls->sell_sched.p, nch, nbl,
-                       ls->sl_col.p, ls->sl_val32.p, ls->su_col.p, ls->su_val32.p, ls->pinvd32.p, hin, hout,
-                       g->wbf.p + lay.w(l), g->theta.p + lay.b(l), g->wbf.p + lay.w_out(), st->F.p, st->scal.p, z);
+    const bool first = l == 0, last = l == g->L - 1;
+    using K = decltype(&k_layer_tile<false, false, 0>);
+    K kern;
+    if (st->tile_fixed)
+      kern = first ? (last ? k_layer_tile<true, true, kFixedWidth> : k_layer_tile<true, false, kFixedWidth>)
+                   : (last ? k_layer_tile<false, true, kFixedWidth> : k_layer_tile<false, false, kFixedWidth>);
+    else
+      kern = first ? (last ? k_layer_tile<true, true, 0> : k_layer_tile<true, false, 0>)
+                   : (last ? k_layer_tile<false, true, 0> : k_layer_tile<false, false, 0>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)xcd_grid(st->ntiles)), dim3(kTileRows), 0, s, tl, st->ntiles,
+                       st->tfoot.p, st->tedge.p, hin, hout, g->wbf.p + lay.w(l), g->theta.p + lay.b(l),
+                       g->wbf.p + lay.w_out(), st->F.p, st->scal.p, trow, z, enc);
     std::swap(hin, hout);
   }
 }
@@ -1166,13 +1508,14 @@ void upload_params(nalu_gnn* g, const float* host, hipStream_t s) {
 // ------------------------------------------------------------ hooks for the solver
 void learned_setup(nalu_linsys* ls) {
   Learned* st = state(ls);
-  if (!st || !st->net) return;
-  build_msg_graph(ls, st, false);
+  if (!st) return;
+  st->graph_ready = st->wt_ready = st->infer_ready = false;  // the matrix changed
+  if (st->net) refresh_inference(ls, st);
 }
 
 bool learned_active(const nalu_linsys* ls) {
   const Learned* st = state(ls);
-  return st && st->net && st->graph_ready && ls->precond_ready && !st->fallback;
+  return st && st->net && st->infer_ready && ls->precond_ready && !st->fallback;
 }
 
 void learned_set_fallback(nalu_linsys* ls, bool on) {
@@ -1290,7 +1633,7 @@ extern "C" int nalu_linsys_set_learned_precond(nalu_linsys* ls, nalu_gnn* g) {
     st->fallback = false;
     if (ls->precond_ready) {
       nalu::ensure_split32(ls);
-      nalu::build_msg_graph(ls, st, false);
+      nalu::refresh_inference(ls, st);
     }
     return NALU_OK;
   });
@@ -1308,7 +1651,7 @@ extern "C" int nalu_gnn_apply(nalu_gnn* g, nalu_linsys* ls, const float* r_dev, 
     NALU_REQUIRE(ls->precond_ready, NALU_ERR_STATE, "gnn_apply: set up the preconditioner first");
     nalu::ensure_split32(ls);
     nalu::Learned* st = nalu::ensure_state(ls);
-    if (!st->graph_ready) nalu::build_msg_graph(ls, st, false);
+    if (!st->infer_ready) nalu::refresh_inference(ls, st);
     const int64_t n = ls->nrows;
     hipStream_t s = ls->stream;
     nalu::ensure_one(st, n);

@@ -17,8 +17,9 @@ of the kernels' BF16 rounding points:
   output:   out_i = w_out . h^L_i + c0 f0 + c1 f1,   z = out s_b / dbar
 
 BF16 rounding points (round-to-nearest-even): every weight matrix/vector
-(W_in, W_l, w_out, c), every stored hidden state h^l, the aggregated message m
-as an MFMA operand, and the backward signal delta = dL/du as an MFMA operand.
+(W_in, W_l, w_out, c), the message weights w_ij (rounded from float32), every
+stored hidden state h^l, the aggregated message m as an MFMA operand, and the
+backward signal delta = dL/du as an MFMA operand.
 Biases and all accumulations are FP32 on the GPU (float64 here).
 
 Loss (per sample; batch mean): the A-norm relative error of the optimally
@@ -75,6 +76,9 @@ class Graph:
         self.n = n
         self.W = sp.csr_matrix((w, (rows[off], cols[off])), shape=(n, n))
         self.WT = self.W.T.tocsr()
+        # message weights as the layers use them with rounding on (BF16)
+        self.Wb = sp.csr_matrix((bf16(w), (rows[off], cols[off])), shape=(n, n))
+        self.WbT = self.Wb.T.tocsr()
         self.d = d.astype(np.float32).astype(np.float64)
         self.dbar = float(np.mean(d))
         self.delta = (self.dbar / d).astype(np.float32).astype(np.float64)
@@ -126,8 +130,9 @@ def forward(th, G, r, zs, num_layers=4, rnd=True):
     F = np.stack([f0, f1, f2], axis=-1)
     h = R(F @ P["w_in"].T + P["b_in"])
     hs, As, masks = [h], [], []
+    Wm = G.Wb if rnd else G.W
     for l in range(num_layers):
-        m = np.stack([G.W @ h[b] for b in range(h.shape[0])])
+        m = np.stack([Wm @ h[b] for b in range(h.shape[0])])
         a = np.concatenate([h, R(m)], axis=-1)
         u = a @ P[f"w{l}"].T + P[f"b{l}"]
         h = R(h + np.maximum(u, 0.0))
@@ -172,6 +177,7 @@ def backward(th, G, cache, out, r, e, num_layers=4, rnd=True):
     grad[o["c"]] = np.sum(gout * F[..., 0])
     grad[o["c"] + 1] = np.sum(gout * F[..., 1])
     Gh = gout[..., None] * P_["w_out"][None, None, :]
+    WmT = G.WbT if rnd else G.WT
     for l in reversed(range(num_layers)):
         dl = Gh * cache["masks"][l]
         dlr = R(dl)
@@ -179,7 +185,7 @@ def backward(th, G, cache, out, r, e, num_layers=4, rnd=True):
         grad[o[f"b{l}"]:o[f"b{l}"] + H] = dl.sum(axis=(0, 1))
         dA = dlr @ P_[f"w{l}"]
         dm = dA[..., H:]
-        Gh = Gh + dA[..., :H] + np.stack([G.WT @ dm[b] for b in range(B)])
+        Gh = Gh + dA[..., :H] + np.stack([WmT @ dm[b] for b in range(B)])
     grad[o["w_in"]:o["b_in"]] = np.einsum("bnf,bnk->fk", Gh, F).ravel()
     grad[o["b_in"]:o["b_in"] + H] = Gh.sum(axis=(0, 1))
     return grad

@@ -94,11 +94,27 @@ def test_forward_loss_grad_match_oracle(oracle):
         assert err <= 2e-2, name
 
 
-def test_inference_path_matches_oracle(oracle):
-    """The solver's inference kernels (SELL-64 layers, fused output) against the
-    oracle's forward pass with the same weights."""
-    print("case: learned inference vs oracle, n=7", flush=True)
-    _, _, ls = poisson_system(7)
+@pytest.mark.parametrize("n,ids,runtime_width", [(7, False, False), (7, False, True), (17, True, False)])
+def test_inference_path_matches_oracle(oracle, monkeypatch, n, ids, runtime_width):
+    """The solver's inference kernels (message tiles in LDS, BF16 dot2 aggregation,
+    transposed MFMA layers, fused output) against the oracle's forward pass with
+    the same weights; explicit shuffled node ids and the runtime-width kernel too."""
+    print("case: learned inference vs oracle, n=%d ids=%s runtime_width=%s" % (n, ids, runtime_width), flush=True)
+    monkeypatch.setenv("NALU_GNN_RUNTIME_WIDTH", "1" if runtime_width else "0")
+    if ids:
+        P = lowmach_problem(n, rho_sigma=0.3, seed=4321)
+        nid = np.random.default_rng(6).permutation(P["coords"].shape[0]).astype(np.int64) + 1
+        mesh = na.Mesh(P["coords"], P["conn"], nid)
+        ls = na.LinearSystem(1, na.SolverConfig("fgmres", "learned", precision="mixed"))
+        ls.buildElemToNodeGraph(mesh)
+        ls.finalizeLinearSystem()
+        ls.zeroSystem()
+        na.assemble_continuity(ls, mesh, dev(P["rho"]), dev(P["p"]), dev(P["u"]), dev(P["gpdx"]), tau=1.0,
+                               mode="gather")
+        ls.loadComplete()
+        ls.precond_setup()
+    else:
+        _, _, ls = poisson_system(n)
     rp, ci, v, _ = ls.csr()
     n = len(rp) - 1
     gr = G.Graph(rp, ci, v)
