@@ -211,6 +211,21 @@ int nalu_assemble_continuity(nalu_linsys *ls, const nalu_mesh *m, const double *
 int nalu_assemble_momentum(nalu_linsys *ls, const nalu_mesh *m, const double *u_dev,
                            const double *mu_dev, const double *mdot_dev, double include_divu,
                            int skew_symmetric, int shifted_grad_op, int mode);
+/* Time-derivative (mass) terms added into the system (BDF: gamma = (g1, g2, g3),
+ * e.g. (1, -1, 0) for backward Euler; the reference's TimeIntegrator gammas).
+ * Fields are nodal device arrays of the three states Nm1, N, Np1.
+ * MomentumMassElemKernel (src/kernel/MomentumMassElemKernel.C:27-170), numDof 3:
+ * lumped selects the shifted (nodal) scv shape functions; balanced_force scales
+ * the Gjp term by rhoNp1 (SolutionOptions balancedForce_). */
+int nalu_assemble_momentum_mass(nalu_linsys *ls, const nalu_mesh *m, const double *rho_nm1,
+                                const double *rho_n, const double *rho_np1, const double *u_nm1,
+                                const double *u_n, const double *u_np1, const double *gjp, double dt,
+                                const double *gamma, int lumped, int balanced_force);
+/* ContinuityMassElemKernel (src/kernel/ContinuityMassElemKernel.C:27-118), numDof 1:
+ * rhs only, scaled by 1 / (dt / g1) like the continuity system. */
+int nalu_assemble_continuity_mass(nalu_linsys *ls, const nalu_mesh *m, const double *rho_nm1,
+                                  const double *rho_n, const double *rho_np1, double dt,
+                                  const double *gamma, int lumped);
 
 /* LinearSolverConfig "write_matrix_files" (src/LinearSolverConfig.C:102): when
  * base_name is not NULL, every solve writes <base>-O-<k>.mm.1 (MatrixMarket

@@ -108,6 +108,9 @@ def lib():
         L.nalu_assemble_continuity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int,
                                                C.c_int, C.c_int]
         L.nalu_assemble_momentum.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_int]
+        L.nalu_assemble_momentum_mass.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _dp,
+                                                  C.c_int, C.c_int]
+        L.nalu_assemble_continuity_mass.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, _dp, C.c_int]
         # learned preconditioner
         _fp = C.POINTER(C.c_float)
         L.nalu_gnn_create.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp)]
@@ -540,3 +543,31 @@ def assemble_continuity(ls, mesh, rho, p, u, gpdx, tau=1.0, shift_mdot=False, sh
 def assemble_momentum(ls, mesh, u, mu, mdot, include_divu=0.0, skew=False, shifted=False, mode="tile"):
     _check(lib().nalu_assemble_momentum(ls.handle, mesh.handle, _ptr(u), _ptr(mu), _ptr(mdot), float(include_divu),
                                         int(skew), int(shifted), MODE[mode]), "assemble_momentum")
+
+
+def _states(fields, name):
+    if len(fields) != 3:
+        raise NaluError("%s: three states (Nm1, N, Np1) expected" % name)
+    return [_ptr(f) for f in fields]
+
+
+def _gammas(gamma):
+    g = np.ascontiguousarray(np.asarray(gamma, dtype=np.float64).ravel())
+    if g.shape != (3,):
+        raise NaluError("gamma: three BDF coefficients expected")
+    return g
+
+
+def assemble_momentum_mass(ls, mesh, rho, u, gjp, dt, gamma=(1.0, -1.0, 0.0), lumped=False, balanced_force=False):
+    """MomentumMassElemKernel: rho = (rhoNm1, rhoN, rhoNp1), u = (uNm1, uN, uNp1), nodal device arrays."""
+    r, v, g = _states(rho, "momentum_mass rho"), _states(u, "momentum_mass u"), _gammas(gamma)
+    _check(lib().nalu_assemble_momentum_mass(ls.handle, mesh.handle, r[0], r[1], r[2], v[0], v[1], v[2], _ptr(gjp),
+                                             float(dt), g.ctypes.data_as(_dp), int(lumped), int(balanced_force)),
+           "assemble_momentum_mass")
+
+
+def assemble_continuity_mass(ls, mesh, rho, dt, gamma=(1.0, -1.0, 0.0), lumped=False):
+    """ContinuityMassElemKernel: rho = (rhoNm1, rhoN, rhoNp1), nodal device arrays."""
+    r, g = _states(rho, "continuity_mass rho"), _gammas(gamma)
+    _check(lib().nalu_assemble_continuity_mass(ls.handle, mesh.handle, r[0], r[1], r[2], float(dt),
+                                               g.ctypes.data_as(_dp), int(lumped)), "assemble_continuity_mass")

@@ -574,6 +574,99 @@ void check_flag(DevBuf<int>& flag, hipStream_t s, const char* msg) {
   NALU_REQUIRE(h == 0, NALU_ERR_ARG, msg);
 }
 
+// ---------------------------------------------------------------- mass (time derivative)
+// MomentumMassElemKernel (src/kernel/MomentumMassElemKernel.C:95-170) and
+// ContinuityMassElemKernel (src/kernel/ContinuityMassElemKernel.C:84-118).  The
+// scv ip of an element belongs to one node (HexSCV::ipNodeMap_), so the node's
+// thread owns every contribution to its rows: one thread per node walks its
+// incident elements (ip = the node's local index) and adds into its rows and
+// rhs in place -- no atomics, no LDS.
+struct MassArgs {
+  const double* rho[3];  // Nm1, N, Np1 (nodal)
+  const double* u[3];    // Nm1, N, Np1 (nodal, 3 components; momentum only)
+  const double* gjp;     // projected pressure gradient (momentum only)
+  double dt, g1, g2, g3;
+  int balanced;          // densFac = rhoNp1 (balanced force) or 1
+};
+
+template <int LA, bool MOM, bool LUMPED>
+__device__ __forceinline__ void mass_row(const MassArgs& A, const int32_t (&nd)[8], const double (&x)[8][3],
+                                         uint64_t pk, double* __restrict__ row, int32_t L, double (&rh)[3]) {
+  const double scV = hex8::scv_volume<LA>(x);
+  double rs[3] = {0.0, 0.0, 0.0}, us[3][3] = {}, g[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const double r = hex8::scv_shape<LA, LUMPED>(c);
+    if (LUMPED && c != LA) continue;
+#pragma unroll
+    for (int st = 0; st < 3; ++st) rs[st] += r * A.rho[st][nd[c]];
+    if (MOM) {
+#pragma unroll
+      for (int st = 0; st < 3; ++st)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) us[st][j] += r * A.u[st][3 * (int64_t)nd[c] + j];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) g[j] += r * A.gjp[3 * (int64_t)nd[c] + j];
+    }
+  }
+  if (MOM) {
+    const double densFac = A.balanced ? rs[2] : 1.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      rh[j] += -(A.g1 * rs[2] * us[2][j] + A.g2 * rs[1] * us[1][j] + A.g3 * rs[0] * us[0][j]) * scV / A.dt -
+               g[j] * scV * densFac;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (LUMPED && c != LA) continue;
+      const double lf = hex8::scv_shape<LA, LUMPED>(c) * A.g1 * rs[2] * scV / A.dt;
+      const int p = (int)((pk >> (8 * c)) & 0xff);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) row[3 * j * L + 3 * p + j] += lf;
+    }
+  } else {
+    const double projTimeScale = A.dt / A.g1;
+    rh[0] += -(A.g1 * rs[2] + A.g2 * rs[1] + A.g3 * rs[0]) * scV / A.dt / projTimeScale;
+  }
+}
+
+template <bool MOM, bool LUMPED>
+__global__ __launch_bounds__(256) void k_mass_gather(MassArgs A, const int32_t* __restrict__ conn,
+                                                     const double* __restrict__ coords,
+                                                     const int32_t* __restrict__ row_node,
+                                                     const int32_t* __restrict__ inc_ptr,
+                                                     const int32_t* __restrict__ inc_elem,
+                                                     const int32_t* __restrict__ nrp,
+                                                     const uint64_t* __restrict__ elem_pos, int64_t nnodes,
+                                                     double* __restrict__ vals, double* __restrict__ rhs) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nnodes) return;
+  const int32_t a = row_node ? row_node[r] : (int32_t)r;
+  const int32_t L = nrp[r + 1] - nrp[r];
+  double* row = vals + (MOM ? 9 : 1) * (int64_t)nrp[r];
+  double rh[3] = {0.0, 0.0, 0.0};
+  for (int32_t q = inc_ptr[a]; q < inc_ptr[a + 1]; ++q) {
+    const int32_t t = inc_elem[q];
+    const int64_t e = t >> 3;
+    int32_t nd[8];
+    load_nodes(conn, e, nd);
+    double x[8][3];
+    load_vec3(coords, nd, x);
+    const uint64_t pk = elem_pos[t];
+    switch (t & 7) {
+#define NALU_MS(Lx) \
+  case Lx: mass_row<Lx, MOM, LUMPED>(A, nd, x, pk, row, L, rh); break;
+      NALU_MS(0) NALU_MS(1) NALU_MS(2) NALU_MS(3) NALU_MS(4) NALU_MS(5) NALU_MS(6) NALU_MS(7)
+#undef NALU_MS
+    }
+  }
+  if (MOM) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) rhs[3 * r + j] += rh[j];
+  } else {
+    rhs[r] += rh[0];
+  }
+}
+
 void check_assembly_args(nalu_linsys* ls, const nalu_mesh* m, int nd) {
   NALU_REQUIRE(ls && m, NALU_ERR_ARG, "assembly: null handle");
   NALU_REQUIRE(ls->finalized, NALU_ERR_STATE, "assembly: finalizeLinearSystem not called");
@@ -768,6 +861,61 @@ extern "C" int nalu_linsys_reset_rows(nalu_linsys* ls, int64_t nn, const int32_t
                        (const double*)nullptr, ls->values.p, ls->rhs.p, 1, diag_value, rhs_residual, bad.p);
     NALU_HIP_CHECK(hipGetLastError());
     check_flag(bad, ls->stream, "resetRows: node index outside the mesh");
+    return NALU_OK;
+  });
+}
+
+namespace {
+void launch_mass(nalu_linsys* ls, const nalu_mesh* m, const MassArgs& A, bool mom, bool lumped) {
+  build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
+  materialize_pending_zero(ls);
+  const dim3 g(blocks_for(ls->num_nodes, 256)), b(256);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, g, b, 0, ls->stream, A, m->conn.p, m->coords.p, (const int32_t*)ls->row_node.p,
+                       m->inc_ptr.p, m->inc_elem.p, ls->nrp.p, ls->elem_pos.p, ls->num_nodes, ls->values.p,
+                       ls->rhs.p);
+  };
+  if (mom) lumped ? go(k_mass_gather<true, true>) : go(k_mass_gather<true, false>);
+  else lumped ? go(k_mass_gather<false, true>) : go(k_mass_gather<false, false>);
+  NALU_HIP_CHECK(hipGetLastError());
+}
+
+void check_mass_args(double dt, const double* gamma) {
+  NALU_REQUIRE(gamma, NALU_ERR_ARG, "mass: null gamma");
+  NALU_REQUIRE(std::isfinite(dt) && dt > 0.0, NALU_ERR_ARG, "mass: dt must be positive");
+  NALU_REQUIRE(std::isfinite(gamma[0]) && gamma[0] != 0.0 && std::isfinite(gamma[1]) && std::isfinite(gamma[2]),
+               NALU_ERR_ARG, "mass: gamma1 must be nonzero and all gammas finite");
+}
+}  // namespace
+
+extern "C" int nalu_assemble_momentum_mass(nalu_linsys* ls, const nalu_mesh* m, const double* rho_nm1,
+                                           const double* rho_n, const double* rho_np1, const double* u_nm1,
+                                           const double* u_n, const double* u_np1, const double* gjp, double dt,
+                                           const double* gamma, int lumped, int balanced_force) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 3);
+    check_mass_args(dt, gamma);
+    NALU_REQUIRE(rho_nm1 && rho_n && rho_np1 && u_nm1 && u_n && u_np1 && gjp, NALU_ERR_ARG,
+                 "momentum_mass: null field");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    const MassArgs A{{rho_nm1, rho_n, rho_np1}, {u_nm1, u_n, u_np1}, gjp, dt, gamma[0], gamma[1], gamma[2],
+                     balanced_force ? 1 : 0};
+    launch_mass(ls, m, A, true, lumped != 0);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_assemble_continuity_mass(nalu_linsys* ls, const nalu_mesh* m, const double* rho_nm1,
+                                             const double* rho_n, const double* rho_np1, double dt,
+                                             const double* gamma, int lumped) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 1);
+    check_mass_args(dt, gamma);
+    NALU_REQUIRE(rho_nm1 && rho_n && rho_np1, NALU_ERR_ARG, "continuity_mass: null field");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    const MassArgs A{{rho_nm1, rho_n, rho_np1}, {nullptr, nullptr, nullptr}, nullptr, dt, gamma[0], gamma[1],
+                     gamma[2], 0};
+    launch_mass(ls, m, A, false, lumped != 0);
     return NALU_OK;
   });
 }

@@ -460,5 +460,74 @@ __device__ __forceinline__ void row_momentum(const R (&x)[8][3], const Gather& g
   }
 }
 
+
+// ---------------------------------------------------------------- sub-control volumes
+// Sub-hexahedron of node LA as subdivision point numbers (HexSCV::determinant,
+// src/master_element/Hex8CVFEM.C:149-178) and its volume by Grandy's formula
+// (hex_volume_grandy, include/master_element/Hex8GeometryFunctions.h:78-163):
+// six face centroids, 24 triangles, vol = 1/18 sum (p + q + r) . ((q - p) x (r - p)).
+constexpr int kScvSub[8][8] = {{0, 8, 12, 11, 19, 20, 26, 25},  {8, 1, 9, 12, 20, 18, 24, 26},
+                               {12, 9, 2, 10, 26, 24, 22, 23},  {11, 12, 10, 3, 25, 26, 23, 21},
+                               {19, 20, 26, 25, 4, 13, 17, 16}, {20, 18, 24, 26, 13, 5, 14, 17},
+                               {26, 24, 22, 23, 17, 14, 6, 15}, {25, 26, 23, 21, 16, 17, 15, 7}};
+constexpr int kGrandyFace[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4}, {2, 3, 7, 6}, {1, 2, 6, 5}, {0, 4, 3, 7}};
+constexpr int kGrandyTri[24][3] = {{0, 8, 1},   {8, 2, 1},   {3, 2, 8},   {3, 8, 0},   {6, 9, 5},   {7, 9, 6},
+                                   {4, 9, 7},   {4, 5, 9},   {10, 0, 1},  {5, 10, 1},  {4, 10, 5},  {4, 0, 10},
+                                   {7, 6, 11},  {6, 2, 11},  {2, 3, 11},  {3, 7, 11},  {6, 12, 2},  {5, 12, 6},
+                                   {5, 1, 12},  {1, 2, 12},  {0, 4, 13},  {4, 7, 13},  {7, 3, 13},  {3, 0, 13}};
+
+template <int LA>
+__device__ __forceinline__ double scv_volume(const double (&x)[8][3]) {
+  double c[14][3];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const uint32_t m = subdiv_mask(kScvSub[LA][n]);
+    const int cnt = popc8(m);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double s = 0.0;
+      bool first = true;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if ((m >> i) & 1u) {
+          s = first ? x[i][d] : s + x[i][d];
+          first = false;
+        }
+      c[n][d] = cnt == 1 ? s : s * (1.0 / cnt);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      c[8 + k][d] = 0.25 * (((c[kGrandyFace[k][0]][d] + c[kGrandyFace[k][1]][d]) + c[kGrandyFace[k][2]][d]) +
+                            c[kGrandyFace[k][3]][d]);
+  double vol = 0.0;
+#pragma unroll
+  for (int k = 0; k < 24; ++k) {
+    const int p = kGrandyTri[k][0], q = kGrandyTri[k][1], r = kGrandyTri[k][2];
+    const double m0 = (c[p][0] + c[q][0]) + c[r][0];
+    const double m1 = (c[p][1] + c[q][1]) + c[r][1];
+    const double m2 = (c[p][2] + c[q][2]) + c[r][2];
+    const double d0 = (c[q][1] - c[p][1]) * (c[r][2] - c[p][2]) - (c[r][1] - c[p][1]) * (c[q][2] - c[p][2]);
+    const double d1 = (c[r][0] - c[p][0]) * (c[q][2] - c[p][2]) - (c[q][0] - c[p][0]) * (c[r][2] - c[p][2]);
+    const double d2 = (c[q][0] - c[p][0]) * (c[r][1] - c[p][1]) - (c[r][0] - c[p][0]) * (c[q][1] - c[p][1]);
+    vol += (m0 * d0 + m1 * d1) + m2 * d2;
+  }
+  return vol / 18.0;
+}
+
+// Shape function of column node c at the scv ip of node LA: the scv centroid is
+// at +-1/4 (HexSCV::intgLoc_, Hex8CVFEM.C:88-96), so N_c = prod (1/2 + sign/4):
+// (3/4)^k (1/4)^(3-k), k = axes on which c and LA agree (27/64, 9/64, 3/64, 1/64);
+// the lumped form (shifted locations, :99-107, at the nodes) is delta_{c,LA}.
+template <int LA, bool LUMPED>
+__host__ __device__ constexpr double scv_shape(int c) {
+  if (LUMPED) return c == LA ? 1.0 : 0.0;
+  int k = 0;
+  for (int d = 0; d < 3; ++d) k += node_sign(c, d) == node_sign(LA, d) ? 1 : 0;
+  return k == 3 ? 27.0 / 64 : k == 2 ? 9.0 / 64 : k == 1 ? 3.0 / 64 : 1.0 / 64;
+}
+
 }  // namespace hex8
 }  // namespace nalu

@@ -687,6 +687,122 @@ void or_assemble_momentum(int64_t nelem, const int32_t *conn, const double *coor
 }
 
 /* ------------------------------------------------------------------------ */
+/* Time-derivative (mass) kernels                                           */
+/* ------------------------------------------------------------------------ */
+
+/* MomentumMassElemKernel<Hex8>::execute (src/kernel/MomentumMassElemKernel.C:95-170):
+ * per scv ip (nearest node nn = ip, HexSCV::ipNodeMap_), fields interpolated
+ * with the scv shape functions (the node values themselves when lumped, the
+ * shifted scv locations being the nodes):
+ *   rhs(nn, j) -= (g1 rhoNp1 uNp1_j + g2 rhoN uN_j + g3 rhoNm1 uNm1_j) scV / dt
+ *                 + Gjp_j scV densFac,  densFac = balanced ? rhoNp1 : 1,
+ *   lhs((nn, j), (c, j)) += N_c g1 rhoNp1 scV / dt.
+ * rho3 = [Nm1 | N | Np1] x 8, u3 = [Nm1 | N | Np1] x 8 x 3, gjp 8 x 3,
+ * gamma = (g1, g2, g3); lhs 24 x 24 and rhs 24 are accumulated into. */
+void or_momentum_mass_elem(const double *coords, const double *rho3, const double *u3,
+                           const double *gjp, double dt, const double *gamma, int lumped,
+                           int balanced, double *lhs, double *rhs) {
+  double shape[64], vol[8];
+  or_hex8_shape_fcn(8, or_hexscv_intgloc(lumped), shape);
+  or_hexscv_volume(coords, vol);
+  for (int ip = 0; ip < 8; ++ip) {
+    const int nn = ip;
+    double rNm1 = 0.0, rN = 0.0, rNp1 = 0.0, uNm1[3] = {0, 0, 0}, uN[3] = {0, 0, 0},
+           uNp1[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+    for (int c = 0; c < 8; ++c) {
+      const double r = shape[8 * ip + c];
+      rNm1 += r * rho3[c];
+      rN += r * rho3[8 + c];
+      rNp1 += r * rho3[16 + c];
+      for (int j = 0; j < 3; ++j) {
+        uNm1[j] += r * u3[3 * c + j];
+        uN[j] += r * u3[24 + 3 * c + j];
+        uNp1[j] += r * u3[48 + 3 * c + j];
+        g[j] += r * gjp[3 * c + j];
+      }
+    }
+    const double scV = vol[ip];
+    const double densFac = balanced ? rNp1 : 1.0;
+    for (int j = 0; j < 3; ++j)
+      rhs[3 * nn + j] += -(gamma[0] * rNp1 * uNp1[j] + gamma[1] * rN * uN[j] + gamma[2] * rNm1 * uNm1[j]) *
+                             scV / dt -
+                         g[j] * scV * densFac;
+    for (int c = 0; c < 8; ++c) {
+      const double lf = shape[8 * ip + c] * gamma[0] * rNp1 * scV / dt;
+      for (int j = 0; j < 3; ++j) lhs[(3 * nn + j) * 24 + 3 * c + j] += lf;
+    }
+  }
+}
+
+/* ContinuityMassElemKernel<Hex8>::execute (src/kernel/ContinuityMassElemKernel.C:84-118):
+ * rhs(nn) -= (g1 rhoNp1 + g2 rhoN + g3 rhoNm1) scV / dt / (dt / g1); no lhs. */
+void or_continuity_mass_elem(const double *coords, const double *rho3, double dt,
+                             const double *gamma, int lumped, double *rhs) {
+  double shape[64], vol[8];
+  or_hex8_shape_fcn(8, or_hexscv_intgloc(lumped), shape);
+  or_hexscv_volume(coords, vol);
+  const double projTimeScale = dt / gamma[0];
+  for (int ip = 0; ip < 8; ++ip) {
+    double rNm1 = 0.0, rN = 0.0, rNp1 = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      const double r = shape[8 * ip + c];
+      rNm1 += r * rho3[c];
+      rN += r * rho3[8 + c];
+      rNp1 += r * rho3[16 + c];
+    }
+    rhs[ip] += -(gamma[0] * rNp1 + gamma[1] * rN + gamma[2] * rNm1) * vol[ip] / dt / projTimeScale;
+  }
+}
+
+/* Whole-mesh assembly of the two mass kernels into a graph's CSR (the field
+ * states are nodal arrays: rho [N] each, u [N][3] each, gjp [N][3]). */
+void or_assemble_momentum_mass(int64_t nelem, const int32_t *conn, const double *coords,
+                               const double *rhoNm1, const double *rhoN, const double *rhoNp1,
+                               const double *uNm1, const double *uN, const double *uNp1,
+                               const double *gjp, double dt, const double *gamma, int lumped,
+                               int balanced, const int64_t *row_ptr, const int32_t *col_idx,
+                               const int32_t *node_lid, double *values, double *rhs) {
+  double *lhs = (double *)malloc(sizeof(double) * 576);
+  for (int64_t e = 0; e < nelem; ++e) {
+    const int32_t *nd = conn + 8 * e;
+    double x[24], r3[24], u3[72], ge[24], r[24];
+    gather(nd, coords, 3, x);
+    gather(nd, rhoNm1, 1, r3);
+    gather(nd, rhoN, 1, r3 + 8);
+    gather(nd, rhoNp1, 1, r3 + 16);
+    gather(nd, uNm1, 3, u3);
+    gather(nd, uN, 3, u3 + 24);
+    gather(nd, uNp1, 3, u3 + 48);
+    gather(nd, gjp, 3, ge);
+    memset(lhs, 0, sizeof(double) * 576);
+    memset(r, 0, sizeof(r));
+    or_momentum_mass_elem(x, r3, u3, ge, dt, gamma, lumped, balanced, lhs, r);
+    or_sum_into(3, row_ptr, col_idx, values, rhs, node_lid, 8, nd, lhs, r);
+  }
+  free(lhs);
+}
+
+void or_assemble_continuity_mass(int64_t nelem, const int32_t *conn, const double *coords,
+                                 const double *rhoNm1, const double *rhoN, const double *rhoNp1,
+                                 double dt, const double *gamma, int lumped, const int64_t *row_ptr,
+                                 const int32_t *col_idx, const int32_t *node_lid, double *values,
+                                 double *rhs) {
+  double lhs[64];
+  memset(lhs, 0, sizeof(lhs));
+  for (int64_t e = 0; e < nelem; ++e) {
+    const int32_t *nd = conn + 8 * e;
+    double x[24], r3[24], r[8];
+    gather(nd, coords, 3, x);
+    gather(nd, rhoNm1, 1, r3);
+    gather(nd, rhoN, 1, r3 + 8);
+    gather(nd, rhoNp1, 1, r3 + 16);
+    memset(r, 0, sizeof(r));
+    or_continuity_mass_elem(x, r3, dt, gamma, lumped, r);
+    or_sum_into(1, row_ptr, col_idx, values, rhs, node_lid, 8, nd, lhs, r);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Krylov + relaxation: textbook restatements (Saad, Iterative Methods for   */
 /* Sparse Linear Systems, 2nd ed.: GMRES Alg. 6.9 with Givens rotations,     */
 /* right preconditioning Alg. 9.5; PCG Alg. 9.1; SGS M=(D+L)D^-1(D+U) 4.1).  */

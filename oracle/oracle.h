@@ -94,6 +94,24 @@ void or_assemble_momentum(int64_t nelem, const int32_t *conn, const double *coor
                           const double *u, const double *mu, const double *mdot, double includeDivU,
                           int skewSymmetric, int shiftedGradOp, const int64_t *row_ptr,
                           const int32_t *col_idx, const int32_t *node_lid, double *values, double *rhs);
+/* mass kernels (MomentumMassElemKernel.C:95-170, ContinuityMassElemKernel.C:84-118);
+ * pinned by the reference's unit-test golds (consistent and lumped) */
+void or_momentum_mass_elem(const double *coords, const double *rho3, const double *u3,
+                           const double *gjp, double dt, const double *gamma, int lumped,
+                           int balanced, double *lhs, double *rhs);
+void or_continuity_mass_elem(const double *coords, const double *rho3, double dt,
+                             const double *gamma, int lumped, double *rhs);
+void or_assemble_momentum_mass(int64_t nelem, const int32_t *conn, const double *coords,
+                               const double *rhoNm1, const double *rhoN, const double *rhoNp1,
+                               const double *uNm1, const double *uN, const double *uNp1,
+                               const double *gjp, double dt, const double *gamma, int lumped,
+                               int balanced, const int64_t *row_ptr, const int32_t *col_idx,
+                               const int32_t *node_lid, double *values, double *rhs);
+void or_assemble_continuity_mass(int64_t nelem, const int32_t *conn, const double *coords,
+                                 const double *rhoNm1, const double *rhoN, const double *rhoNp1,
+                                 double dt, const double *gamma, int lumped, const int64_t *row_ptr,
+                                 const int32_t *col_idx, const int32_t *node_lid, double *values,
+                                 double *rhs);
 
 /* ---- Krylov + relaxation (Belos / Ifpack2 semantics restated from the textbook) ---- */
 void or_spmv(int64_t nrows, const int64_t *row_ptr, const int32_t *col_idx, const double *vals,

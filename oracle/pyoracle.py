@@ -50,6 +50,11 @@ def lib():
             "or_compute_mdot": [I64, _i32, _d, _d, _d, _d, _d, D, I, I, _d],
             "or_assemble_continuity": [I64, _i32, _d, _d, _d, _d, _d, D, I, I, I, _i64, _i32, _i32, _d, _d],
             "or_assemble_momentum": [I64, _i32, _d, _d, _d, _d, D, I, I, _i64, _i32, _i32, _d, _d],
+            "or_momentum_mass_elem": [_d, _d, _d, _d, D, _d, I, I, _d, _d],
+            "or_continuity_mass_elem": [_d, _d, D, _d, I, _d],
+            "or_assemble_momentum_mass": [I64, _i32, _d, _d, _d, _d, _d, _d, _d, _d, D, _d, I, I, _i64, _i32,
+                                          _i32, _d, _d],
+            "or_assemble_continuity_mass": [I64, _i32, _d, _d, _d, _d, D, _d, I, _i64, _i32, _i32, _d, _d],
             "or_spmv": [I64, _i64, _i32, _d, _d, _d],
             "or_sgs_apply": [I64, _i64, _i32, _d, _d, _d],
             "or_mcsgs_apply": [I64, _i64, _i32, _d, I, _i64, _i32, _d, _d],
@@ -280,6 +285,46 @@ def assemble_momentum(g, conn, coords, u, mu, mdot, include_divu=0.0, skew=False
                                _p(f64(mu)), _p(f64(mdot)), C.c_double(include_divu), int(skew),
                                int(shifted), _p(g.row_ptr, np.int64), _p(g.col_idx, np.int32),
                                _p(g.node_lid, np.int32), _p(values), _p(rhs))
+    return values, rhs
+
+
+def momentum_mass_elem(coords, rho3, u3, gjp, dt, gamma, lumped=False, balanced=False):
+    """One element: rho3 [3][8] (Nm1, N, Np1), u3 [3][8][3], gjp [8][3] -> lhs [24][24], rhs [24]."""
+    lhs = np.zeros((24, 24))
+    rhs = np.zeros(24)
+    lib().or_momentum_mass_elem(_p(f64(coords)), _p(f64(rho3)), _p(f64(u3)), _p(f64(gjp)), C.c_double(dt),
+                                _p(f64(gamma)), int(lumped), int(balanced), _p(lhs), _p(rhs))
+    return lhs, rhs
+
+
+def continuity_mass_elem(coords, rho3, dt, gamma, lumped=False):
+    rhs = np.zeros(8)
+    lib().or_continuity_mass_elem(_p(f64(coords)), _p(f64(rho3)), C.c_double(dt), _p(f64(gamma)), int(lumped),
+                                  _p(rhs))
+    return rhs
+
+
+def assemble_momentum_mass(g, conn, coords, rho, u, gjp, dt, gamma, lumped=False, balanced=False):
+    """rho = (rhoNm1, rhoN, rhoNp1) nodal arrays, u = (uNm1, uN, uNp1) [N][3] arrays."""
+    conn = i32(conn)
+    values = np.zeros(g.nnz)
+    rhs = np.zeros(g.nrows)
+    lib().or_assemble_momentum_mass(conn.shape[0], _p(conn, np.int32), _p(f64(coords)), _p(f64(rho[0])),
+                                    _p(f64(rho[1])), _p(f64(rho[2])), _p(f64(u[0])), _p(f64(u[1])),
+                                    _p(f64(u[2])), _p(f64(gjp)), C.c_double(dt), _p(f64(gamma)), int(lumped),
+                                    int(balanced), _p(g.row_ptr, np.int64), _p(g.col_idx, np.int32),
+                                    _p(g.node_lid, np.int32), _p(values), _p(rhs))
+    return values, rhs
+
+
+def assemble_continuity_mass(g, conn, coords, rho, dt, gamma, lumped=False):
+    conn = i32(conn)
+    values = np.zeros(g.nnz)
+    rhs = np.zeros(g.nrows)
+    lib().or_assemble_continuity_mass(conn.shape[0], _p(conn, np.int32), _p(f64(coords)), _p(f64(rho[0])),
+                                      _p(f64(rho[1])), _p(f64(rho[2])), C.c_double(dt), _p(f64(gamma)),
+                                      int(lumped), _p(g.row_ptr, np.int64), _p(g.col_idx, np.int32),
+                                      _p(g.node_lid, np.int32), _p(values), _p(rhs))
     return values, rhs
 
 

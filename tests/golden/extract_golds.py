@@ -10,6 +10,8 @@ Sources (paths relative to the reference root):
   unit_tests/kernels/UnitTestScalarDiffElem.C:14-27       heat-conduction 8x8 lhs
   unit_tests/kernels/UnitTestContinuityAdvElem.C:14-57    continuity rhs/lhs (default, reduced sens.)
   unit_tests/kernels/UnitTestMomentumAdvDiffElem.C:14-270 momentum rhs[24], lhs[24][24]
+  unit_tests/kernels/UnitTestMomentumMassElem.C:14-150     momentum time derivative (consistent, lumped)
+  unit_tests/kernels/UnitTestContinuityMassElem.C:14-96    density time derivative (rhs -12.5 everywhere)
   unit_tests/UnitTestKokkosMEGold.h:13-79                 hex8 SCV grad_op golds (perturbed hex)
   unit_tests/UnitTestUtils.C:360-371                      perturbed hex8 coordinates
   unit_tests/UnitTestTpetra.C:86-114,146-158              Tpetra 1x1x2 element matrix, assembled 12x12, row lengths
@@ -107,6 +109,31 @@ def main():
         "mu": 0.1, "rho": 1.0, "includeDivU": 0.0,
         "rhs": _array(t, "double rhs[24]", 0, (24,)),
         "lhs": _array(t, "double lhs[24][24]", 0, (24, 24)),
+        "tol": 1e-15,
+    }
+
+    t = _read("unit_tests/kernels/UnitTestMomentumMassElem.C")
+    p0 = t.index("namespace momentum_time_derivative {")
+    p1 = t.index("namespace momentum_time_derivative_lumped")
+    golds["momentum_mass"] = {
+        "source": "unit_tests/kernels/UnitTestMomentumMassElem.C:16-136,140-181",
+        "dt": 0.1, "gamma": [1.0, -1.0, 0.0],
+        "fields": "rho Np1 = 1, rho N = 0 (unset state), u Np1 = trig velocity, u N = 0, Gjp = trig dpdx",
+        "rhs": _array(t, "double rhs[24]", p0, (24,)),
+        "lhs": _array(t, "double lhs[24][24]", p0, (24, 24)),
+        "tol": 1e-15,
+    }
+    golds["momentum_mass_lumped"] = {
+        "source": "unit_tests/kernels/UnitTestMomentumMassElem.C:138-148,186-225",
+        "dt": 0.1, "gamma": [1.0, -1.0, 0.0],
+        "rhs": _array(t, "double rhs[24]", p1, (24,)),
+        "lhs_diag": 1.25,
+        "tol": 1e-15,
+    }
+    golds["continuity_mass"] = {
+        "source": "unit_tests/kernels/UnitTestContinuityMassElem.C:14-54,56-96",
+        "dt": 0.1, "gamma": [1.0, -1.0, 0.0],
+        "rhs_all": -12.5, "lhs_all": 0.0, "lumped_too": True,
         "tol": 1e-15,
     }
 

@@ -1,0 +1,119 @@
+"""GPU parity of the time-derivative (mass) kernels against the reference's
+unit-test golds and the CPU oracle.
+
+Bars: golds within 1e-14 absolute (the reference tests use 1e-15 on the same
+values, our sub-volume arithmetic differs in summation order); whole-mesh matrix
+and rhs within 1e-12 of the oracle relative to each row's / the vector's largest
+entry; the graph untouched (bit-exact).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import nalu_amd as na  # noqa: E402
+from nalu_amd.problems import lowmach_problem  # noqa: E402
+from tests.fixtures import UNIT_HEX, rel_err_rowwise, trig_dpdx, trig_velocity  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+TOL = 1e-12
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=DEV)
+
+
+def dense_from_csr(rp, ci, v, n):
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, ci[rp[r]:rp[r + 1]]] = v[rp[r]:rp[r + 1]]
+    return A
+
+
+def system(coords, conn, nd):
+    mesh = na.Mesh(coords, conn)
+    ls = na.LinearSystem(nd, na.SolverConfig())
+    ls.buildElemToNodeGraph(mesh)
+    ls.finalizeLinearSystem()
+    ls.zeroSystem()
+    return mesh, ls
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+def test_momentum_mass_gold(golds, lumped):
+    print("case: momentum mass gold lumped=%s" % lumped, flush=True)
+    g = golds["momentum_mass_lumped" if lumped else "momentum_mass"]
+    X = UNIT_HEX
+    mesh, ls = system(X, np.arange(8, dtype=np.int32)[None, :], 3)
+    z1, z3 = np.zeros(8), np.zeros((8, 3))
+    na.assemble_momentum_mass(ls, mesh, (dev(z1), dev(z1), dev(np.ones(8))), (dev(z3), dev(z3), dev(trig_velocity(X))),
+                              dev(trig_dpdx(X)), g["dt"], g["gamma"], lumped=lumped)
+    rp, ci, v, r = ls.csr()
+    want = np.eye(24) * g["lhs_diag"] if lumped else np.array(g["lhs"]).reshape(24, 24)
+    np.testing.assert_allclose(dense_from_csr(rp, ci, v, 24), want, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(r, g["rhs"], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+def test_continuity_mass_gold(golds, lumped):
+    print("case: continuity mass gold lumped=%s" % lumped, flush=True)
+    g = golds["continuity_mass"]
+    mesh, ls = system(UNIT_HEX, np.arange(8, dtype=np.int32)[None, :], 1)
+    z1 = np.zeros(8)
+    na.assemble_continuity_mass(ls, mesh, (dev(z1), dev(z1), dev(np.ones(8))), g["dt"], g["gamma"], lumped=lumped)
+    rp, ci, v, r = ls.csr()
+    np.testing.assert_allclose(v, 0.0, atol=0)
+    np.testing.assert_allclose(r, g["rhs_all"], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("lumped,balanced", [(False, False), (True, False), (False, True)])
+def test_mass_whole_mesh_parity(oracle, lumped, balanced):
+    n = 7
+    print("case: mass whole mesh n=%d lumped=%s balanced=%s" % (n, lumped, balanced), flush=True)
+    P = lowmach_problem(n, seed=13, rho_sigma=0.3)
+    X, conn = P["coords"], P["conn"]
+    N = X.shape[0]
+    rng = np.random.default_rng(17)
+    rho = [P["rho"] * (1 + 0.1 * rng.random(N)) for _ in range(3)]
+    u = [P["u"] + 0.1 * rng.standard_normal((N, 3)) for _ in range(3)]
+    gjp = P["gpdx"] + 0.1 * rng.standard_normal((N, 3))
+    dt, gam = 0.013, (1.5, -2.0, 0.5)
+    # momentum: mass added on top of the advection-diffusion system
+    mesh, ls = system(X, conn, 3)
+    mdot = torch.zeros((conn.shape[0], 12), dtype=torch.float64, device=DEV)
+    na.compute_mdot(mesh, dev(rho[2]), dev(P["p"]), dev(u[2]), dev(gjp), mdot, tau=P["tau"])
+    na.assemble_momentum(ls, mesh, dev(u[2]), dev(P["mu"]), mdot, mode="tile")
+    na.assemble_momentum_mass(ls, mesh, [dev(a) for a in rho], [dev(a) for a in u], dev(gjp), dt, gam,
+                              lumped=lumped, balanced_force=balanced)
+    rp, ci, v, r = ls.csr()
+    g3 = oracle.graph_build(N, conn, 3)
+    np.testing.assert_array_equal(ci, g3.col_idx)
+    ov, orr = oracle.assemble_momentum(g3, conn, X, u[2], P["mu"], mdot.cpu().numpy())
+    mv, mr = oracle.assemble_momentum_mass(g3, conn, X, rho, u, gjp, dt, gam, lumped=lumped, balanced=balanced)
+    e_mat = rel_err_rowwise(v, ov + mv, g3.row_ptr)
+    e_rhs = np.abs(r - (orr + mr)).max() / np.abs(orr + mr).max()
+    # the mass part alone is not lost in the sum
+    assert np.abs(mv).max() > 1e-3 * np.abs(ov).max()
+    # continuity: mass rhs only
+    mesh1, ls1 = system(X, conn, 1)
+    na.assemble_continuity_mass(ls1, mesh1, [dev(a) for a in rho], dt, gam, lumped=lumped)
+    _, _, v1, r1 = ls1.csr()
+    g1 = oracle.graph_build(N, conn, 1)
+    _, cr = oracle.assemble_continuity_mass(g1, conn, X, rho, dt, gam, lumped=lumped)
+    e_c = np.abs(r1 - cr).max() / np.abs(cr).max()
+    print("  rel err: momentum matrix %.2e rhs %.2e continuity rhs %.2e" % (e_mat, e_rhs, e_c), flush=True)
+    assert e_mat <= TOL and e_rhs <= TOL and e_c <= TOL
+    assert not np.any(v1)
+
+
+def test_mass_rejects_bad_arguments():
+    mesh, ls = system(UNIT_HEX, np.arange(8, dtype=np.int32)[None, :], 1)
+    z = dev(np.zeros(8))
+    with pytest.raises(na.NaluError):
+        na.assemble_continuity_mass(ls, mesh, (z, z, z), 0.0, (1.0, -1.0, 0.0))
+    with pytest.raises(na.NaluError):
+        na.assemble_continuity_mass(ls, mesh, (z, z, z), 0.1, (0.0, -1.0, 0.0))
+    with pytest.raises(na.NaluError):  # numDof 1 system for the momentum kernel
+        na.assemble_momentum_mass(ls, mesh, (z, z, z), (z, z, z), z, 0.1)

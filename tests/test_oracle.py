@@ -50,6 +50,60 @@ def test_momentum_gold(oracle, golds):
     np.testing.assert_allclose(rhs, g["rhs"], rtol=0, atol=g["tol"])
 
 
+def mass_fields(X):
+    """Fields of the reference's mass-kernel unit tests: the NP1 states hold the
+    test functions (density 1), the N / NM1 states are the fields' zero initial
+    state; gamma = (1, -1, 0), dt = 0.1."""
+    rho3 = np.stack([np.zeros(8), np.zeros(8), np.ones(8)])
+    u3 = np.stack([np.zeros((8, 3)), np.zeros((8, 3)), trig_velocity(X)])
+    return rho3, u3, trig_dpdx(X)
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+def test_momentum_mass_gold(oracle, golds, lumped):
+    g = golds["momentum_mass_lumped" if lumped else "momentum_mass"]
+    rho3, u3, gjp = mass_fields(UNIT_HEX)
+    lhs, rhs = oracle.momentum_mass_elem(UNIT_HEX, rho3, u3, gjp, g["dt"], g["gamma"], lumped=lumped)
+    want = np.eye(24) * g["lhs_diag"] if lumped else np.array(g["lhs"]).reshape(24, 24)
+    np.testing.assert_allclose(lhs, want, rtol=0, atol=g["tol"])
+    np.testing.assert_allclose(rhs, g["rhs"], rtol=0, atol=g["tol"])
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+def test_continuity_mass_gold(oracle, golds, lumped):
+    g = golds["continuity_mass"]
+    rho3, _, _ = mass_fields(UNIT_HEX)
+    rhs = oracle.continuity_mass_elem(UNIT_HEX, rho3, g["dt"], g["gamma"], lumped=lumped)
+    np.testing.assert_allclose(rhs, np.full(8, g["rhs_all"]), rtol=0, atol=g["tol"])
+
+
+def test_mass_closed_forms(oracle, golds):
+    """On the perturbed element: with constant fields every formulation gives
+    rhs = -(g1 + g2 + g3) rho u scV / dt - Gjp scV and, for continuity,
+    -(g1 + g2 + g3) rho scV g1 / dt^2; the consistent mass rows sum to the lumped
+    diagonal g1 rho scV / dt (partition of unity of the scv shape functions)."""
+    X = np.array(golds["perturbed_hex8_coords"]["values"]).reshape(8, 3)
+    scv = oracle.scv_volume(X)
+    gam, dt, rho = (1.5, -2.0, 0.7), 0.05, 1.3
+    r3 = np.full((3, 8), rho)
+    uv, gv = np.array([0.7, -0.2, 0.4]), np.array([0.1, 0.2, -0.3])
+    uc = np.broadcast_to(uv, (3, 8, 3)).copy()
+    gc = np.broadcast_to(gv, (8, 3)).copy()
+    diag = {}
+    for lumped in (False, True):
+        lhs, rhs = oracle.momentum_mass_elem(X, r3, uc, gc, dt, gam, lumped=lumped)
+        want = -(sum(gam) * rho * uv[None, :] / dt + gv[None, :]) * scv[:, None]
+        np.testing.assert_allclose(rhs.reshape(8, 3), want, rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(lhs.sum(axis=1), np.repeat(gam[0] * rho * scv / dt, 3), rtol=1e-12)
+        cm = oracle.continuity_mass_elem(X, r3, dt, gam, lumped=lumped)
+        np.testing.assert_allclose(cm, -sum(gam) * rho * scv * gam[0] / dt ** 2, rtol=1e-12)
+        diag[lumped] = lhs
+    # lumped: diagonal only; consistent: 3x3 blocks are multiples of the identity
+    assert np.count_nonzero(diag[True] - np.diag(np.diag(diag[True]))) == 0
+    blk = diag[False].reshape(8, 3, 8, 3)
+    assert np.count_nonzero(blk[:, 0, :, 1]) == 0 and np.allclose(blk[:, 0, :, 0], blk[:, 2, :, 2])
+
+
 @pytest.mark.parametrize("shifted,key", [(False, "hex8_scv_grad_op"), (True, "hex8_scv_shifted_grad_op")])
 def test_scv_grad_op_gold(oracle, golds, shifted, key):
     X = np.array(golds["perturbed_hex8_coords"]["values"]).reshape(8, 3)
