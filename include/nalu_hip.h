@@ -177,6 +177,12 @@ int nalu_linsys_spmv_permuted(nalu_linsys *ls, const double *x_dev, double *y_de
 /* Solve A x = b for a caller-provided rhs (x in/out is the initial guess). */
 int nalu_linsys_solve_rhs(nalu_linsys *ls, const double *b_dev, double *x_dev, int *iters,
                           double *final_rel_resid);
+/* k right-hand sides sharing the matrix and one preconditioner set-up (the
+ * three momentum components of the segregated form): B, X [k][nrows] device
+ * arrays (X holds the initial guesses), iters / rel: [k] host arrays or NULL.
+ * Returns the worst status (0 converged, 1 iteration cap). */
+int nalu_linsys_solve_multi(nalu_linsys *ls, int k, const double *B_dev, double *X_dev, int *iters,
+                            double *rel);
 
 /* ------------------------------------------------------ assembly kernels */
 /* Each fused kernel performs, for every element: gather, Hex8 SCS geometry,
@@ -221,6 +227,22 @@ int nalu_assemble_momentum_mass(nalu_linsys *ls, const nalu_mesh *m, const doubl
                                 const double *rho_n, const double *rho_np1, const double *u_nm1,
                                 const double *u_n, const double *u_np1, const double *gjp, double dt,
                                 const double *gamma, int lumped, int balanced_force);
+/* Momentum as three right-hand sides sharing one scalar matrix (the segregated
+ * form of MomentumAdvDiffElemKernel, src/kernel/MomentumAdvDiffElemKernel.C:72-186):
+ * ls is a numDof 1 system on the mesh; the matrix gets the advection-diffusion
+ * operator a_c I of every 3x3 block (the transpose-gradient stress coupling is
+ * left out of the lhs), rhs3 [3][nrows] (row order) is added the full residual of
+ * each velocity component -- the same rhs as the coupled numDof 3 system.  The
+ * caller zeroes rhs3; the matrix follows zeroSystem like the other kernels. */
+int nalu_assemble_momentum_segregated(nalu_linsys *ls, const nalu_mesh *m, const double *u_dev,
+                                      const double *mu_dev, const double *mdot_dev, double include_divu,
+                                      int skew_symmetric, int shifted_grad_op, double *rhs3_dev);
+/* MomentumMassElemKernel into the segregated system (scalar lhs, rhs3 [3][nrows]). */
+int nalu_assemble_momentum_mass_segregated(nalu_linsys *ls, const nalu_mesh *m, const double *rho_nm1,
+                                           const double *rho_n, const double *rho_np1, const double *u_nm1,
+                                           const double *u_n, const double *u_np1, const double *gjp,
+                                           double dt, const double *gamma, int lumped, int balanced_force,
+                                           double *rhs3_dev);
 /* ContinuityMassElemKernel (src/kernel/ContinuityMassElemKernel.C:27-118), numDof 1:
  * rhs only, scaled by 1 / (dt / g1) like the continuity system. */
 int nalu_assemble_continuity_mass(nalu_linsys *ls, const nalu_mesh *m, const double *rho_nm1,

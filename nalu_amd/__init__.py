@@ -111,6 +111,10 @@ def lib():
         L.nalu_assemble_momentum_mass.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _dp,
                                                   C.c_int, C.c_int]
         L.nalu_assemble_continuity_mass.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, _dp, C.c_int]
+        L.nalu_assemble_momentum_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp]
+        L.nalu_assemble_momentum_mass_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
+                                                             _dp, C.c_int, C.c_int, _vp]
+        L.nalu_linsys_solve_multi.argtypes = [_vp, C.c_int, _vp, _vp, _i32p, _dp]
         # learned preconditioner
         _fp = C.POINTER(C.c_float)
         L.nalu_gnn_create.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp)]
@@ -322,6 +326,15 @@ class LinearSystem:
     def solve(self, delta_out=None):
         """Returns 0 when converged, 1 when max_iterations was reached."""
         return _check(lib().nalu_linsys_solve(self._h, _ptr(delta_out)), "solve")
+
+    def solve_multi(self, B, X):
+        """k right-hand sides (rows of B [k][n]) sharing the matrix; X [k][n] holds the
+        initial guesses.  Returns (worst status, iterations per rhs, rel. residual per rhs)."""
+        k = int(B.shape[0])
+        its = (C.c_int32 * k)()
+        rel = (C.c_double * k)()
+        st = _check(lib().nalu_linsys_solve_multi(self._h, k, _ptr(B), _ptr(X), its, rel), "solve_multi")
+        return st, list(its), list(rel)
 
     def solve_rhs(self, b, x):
         it = C.c_int(0)
@@ -564,6 +577,23 @@ def assemble_momentum_mass(ls, mesh, rho, u, gjp, dt, gamma=(1.0, -1.0, 0.0), lu
     _check(lib().nalu_assemble_momentum_mass(ls.handle, mesh.handle, r[0], r[1], r[2], v[0], v[1], v[2], _ptr(gjp),
                                              float(dt), g.ctypes.data_as(_dp), int(lumped), int(balanced_force)),
            "assemble_momentum_mass")
+
+
+def assemble_momentum_segregated(ls, mesh, u, mu, mdot, rhs3, include_divu=0.0, skew=False, shifted=False):
+    """Momentum as 3 right-hand sides sharing one scalar matrix (numDof 1 system);
+    rhs3 [3][N] (device, row order) is added the residual of each component."""
+    _check(lib().nalu_assemble_momentum_segregated(ls.handle, mesh.handle, _ptr(u), _ptr(mu), _ptr(mdot),
+                                                   float(include_divu), int(skew), int(shifted), _ptr(rhs3)),
+           "assemble_momentum_segregated")
+
+
+def assemble_momentum_mass_segregated(ls, mesh, rho, u, gjp, dt, rhs3, gamma=(1.0, -1.0, 0.0), lumped=False,
+                                      balanced_force=False):
+    r, v, g = _states(rho, "momentum_mass rho"), _states(u, "momentum_mass u"), _gammas(gamma)
+    _check(lib().nalu_assemble_momentum_mass_segregated(ls.handle, mesh.handle, r[0], r[1], r[2], v[0], v[1], v[2],
+                                                        _ptr(gjp), float(dt), g.ctypes.data_as(_dp), int(lumped),
+                                                        int(balanced_force), _ptr(rhs3)),
+           "assemble_momentum_mass_segregated")
 
 
 def assemble_continuity_mass(ls, mesh, rho, dt, gamma=(1.0, -1.0, 0.0), lumped=False):

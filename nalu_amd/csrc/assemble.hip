@@ -574,6 +574,66 @@ void check_flag(DevBuf<int>& flag, hipStream_t s, const char* msg) {
   NALU_REQUIRE(h == 0, NALU_ERR_ARG, msg);
 }
 
+// ---------------------------------------------------------------- segregated momentum
+// Momentum as three right-hand sides sharing one scalar matrix: the coupled
+// MomentumAdvDiffElemKernel (src/kernel/MomentumAdvDiffElemKernel.C:72-186) with
+// the transpose-gradient stress coupling kept on the rhs only (the rhs is the
+// full residual, so converged velocities are unchanged).  Owner-computes row
+// gather on a numDof 1 graph: the row segment accumulates in LDS, the three
+// residual components go to rhs3[i][row].
+template <bool SKEW, bool SHIFT>
+__global__ __launch_bounds__(256) void k_momentum_seg_gather(
+    const int32_t* __restrict__ conn, const double* __restrict__ coords, const double* __restrict__ u,
+    const double* __restrict__ mu, const double* __restrict__ mdot, double incl_divu,
+    const int32_t* __restrict__ row_node, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_elem,
+    const int32_t* __restrict__ nrp, const uint64_t* __restrict__ elem_pos, int64_t nrows,
+    double* __restrict__ vals, double* __restrict__ rhs3, int accumulate) {
+  extern __shared__ double seg[];
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t rend = min(r0 + (int64_t)blockDim.x, nrows);
+  const int64_t r = r0 + threadIdx.x;
+  const int32_t s0 = nrp[r0], s1 = nrp[rend];
+  const int nseg = s1 - s0;
+  for (int i = threadIdx.x; i < nseg; i += blockDim.x) seg[i] = 0.0;
+  __syncthreads();
+  double rr[3] = {0.0, 0.0, 0.0};
+  if (r < nrows) {
+    const int32_t a = row_node ? row_node[r] : (int32_t)r;
+    double* myrow = seg + (nrp[r] - s0);
+    for (int32_t q = inc_ptr[a]; q < inc_ptr[a + 1]; ++q) {
+      const int32_t t = inc_elem[q];
+      const int64_t e = t >> 3;
+      int32_t nd[8];
+      load_nodes(conn, e, nd);
+      double x[8][3];
+      load_vec3(coords, nd, x);
+      const uint64_t pk = elem_pos[t];
+      const double* md = mdot + 12 * e;
+      const hex8::Gather gt{nd};
+      auto emit = [&](int c, const double (&blk)[3][3]) { myrow[(int)((pk >> (8 * c)) & 0xff)] += blk[0][0]; };
+      switch (t & 7) {
+#define NALU_SG(Lx)                                                                                           \
+  case Lx: {                                                                                                  \
+    const double m3[3] = {md[hex8::adj_ip(Lx, 0)], md[hex8::adj_ip(Lx, 1)], md[hex8::adj_ip(Lx, 2)]};         \
+    hex8::row_momentum<Lx, SKEW, SHIFT, double, true>(x, gt, u, mu, m3, incl_divu, rr, emit);                 \
+  } break;
+        NALU_SG(0) NALU_SG(1) NALU_SG(2) NALU_SG(3) NALU_SG(4) NALU_SG(5) NALU_SG(6) NALU_SG(7)
+#undef NALU_SG
+      }
+    }
+  }
+  __syncthreads();
+  if (accumulate) {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
+  } else {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] = seg[i];
+  }
+  if (r < nrows) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rhs3[i * nrows + r] += rr[i];
+  }
+}
+
 // ---------------------------------------------------------------- mass (time derivative)
 // MomentumMassElemKernel (src/kernel/MomentumMassElemKernel.C:95-170) and
 // ContinuityMassElemKernel (src/kernel/ContinuityMassElemKernel.C:84-118).  The
@@ -589,7 +649,7 @@ struct MassArgs {
   int balanced;          // densFac = rhoNp1 (balanced force) or 1
 };
 
-template <int LA, bool MOM, bool LUMPED>
+template <int LA, bool MOM, bool LUMPED, bool SEG>
 __device__ __forceinline__ void mass_row(const MassArgs& A, const int32_t (&nd)[8], const double (&x)[8][3],
                                          uint64_t pk, double* __restrict__ row, int32_t L, double (&rh)[3]) {
   const double scV = hex8::scv_volume<LA>(x);
@@ -620,8 +680,12 @@ __device__ __forceinline__ void mass_row(const MassArgs& A, const int32_t (&nd)[
       if (LUMPED && c != LA) continue;
       const double lf = hex8::scv_shape<LA, LUMPED>(c) * A.g1 * rs[2] * scV / A.dt;
       const int p = (int)((pk >> (8 * c)) & 0xff);
+      if (SEG) {
+        row[p] += lf;
+      } else {
 #pragma unroll
-      for (int j = 0; j < 3; ++j) row[3 * j * L + 3 * p + j] += lf;
+        for (int j = 0; j < 3; ++j) row[3 * j * L + 3 * p + j] += lf;
+      }
     }
   } else {
     const double projTimeScale = A.dt / A.g1;
@@ -629,7 +693,7 @@ __device__ __forceinline__ void mass_row(const MassArgs& A, const int32_t (&nd)[
   }
 }
 
-template <bool MOM, bool LUMPED>
+template <bool MOM, bool LUMPED, bool SEG>
 __global__ __launch_bounds__(256) void k_mass_gather(MassArgs A, const int32_t* __restrict__ conn,
                                                      const double* __restrict__ coords,
                                                      const int32_t* __restrict__ row_node,
@@ -642,7 +706,7 @@ __global__ __launch_bounds__(256) void k_mass_gather(MassArgs A, const int32_t* 
   if (r >= nnodes) return;
   const int32_t a = row_node ? row_node[r] : (int32_t)r;
   const int32_t L = nrp[r + 1] - nrp[r];
-  double* row = vals + (MOM ? 9 : 1) * (int64_t)nrp[r];
+  double* row = vals + (MOM && !SEG ? 9 : 1) * (int64_t)nrp[r];
   double rh[3] = {0.0, 0.0, 0.0};
   for (int32_t q = inc_ptr[a]; q < inc_ptr[a + 1]; ++q) {
     const int32_t t = inc_elem[q];
@@ -654,12 +718,15 @@ __global__ __launch_bounds__(256) void k_mass_gather(MassArgs A, const int32_t* 
     const uint64_t pk = elem_pos[t];
     switch (t & 7) {
 #define NALU_MS(Lx) \
-  case Lx: mass_row<Lx, MOM, LUMPED>(A, nd, x, pk, row, L, rh); break;
+  case Lx: mass_row<Lx, MOM, LUMPED, SEG>(A, nd, x, pk, row, L, rh); break;
       NALU_MS(0) NALU_MS(1) NALU_MS(2) NALU_MS(3) NALU_MS(4) NALU_MS(5) NALU_MS(6) NALU_MS(7)
 #undef NALU_MS
     }
   }
-  if (MOM) {
+  if (MOM && SEG) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) rhs[j * nnodes + r] += rh[j];
+  } else if (MOM) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) rhs[3 * r + j] += rh[j];
   } else {
@@ -866,17 +933,19 @@ extern "C" int nalu_linsys_reset_rows(nalu_linsys* ls, int64_t nn, const int32_t
 }
 
 namespace {
-void launch_mass(nalu_linsys* ls, const nalu_mesh* m, const MassArgs& A, bool mom, bool lumped) {
+void launch_mass(nalu_linsys* ls, const nalu_mesh* m, const MassArgs& A, bool mom, bool lumped,
+                 double* rhs3 = nullptr) {
   build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
   materialize_pending_zero(ls);
   const dim3 g(blocks_for(ls->num_nodes, 256)), b(256);
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, g, b, 0, ls->stream, A, m->conn.p, m->coords.p, (const int32_t*)ls->row_node.p,
                        m->inc_ptr.p, m->inc_elem.p, ls->nrp.p, ls->elem_pos.p, ls->num_nodes, ls->values.p,
-                       ls->rhs.p);
+                       rhs3 ? rhs3 : ls->rhs.p);
   };
-  if (mom) lumped ? go(k_mass_gather<true, true>) : go(k_mass_gather<true, false>);
-  else lumped ? go(k_mass_gather<false, true>) : go(k_mass_gather<false, false>);
+  if (rhs3) lumped ? go(k_mass_gather<true, true, true>) : go(k_mass_gather<true, false, true>);
+  else if (mom) lumped ? go(k_mass_gather<true, true, false>) : go(k_mass_gather<true, false, false>);
+  else lumped ? go(k_mass_gather<false, true, false>) : go(k_mass_gather<false, false, false>);
   NALU_HIP_CHECK(hipGetLastError());
 }
 
@@ -916,6 +985,53 @@ extern "C" int nalu_assemble_continuity_mass(nalu_linsys* ls, const nalu_mesh* m
     const MassArgs A{{rho_nm1, rho_n, rho_np1}, {nullptr, nullptr, nullptr}, nullptr, dt, gamma[0], gamma[1],
                      gamma[2], 0};
     launch_mass(ls, m, A, false, lumped != 0);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_assemble_momentum_segregated(nalu_linsys* ls, const nalu_mesh* m, const double* u,
+                                                 const double* mu, const double* mdot, double include_divu,
+                                                 int skew, int shifted, double* rhs3) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 1);
+    NALU_REQUIRE(u && mu && mdot && rhs3, NALU_ERR_ARG, "momentum_segregated: null field");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
+    if (ls->rhs_pending_zero) NALU_HIP_CHECK(hipMemsetAsync(ls->rhs.p, 0, ls->rhs.bytes(), ls->stream));
+    ls->rhs_pending_zero = false;
+    const int accumulate = ls->values_pending_zero ? 0 : 1;
+    constexpr int B = 256;
+    const size_t lds = (size_t)B * ls->max_node_row_len * sizeof(double);
+    auto go = [&](auto kern) {
+      set_lds_limit((const void*)kern, lds);
+      hipLaunchKernelGGL(kern, dim3(blocks_for(ls->nrows, B)), dim3(B), lds, ls->stream, m->conn.p, m->coords.p, u,
+                         mu, mdot, include_divu, (const int32_t*)ls->row_node.p, m->inc_ptr.p, m->inc_elem.p,
+                         ls->nrp.p, ls->elem_pos.p, ls->nrows, ls->values.p, rhs3, accumulate);
+    };
+    if (skew && shifted) go(k_momentum_seg_gather<true, true>);
+    else if (skew) go(k_momentum_seg_gather<true, false>);
+    else if (shifted) go(k_momentum_seg_gather<false, true>);
+    else go(k_momentum_seg_gather<false, false>);
+    ls->values_pending_zero = false;
+    NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_assemble_momentum_mass_segregated(nalu_linsys* ls, const nalu_mesh* m, const double* rho_nm1,
+                                                      const double* rho_n, const double* rho_np1,
+                                                      const double* u_nm1, const double* u_n, const double* u_np1,
+                                                      const double* gjp, double dt, const double* gamma, int lumped,
+                                                      int balanced_force, double* rhs3) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 1);
+    check_mass_args(dt, gamma);
+    NALU_REQUIRE(rho_nm1 && rho_n && rho_np1 && u_nm1 && u_n && u_np1 && gjp && rhs3, NALU_ERR_ARG,
+                 "momentum_mass_segregated: null field");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    const MassArgs A{{rho_nm1, rho_n, rho_np1}, {u_nm1, u_n, u_np1}, gjp, dt, gamma[0], gamma[1], gamma[2],
+                     balanced_force ? 1 : 0};
+    launch_mass(ls, m, A, true, lumped != 0, rhs3);
     return NALU_OK;
   });
 }

@@ -367,7 +367,9 @@ __device__ __forceinline__ R row_continuity(const R (&x)[8][3], const Gather& gt
 //   lhs += S [ delta_ij (N_adv(ip,c) mdot - mu_ip dN_c/dx . A) - mu_ip dN_c/dx_i A_j ]
 //   rhs += S [ -mdot u_ip,i - 2/3 mu_ip divU A_i incl + mu_ip sum_c dN_c/dx_i (A . u_c)
 //              + mu_ip sum_c (dN_c/dx . A) u_c,i ]
-// emit(c, blk) receives the 3x3 block blk[i][j] of column node c.
+// emit(c, blk) receives the 3x3 block blk[i][j] of column node c.  SEG leaves
+// the transpose-gradient term -mu_ip dN_c/dx_i A_j out of the lhs (it stays in
+// the rhs), so that blk = a_c I: the three components share one scalar matrix.
 template <int IP, int C, bool SHIFT, typename R>
 __device__ __forceinline__ void dndx_scaled(const R (&Cs)[3][3], R (&dn)[3]) {
 #pragma unroll
@@ -376,7 +378,7 @@ __device__ __forceinline__ void dndx_scaled(const R (&Cs)[3][3], R (&dn)[3]) {
             Cs[i][2] * R(deriv(IP, C, 2, SHIFT));
 }
 
-template <int LA, bool SKEW, bool SHIFT, typename JR, typename R, class Emit>
+template <int LA, bool SKEW, bool SHIFT, typename JR, bool SEG = false, typename R, class Emit>
 __device__ __forceinline__ void row_momentum(const R (&x)[8][3], const Gather& gt, const double* __restrict__ u,
                                              const double* __restrict__ mu, const R (&mdot3)[3], R incl_divu,
                                              R (&rhs)[3], Emit&& emit) {
@@ -451,8 +453,10 @@ __device__ __forceinline__ void row_momentum(const R (&x)[8][3], const Gather& g
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         blk[i][i] += S * (adv - h);
+        if (!SEG) {
 #pragma unroll
-        for (int j = 0; j < 3; ++j) blk[i][j] -= S * (dn[i] * A[kk][j]);
+          for (int j = 0; j < 3; ++j) blk[i][j] -= S * (dn[i] * A[kk][j]);
+        }
         rhs[i] += S * (dn[i] * Au + h * uc[i]);
       }
     }

@@ -366,6 +366,31 @@ extern "C" int nalu_linsys_solve_rhs(nalu_linsys* ls, const double* b, double* x
   });
 }
 
+// k right-hand sides sharing the system's matrix and preconditioner set-up
+// (B, X: [k][nrows] device arrays; iters, rel: [k] host arrays or NULL).
+extern "C" int nalu_linsys_solve_multi(nalu_linsys* ls, int k, const double* B, double* X, int* iters, double* rel) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized && B && X && k >= 1, NALU_ERR_ARG, "solve_multi: bad arguments");
+    if (!ls->precond_ready) {
+      PhaseTimer t(ls, PH_PRECOND);
+      precond_setup(ls);
+    }
+    PhaseTimer t(ls, PH_SOLVE);
+    int status = 0, total = 0;
+    for (int i = 0; i < k; ++i) {
+      int it = 0;
+      double r = 0.0;
+      status = std::max(status, krylov_solve(ls, B + (size_t)i * ls->nrows, X + (size_t)i * ls->nrows, &it, &r));
+      total += it;
+      if (iters) iters[i] = it;
+      if (rel) rel[i] = r;
+    }
+    ls->iters = total;
+    NALU_HIP_CHECK(hipStreamSynchronize(ls->stream));
+    return status;
+  });
+}
+
 extern "C" int nalu_linsys_iterations(const nalu_linsys* ls) { return ls ? ls->iters : -1; }
 extern "C" double nalu_linsys_linear_residual(const nalu_linsys* ls) { return ls ? ls->linear_residual : NAN; }
 extern "C" double nalu_linsys_nonlinear_residual(const nalu_linsys* ls) { return ls ? ls->nonlinear_residual : NAN; }

@@ -288,6 +288,25 @@ def assemble_momentum(g, conn, coords, u, mu, mdot, include_divu=0.0, skew=False
     return values, rhs
 
 
+def momentum_segregated(g1, g3, conn, coords, u, mu, mdot, include_divu=0.0, skew=False, shifted=False):
+    """The segregated (3 right-hand sides) momentum system from two pinned
+    assemblies: the coupled numDof 3 system (reference golds) and the scalar
+    diffusion operator D with conductivity mu (its own gold).  The coupled 3x3
+    block of columns c is (adv + D) I + T with T_ij = -sum S mu dN_c/dx_i A_j and
+    trace(T) = D, so the shared scalar matrix (adv + D) = (trace(block) - D) / 3;
+    the right-hand sides are the coupled residual components.
+    Returns (values in g1's CSR order, rhs3 [3][N])."""
+    import scipy.sparse as sp
+    v3, r3 = assemble_momentum(g3, conn, coords, u, mu, mdot, include_divu, skew, shifted)
+    dv, _ = assemble_scalar_diff(g1, conn, coords, np.zeros(g1.nrows), mu, shifted)
+    n = g1.nrows
+    A3 = sp.csr_matrix((v3, g3.col_idx, g3.row_ptr), shape=(3 * n, 3 * n))
+    T = (A3[0::3, 0::3] + A3[1::3, 1::3] + A3[2::3, 2::3]).tocsr()
+    rows = np.repeat(np.arange(n), np.diff(g1.row_ptr))
+    tr = np.asarray(T[rows, g1.col_idx]).ravel()
+    return (tr - dv) / 3.0, r3.reshape(n, 3).T.copy()
+
+
 def momentum_mass_elem(coords, rho3, u3, gjp, dt, gamma, lumped=False, balanced=False):
     """One element: rho3 [3][8] (Nm1, N, Np1), u3 [3][8][3], gjp [8][3] -> lhs [24][24], rhs [24]."""
     lhs = np.zeros((24, 24))
