@@ -205,6 +205,17 @@ int nalu_compute_mdot(const nalu_mesh *m, const double *rho_dev, const double *p
 int nalu_compute_mdot_f32(const nalu_mesh *m, const double *rho_dev, const double *p_dev,
                           const double *u_dev, const double *gpdx_dev, double proj_time_scale,
                           int shift_mdot, int shift_poisson, double *mdot_dev, void *stream);
+/* Dual nodal volume: the scv volumes of every element summed to their nodes
+ * (ComputeGeometryInteriorAlgorithm, src/ComputeGeometryInteriorAlgorithm.C:77-120);
+ * dual_vol_dev [num_nodes] is overwritten.  stream: hipStream_t or NULL. */
+int nalu_compute_dual_volume(const nalu_mesh *m, double *dual_vol_dev, void *stream);
+/* Projected nodal gradient, interior Green-Gauss part (AssembleNodalGradElemAlgorithm,
+ * src/AssembleNodalGradElemAlgorithm.C:85-140): dqdx_dev [num_nodes][3] is
+ * overwritten with sum_scs (+-) q_ip A / V_node; shifted uses the shifted scs
+ * shape functions.  The exposed-face closure (AssembleNodalGradBoundaryAlgorithm)
+ * is not part of this call. */
+int nalu_compute_nodal_gradient(const nalu_mesh *m, const double *q_dev, const double *dual_vol_dev,
+                                int shifted, double *dqdx_dev, void *stream);
 /* ContinuityAdvElemKernel (src/kernel/ContinuityAdvElemKernel.C:88-148).
  * Assembly kernels evaluate with FP32 arithmetic (FP64 Jacobians, FP64
  * accumulation into the matrix) when the system was created with precision

@@ -115,6 +115,8 @@ def lib():
         L.nalu_assemble_momentum_mass_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
                                                              _dp, C.c_int, C.c_int, _vp]
         L.nalu_linsys_solve_multi.argtypes = [_vp, C.c_int, _vp, _vp, _i32p, _dp]
+        L.nalu_compute_dual_volume.argtypes = [_vp, _vp, _vp]
+        L.nalu_compute_nodal_gradient.argtypes = [_vp, _vp, _vp, C.c_int, _vp, _vp]
         # learned preconditioner
         _fp = C.POINTER(C.c_float)
         L.nalu_gnn_create.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp)]
@@ -544,6 +546,17 @@ def compute_mdot(mesh, rho, p, u, gpdx, mdot, tau=1.0, shift_mdot=False, shift_p
     fn = lib().nalu_compute_mdot_f32 if precision == "fp32" else lib().nalu_compute_mdot
     _check(fn(mesh.handle, _ptr(rho), _ptr(p), _ptr(u), _ptr(gpdx), float(tau), int(shift_mdot), int(shift_poisson),
               _ptr(mdot), stream), "compute_mdot")
+
+
+def compute_dual_volume(mesh, dual_vol, stream=None):
+    """Dual nodal volume [N] (sum of the scv volumes around each node)."""
+    _check(lib().nalu_compute_dual_volume(mesh.handle, _ptr(dual_vol), stream), "compute_dual_volume")
+
+
+def compute_nodal_gradient(mesh, q, dual_vol, dqdx, shifted=False, stream=None):
+    """Interior Green-Gauss nodal gradient [N][3] of the nodal field q."""
+    _check(lib().nalu_compute_nodal_gradient(mesh.handle, _ptr(q), _ptr(dual_vol), int(shifted), _ptr(dqdx), stream),
+           "compute_nodal_gradient")
 
 
 def assemble_continuity(ls, mesh, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False,

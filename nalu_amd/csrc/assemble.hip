@@ -574,6 +574,81 @@ void check_flag(DevBuf<int>& flag, hipStream_t s, const char* msg) {
   NALU_REQUIRE(h == 0, NALU_ERR_ARG, msg);
 }
 
+// ---------------------------------------------------------------- kernel-input producers
+// Dual nodal volume (ComputeGeometryInteriorAlgorithm, src/ComputeGeometryInteriorAlgorithm.C:77-120)
+// and the interior Green-Gauss nodal gradient (AssembleNodalGradElemAlgorithm,
+// src/AssembleNodalGradElemAlgorithm.C:85-140), owner-computes: one thread per
+// node walks its incident elements; the node's scv (ip = its local index) and
+// its three adjacent scs (sign +1 where it is the left node) are all that
+// touch it, so every result is written once, with no atomics.
+__global__ __launch_bounds__(256) void k_dual_volume(const int32_t* __restrict__ conn,
+                                                     const double* __restrict__ coords,
+                                                     const int32_t* __restrict__ inc_ptr,
+                                                     const int32_t* __restrict__ inc_elem, int64_t nnodes,
+                                                     double* __restrict__ vol) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nnodes) return;
+  double v = 0.0;
+  for (int32_t q = inc_ptr[a]; q < inc_ptr[a + 1]; ++q) {
+    const int32_t t = inc_elem[q];
+    int32_t nd[8];
+    load_nodes(conn, t >> 3, nd);
+    double x[8][3];
+    load_vec3(coords, nd, x);
+    switch (t & 7) {
+#define NALU_DV(Lx) \
+  case Lx: v += hex8::scv_volume<Lx>(x); break;
+      NALU_DV(0) NALU_DV(1) NALU_DV(2) NALU_DV(3) NALU_DV(4) NALU_DV(5) NALU_DV(6) NALU_DV(7)
+#undef NALU_DV
+    }
+  }
+  vol[a] = v;
+}
+
+template <int LA, bool SHIFT>
+__device__ __forceinline__ void nodal_grad_part(const double (&x)[8][3], const int32_t (&nd)[8],
+                                                const double* __restrict__ qf, double (&g)[3]) {
+  hex8::static_for<3>([&](auto kk) {
+    constexpr int IP = hex8::adj_ip(LA, decltype(kk)::value);
+    constexpr double S = hex8::adj_sign(LA, decltype(kk)::value);
+    double A[3];
+    hex8::scs_area<IP>(x, A);
+    double qIp = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) qIp += hex8::kScsTab.shape[SHIFT][IP][c] * qf[nd[c]];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g[j] += S * (qIp * A[j]);
+  });
+}
+
+template <bool SHIFT>
+__global__ __launch_bounds__(256) void k_nodal_grad(const int32_t* __restrict__ conn,
+                                                    const double* __restrict__ coords, const double* __restrict__ qf,
+                                                    const double* __restrict__ vol,
+                                                    const int32_t* __restrict__ inc_ptr,
+                                                    const int32_t* __restrict__ inc_elem, int64_t nnodes,
+                                                    double* __restrict__ dqdx) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nnodes) return;
+  double g[3] = {0.0, 0.0, 0.0};
+  for (int32_t q = inc_ptr[a]; q < inc_ptr[a + 1]; ++q) {
+    const int32_t t = inc_elem[q];
+    int32_t nd[8];
+    load_nodes(conn, t >> 3, nd);
+    double x[8][3];
+    load_vec3(coords, nd, x);
+    switch (t & 7) {
+#define NALU_NG(Lx) \
+  case Lx: nodal_grad_part<Lx, SHIFT>(x, nd, qf, g); break;
+      NALU_NG(0) NALU_NG(1) NALU_NG(2) NALU_NG(3) NALU_NG(4) NALU_NG(5) NALU_NG(6) NALU_NG(7)
+#undef NALU_NG
+    }
+  }
+  const double inv = 1.0 / vol[a];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) dqdx[3 * a + j] = g[j] * inv;
+}
+
 // ---------------------------------------------------------------- segregated momentum
 // Momentum as three right-hand sides sharing one scalar matrix: the coupled
 // MomentumAdvDiffElemKernel (src/kernel/MomentumAdvDiffElemKernel.C:72-186) with
@@ -1032,6 +1107,35 @@ extern "C" int nalu_assemble_momentum_mass_segregated(nalu_linsys* ls, const nal
     const MassArgs A{{rho_nm1, rho_n, rho_np1}, {u_nm1, u_n, u_np1}, gjp, dt, gamma[0], gamma[1], gamma[2],
                      balanced_force ? 1 : 0};
     launch_mass(ls, m, A, true, lumped != 0, rhs3);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_compute_dual_volume(const nalu_mesh* m, double* dual_vol, void* stream) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(m && dual_vol, NALU_ERR_ARG, "dual_volume: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    build_node_incidence(const_cast<nalu_mesh*>(m), s);
+    hipLaunchKernelGGL(k_dual_volume, dim3(blocks_for(m->num_nodes, 256)), dim3(256), 0, s, m->conn.p, m->coords.p,
+                       m->inc_ptr.p, m->inc_elem.p, m->num_nodes, dual_vol);
+    NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_compute_nodal_gradient(const nalu_mesh* m, const double* q, const double* dual_vol, int shifted,
+                                           double* dqdx, void* stream) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(m && q && dual_vol && dqdx, NALU_ERR_ARG, "nodal_gradient: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    build_node_incidence(const_cast<nalu_mesh*>(m), s);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(blocks_for(m->num_nodes, 256)), dim3(256), 0, s, m->conn.p, m->coords.p, q,
+                         dual_vol, m->inc_ptr.p, m->inc_elem.p, m->num_nodes, dqdx);
+    };
+    if (shifted) go(k_nodal_grad<true>);
+    else go(k_nodal_grad<false>);
+    NALU_HIP_CHECK(hipGetLastError());
     return NALU_OK;
   });
 }

@@ -687,6 +687,54 @@ void or_assemble_momentum(int64_t nelem, const int32_t *conn, const double *coor
 }
 
 /* ------------------------------------------------------------------------ */
+/* Kernel-input producers: dual nodal volume and projected nodal gradient   */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeGeometryInteriorAlgorithm::execute (src/ComputeGeometryInteriorAlgorithm.C:77-120):
+ * every element adds its scv volumes to their nodes (ipNodeMap: ip n -> node n).
+ * dual_vol [nnodes] is overwritten. */
+void or_dual_nodal_volume(int64_t nelem, const int32_t *conn, const double *coords, int64_t nnodes,
+                          double *dual_vol) {
+  memset(dual_vol, 0, sizeof(double) * nnodes);
+  for (int64_t e = 0; e < nelem; ++e) {
+    const int32_t *nd = conn + 8 * e;
+    double x[24], vol[8];
+    gather(nd, coords, 3, x);
+    or_hexscv_volume(x, vol);
+    for (int ip = 0; ip < 8; ++ip) dual_vol[nd[ip]] += vol[ip];
+  }
+}
+
+/* AssembleNodalGradElemAlgorithm (src/AssembleNodalGradElemAlgorithm.C:85-140):
+ * Green-Gauss over the dual control volume, interior (element) part: per scs,
+ * q_ip = sum_c N_c(ip) q_c (shifted: N at the shifted ip), then
+ * dqdx[left] += q_ip A / V_left, dqdx[right] -= q_ip A / V_right.
+ * dqdx [nnodes][3] is overwritten. */
+void or_nodal_grad_elem(int64_t nelem, const int32_t *conn, const double *coords, const double *q,
+                        const double *dual_vol, int shifted, int64_t nnodes, double *dqdx) {
+  double shape[96];
+  or_hex8_shape_fcn(12, or_hexscs_intgloc(shifted), shape);
+  memset(dqdx, 0, sizeof(double) * 3 * nnodes);
+  for (int64_t e = 0; e < nelem; ++e) {
+    const int32_t *nd = conn + 8 * e;
+    double x[24], areav[36];
+    gather(nd, coords, 3, x);
+    or_hex_scs_areav(x, areav);
+    for (int ip = 0; ip < 12; ++ip) {
+      const int il = kLrscv[2 * ip], ir = kLrscv[2 * ip + 1];
+      double qIp = 0.0;
+      for (int c = 0; c < 8; ++c) qIp += shape[8 * ip + c] * q[nd[c]];
+      const double invL = 1.0 / dual_vol[nd[il]], invR = 1.0 / dual_vol[nd[ir]];
+      for (int j = 0; j < 3; ++j) {
+        const double fac = qIp * areav[3 * ip + j];
+        dqdx[3 * nd[il] + j] += fac * invL;
+        dqdx[3 * nd[ir] + j] -= fac * invR;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Time-derivative (mass) kernels                                           */
 /* ------------------------------------------------------------------------ */
 

@@ -94,6 +94,13 @@ void or_assemble_momentum(int64_t nelem, const int32_t *conn, const double *coor
                           const double *u, const double *mu, const double *mdot, double includeDivU,
                           int skewSymmetric, int shiftedGradOp, const int64_t *row_ptr,
                           const int32_t *col_idx, const int32_t *node_lid, double *values, double *rhs);
+/* dual nodal volume (ComputeGeometryInteriorAlgorithm.C:77-120) and the interior
+ * Green-Gauss nodal gradient (AssembleNodalGradElemAlgorithm.C:85-140); pinned by
+ * closed forms (volumes of a lattice, exact gradients of linear fields) */
+void or_dual_nodal_volume(int64_t nelem, const int32_t *conn, const double *coords, int64_t nnodes,
+                          double *dual_vol);
+void or_nodal_grad_elem(int64_t nelem, const int32_t *conn, const double *coords, const double *q,
+                        const double *dual_vol, int shifted, int64_t nnodes, double *dqdx);
 /* mass kernels (MomentumMassElemKernel.C:95-170, ContinuityMassElemKernel.C:84-118);
  * pinned by the reference's unit-test golds (consistent and lumped) */
 void or_momentum_mass_elem(const double *coords, const double *rho3, const double *u3,

@@ -50,6 +50,8 @@ def lib():
             "or_compute_mdot": [I64, _i32, _d, _d, _d, _d, _d, D, I, I, _d],
             "or_assemble_continuity": [I64, _i32, _d, _d, _d, _d, _d, D, I, I, I, _i64, _i32, _i32, _d, _d],
             "or_assemble_momentum": [I64, _i32, _d, _d, _d, _d, D, I, I, _i64, _i32, _i32, _d, _d],
+            "or_dual_nodal_volume": [I64, _i32, _d, I64, _d],
+            "or_nodal_grad_elem": [I64, _i32, _d, _d, _d, I, I64, _d],
             "or_momentum_mass_elem": [_d, _d, _d, _d, D, _d, I, I, _d, _d],
             "or_continuity_mass_elem": [_d, _d, D, _d, I, _d],
             "or_assemble_momentum_mass": [I64, _i32, _d, _d, _d, _d, _d, _d, _d, _d, D, _d, I, I, _i64, _i32,
@@ -286,6 +288,23 @@ def assemble_momentum(g, conn, coords, u, mu, mdot, include_divu=0.0, skew=False
                                int(shifted), _p(g.row_ptr, np.int64), _p(g.col_idx, np.int32),
                                _p(g.node_lid, np.int32), _p(values), _p(rhs))
     return values, rhs
+
+
+def dual_nodal_volume(conn, coords):
+    conn = i32(conn)
+    coords = f64(coords)
+    out = np.zeros(coords.shape[0])
+    lib().or_dual_nodal_volume(conn.shape[0], _p(conn, np.int32), _p(coords), coords.shape[0], _p(out))
+    return out
+
+
+def nodal_grad_elem(conn, coords, q, dual_vol, shifted=False):
+    conn = i32(conn)
+    coords = f64(coords)
+    out = np.zeros((coords.shape[0], 3))
+    lib().or_nodal_grad_elem(conn.shape[0], _p(conn, np.int32), _p(coords), _p(f64(q)), _p(f64(dual_vol)),
+                             int(shifted), coords.shape[0], _p(out))
+    return out
 
 
 def momentum_segregated(g1, g3, conn, coords, u, mu, mdot, include_divu=0.0, skew=False, shifted=False):

@@ -403,3 +403,50 @@ def test_heat_solution_is_linear_on_uniform_mesh(oracle):
     oracle.apply_dirichlet(g, v, r, bc, np.zeros(len(coords)), bcv)
     x, it, res = oracle.gmres(g.row_ptr, g.col_idx, v, r, kspace=50, maxit=500, tol=1e-12, precond="sgs")
     np.testing.assert_allclose(x, coords[:, 0], atol=1e-10)
+
+
+# ----------------------------------------------------------------- kernel-input producers
+
+def _uniform_lattice(n, M=None):
+    from nalu_amd import lattice
+    X, conn = lattice(n, jitter=0.0)
+    if M is not None:
+        X = X @ np.asarray(M).T + np.array([0.3, -0.1, 0.2])
+    return X, conn
+
+
+def test_dual_volume_closed_form(oracle):
+    """Uniform lattice: interior dual volume h^3, halved on every boundary plane
+    the node lies on; a jittered lattice (boundary fixed) still sums to 1."""
+    n = 5
+    X, conn = _uniform_lattice(n)
+    V = oracle.dual_nodal_volume(conn, X)
+    h3 = (1.0 / n) ** 3
+    on = (np.isclose(X, 0.0) | np.isclose(X, 1.0)).sum(axis=1)
+    np.testing.assert_allclose(V, h3 / 2.0 ** on, rtol=1e-13)
+    from nalu_amd.problems import heat_problem
+    P = heat_problem(6, seed=3)
+    assert abs(oracle.dual_nodal_volume(P["conn"], P["coords"]).sum() - 1.0) <= 1e-13
+
+
+@pytest.mark.parametrize("shifted", [False, True])
+def test_nodal_gradient_exact_for_linear_fields(oracle, shifted):
+    """On an affine lattice (parallelepiped elements) the interior Green-Gauss
+    gradient of q = a.x + c is exactly a at every interior node, for the standard
+    and the shifted scs shape functions; boundary nodes lack the exposed-face
+    closure, so their sum differs."""
+    n = 5
+    M = [[1.0, 0.3, 0.0], [0.1, 0.8, 0.2], [0.0, -0.2, 1.1]]
+    X, conn = _uniform_lattice(n, M)
+    a = np.array([0.7, -1.3, 2.1])
+    q = X @ a + 0.4
+    V = oracle.dual_nodal_volume(conn, X)
+    g = oracle.nodal_grad_elem(conn, X, q, V, shifted=shifted)
+    from nalu_amd.problems import lattice_ijk
+    i, j, k = lattice_ijk(n)
+    interior = (i > 0) & (i < n) & (j > 0) & (j < n) & (k > 0) & (k < n)
+    np.testing.assert_allclose(g[interior], np.broadcast_to(a, (interior.sum(), 3)), rtol=0, atol=1e-12)
+    assert np.abs(g[~interior] - a).max() > 0.1
+    # a constant field has zero interior gradient
+    g0 = oracle.nodal_grad_elem(conn, X, np.full(len(q), 3.0), V, shifted=shifted)
+    assert np.abs(g0[interior]).max() <= 1e-12
