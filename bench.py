@@ -284,9 +284,10 @@ def mixed_block(args, na, torch, mesh, P, devc, stream):
         _, _, vB, rB = b64.csr()
         err[name + "_matrix_max_rel"] = float(np.abs(vB - vA).max() / np.abs(vA).max())
         err[name + "_rhs_max_rel"] = float(np.abs(rB - rA).max() / np.abs(rA).max())
-    # pressure solve (scalar, zero-mean); the steady momentum system without its
-    # time (mass) term is an advection-dominated operator that GMRES(50) + SGS
-    # needs thousands of iterations for, so only its assembly is compared here
+    # pressure solve (scalar, zero-mean); the coupled steady momentum system
+    # without its time (mass) term is an advection-dominated operator that
+    # GMRES(50) + SGS needs thousands of iterations for: its assembly is compared
+    # here, the transient 3-rhs form is solved in momentum_3rhs()
     sol = {}
     for prec in ("fp64", "mixed"):
         cont, _ = sys_[prec]
@@ -298,7 +299,65 @@ def mixed_block(args, na, torch, mesh, P, devc, stream):
         sol[prec] = xp - xp.mean()
     res["solutions_vs_fp64"]["pressure_rel_l2"] = float(torch.linalg.norm(sol["mixed"] - sol["fp64"]) /
                                                         torch.linalg.norm(sol["fp64"]))
+    res["momentum_3rhs"] = momentum_3rhs(args, na, torch, mesh, P, devc, stream, md)
     return res
+
+
+def momentum_3rhs(args, na, torch, mesh, P, devc, stream, md):
+    """Momentum as three right-hand sides sharing one scalar matrix: one
+    backward-Euler step (mass + advection-diffusion, dt = h so that the CFL
+    number is about 1), the three velocity components solved with one matrix and
+    one SGS set-up.  FP64 assembly + GMRES(50) against FP32-arithmetic assembly +
+    mixed FGMRES(50), both to 1e-8; errors of the mixed fields against FP64."""
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
+    rho, u, gp, mu = t(P["rho"]), t(P["u"]), t(P["gpdx"]), t(P["mu"])
+    N = P["coords"].shape[0]
+    dt = 1.0 / args.n
+    out = {"dt": dt, "rhs": 3, "errors_vs_fp64": {}}
+    sols, csr = {}, {}
+    for prec in ("fp64", "mixed"):
+        ls = na.LinearSystem(1, na.SolverConfig("fgmres" if prec == "mixed" else "gmres", "sgs", tolerance=1e-8,
+                                                max_iterations=2000, kspace=50, precision=prec))
+        ls.buildElemToNodeGraph(mesh)
+        ls.finalizeLinearSystem()
+        ls.set_stream(stream)
+        rhs3 = torch.zeros((3, N), dtype=torch.float64, device=devc)
+
+        def asm():
+            ls.zeroSystem()
+            rhs3.zero_()
+            na.assemble_momentum_segregated(ls, mesh, u, mu, md[prec], rhs3)
+            na.assemble_momentum_mass_segregated(ls, mesh, (rho, rho, rho), (u, u, u), gp, dt, rhs3)
+            ls.loadComplete()
+
+        asm()
+        out["assembly_s_" + prec] = timed(torch, asm, reps=3)
+        X3 = torch.zeros_like(rhs3)
+        info = {}
+
+        def solve():
+            X3.zero_()
+            info["r"] = ls.solve_multi(rhs3, X3)
+
+        out["solve_s_" + prec] = timed(torch, solve, reps=3)
+        st, its, rel = info["r"]
+        out["iters_" + prec] = its
+        out["true_rel_resid_" + prec] = rel
+        sols[prec] = X3.clone()
+        csr[prec] = (ls.csr(), rhs3.cpu().numpy())
+    err = out["errors_vs_fp64"]
+    (_, _, vA, _), bA = csr["fp64"]
+    (_, _, vB, _), bB = csr["mixed"]
+    err["matrix_max_rel"] = float(np.abs(vB - vA).max() / np.abs(vA).max())
+    for i, c in enumerate("uvw"):
+        err["rhs_%s_max_rel" % c] = float(np.abs(bB[i] - bA[i]).max() / np.abs(bA[i]).max())
+        err["delta_%s_rel_l2" % c] = float(torch.linalg.norm(sols["mixed"][i] - sols["fp64"][i]) /
+                                           torch.linalg.norm(sols["fp64"][i]))
+    out["speedup_mixed_vs_fp64"] = out["solve_s_fp64"] / out["solve_s_mixed"]
+    print("bench: momentum 3 rhs: fp64 %.4f s %s, mixed %.4f s %s" % (out["solve_s_fp64"], out["iters_fp64"],
+                                                                     out["solve_s_mixed"], out["iters_mixed"]),
+          file=sys.stderr, flush=True)
+    return out
 
 
 def stretched_compare(args, na, torch, net, devc):

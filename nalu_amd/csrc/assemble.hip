@@ -656,7 +656,7 @@ __global__ __launch_bounds__(256) void k_nodal_grad(const int32_t* __restrict__ 
 // full residual, so converged velocities are unchanged).  Owner-computes row
 // gather on a numDof 1 graph: the row segment accumulates in LDS, the three
 // residual components go to rhs3[i][row].
-template <bool SKEW, bool SHIFT>
+template <bool SKEW, bool SHIFT, typename R>
 __global__ __launch_bounds__(256) void k_momentum_seg_gather(
     const int32_t* __restrict__ conn, const double* __restrict__ coords, const double* __restrict__ u,
     const double* __restrict__ mu, const double* __restrict__ mdot, double incl_divu,
@@ -680,21 +680,24 @@ __global__ __launch_bounds__(256) void k_momentum_seg_gather(
       const int64_t e = t >> 3;
       int32_t nd[8];
       load_nodes(conn, e, nd);
-      double x[8][3];
-      load_vec3(coords, nd, x);
+      R x[8][3];
+      load_coords(coords, nd, x);
       const uint64_t pk = elem_pos[t];
       const double* md = mdot + 12 * e;
       const hex8::Gather gt{nd};
-      auto emit = [&](int c, const double (&blk)[3][3]) { myrow[(int)((pk >> (8 * c)) & 0xff)] += blk[0][0]; };
+      auto emit = [&](int c, const R (&blk)[3][3]) { myrow[(int)((pk >> (8 * c)) & 0xff)] += (double)blk[0][0]; };
+      R rl[3] = {R(0), R(0), R(0)};
       switch (t & 7) {
 #define NALU_SG(Lx)                                                                                           \
   case Lx: {                                                                                                  \
-    const double m3[3] = {md[hex8::adj_ip(Lx, 0)], md[hex8::adj_ip(Lx, 1)], md[hex8::adj_ip(Lx, 2)]};         \
-    hex8::row_momentum<Lx, SKEW, SHIFT, double, true>(x, gt, u, mu, m3, incl_divu, rr, emit);                 \
+    const R m3[3] = {R(md[hex8::adj_ip(Lx, 0)]), R(md[hex8::adj_ip(Lx, 1)]), R(md[hex8::adj_ip(Lx, 2)])};    \
+    hex8::row_momentum<Lx, SKEW, SHIFT, double, true>(x, gt, u, mu, m3, R(incl_divu), rl, emit);              \
   } break;
         NALU_SG(0) NALU_SG(1) NALU_SG(2) NALU_SG(3) NALU_SG(4) NALU_SG(5) NALU_SG(6) NALU_SG(7)
 #undef NALU_SG
       }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) rr[i] += (double)rl[i];
     }
   }
   __syncthreads();
@@ -1083,10 +1086,11 @@ extern "C" int nalu_assemble_momentum_segregated(nalu_linsys* ls, const nalu_mes
                          mu, mdot, include_divu, (const int32_t*)ls->row_node.p, m->inc_ptr.p, m->inc_elem.p,
                          ls->nrp.p, ls->elem_pos.p, ls->nrows, ls->values.p, rhs3, accumulate);
     };
-    if (skew && shifted) go(k_momentum_seg_gather<true, true>);
-    else if (skew) go(k_momentum_seg_gather<true, false>);
-    else if (shifted) go(k_momentum_seg_gather<false, true>);
-    else go(k_momentum_seg_gather<false, false>);
+    const bool f32 = fp32_assembly(ls);
+    if (skew && shifted) f32 ? go(k_momentum_seg_gather<true, true, float>) : go(k_momentum_seg_gather<true, true, double>);
+    else if (skew) f32 ? go(k_momentum_seg_gather<true, false, float>) : go(k_momentum_seg_gather<true, false, double>);
+    else if (shifted) f32 ? go(k_momentum_seg_gather<false, true, float>) : go(k_momentum_seg_gather<false, true, double>);
+    else f32 ? go(k_momentum_seg_gather<false, false, float>) : go(k_momentum_seg_gather<false, false, double>);
     ls->values_pending_zero = false;
     NALU_HIP_CHECK(hipGetLastError());
     return NALU_OK;

@@ -126,3 +126,30 @@ def test_mixed_pressure_solve_matches_fp64(oracle):
     d = np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"])
     print("  pressure rel diff %.2e" % d, flush=True)
     assert d <= 2e-3
+
+
+def test_fp32_segregated_momentum(oracle):
+    """FP32-arithmetic segregated momentum (3 rhs, one scalar matrix) against the
+    oracle's FP64 derivation, to the FP32 assembly bar."""
+    n = 6
+    print("case: fp32 segregated momentum n=%d" % n, flush=True)
+    P = lowmach_problem(n, seed=31, mu=0.05)
+    X, conn = P["coords"], P["conn"]
+    N = X.shape[0]
+    mesh = na.Mesh(X, conn)
+    mdot = torch.zeros((conn.shape[0], 12), dtype=torch.float64, device=DEV)
+    na.compute_mdot(mesh, dev(P["rho"]), dev(P["p"]), dev(P["u"]), dev(P["gpdx"]), mdot, tau=P["tau"])
+    ls = na.LinearSystem(1, na.SolverConfig("gmres", "sgs", precision="mixed"))
+    ls.buildElemToNodeGraph(mesh)
+    ls.finalizeLinearSystem()
+    ls.zeroSystem()
+    rhs3 = torch.zeros((3, N), dtype=torch.float64, device=DEV)
+    na.assemble_momentum_segregated(ls, mesh, dev(P["u"]), dev(P["mu"]), mdot, rhs3)
+    rp, ci, v, _ = ls.csr()
+    g1 = oracle.graph_build(N, conn, 1)
+    g3 = oracle.graph_build(N, conn, 3)
+    ov, orr = oracle.momentum_segregated(g1, g3, conn, X, P["u"], P["mu"], mdot.cpu().numpy())
+    e_mat = rel_err_rowwise(v, ov, g1.row_ptr)
+    e_rhs = np.abs(rhs3.cpu().numpy() - orr).max() / np.abs(orr).max()
+    print("  rel err: matrix %.2e rhs %.2e" % (e_mat, e_rhs), flush=True)
+    assert e_mat <= TOL32 and e_rhs <= TOL32 and e_mat > 0.0
