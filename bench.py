@@ -522,8 +522,8 @@ def main():
         "spmv": (t_spmv * (iters + 1), byt["spmv"], 1),
     }
     kern_name = {"mdot": "k_mdot", "continuity_assembly": "k_scalar_gather<ContinuityOp>",
-                 "momentum_assembly": "k_momentum_tile", "sgs_apply": "k_fwd_color + k_bwd_color",
-                 "spmv": "k_split_spmv"}
+                 "momentum_assembly": "k_momentum_tile", "sgs_apply": "k_sell_fwd + k_sell_bwd",
+                 "spmv": "k_sell_spmv"}
     dom = max(per_step, key=lambda k: per_step[k][0])
     unit_time = {"mdot": med["mdot"], "continuity_assembly": med["continuity_assembly"],
                  "momentum_assembly": med["momentum_assembly"], "sgs_apply": t_sgs, "spmv": t_spmv}[dom]
