@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-solver-compare", action="store_true",
                     help="skip the FP64 GMRES vs mixed + learned FGMRES pressure-solve comparison")
     ap.add_argument("--train-n", type=int, default=64, help="mesh of the learned preconditioner's training pairs")
-    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--train-epochs", type=int, default=10)
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-assembly / mixed-precision block")
     ap.add_argument("--learned-apply-only", type=int, default=0, metavar="K",
@@ -114,21 +114,43 @@ def solver_compare(args, na, torch, mesh, P, devc):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     # CG solved to 1e-6; pairs are kept while the residual is above 1e-3 of its
-    # start (the error is then known to 1e-3): 200 right-hand sides x 10 pairs
-    pairs = na.HarvestedPairs(lt, num_rhs=200, pairs_per_rhs=10, seed=1, tol=1e-6)
+    # start (the error is then known to 1e-3): 400 right-hand sides x 5 pairs
+    pairs = na.HarvestedPairs(lt, num_rhs=400, pairs_per_rhs=5, seed=1, tol=1e-6)
     torch.cuda.synchronize()
     t_harvest = time.perf_counter() - t0
     print("bench: harvested %d pairs at %d^3 in %.2f s" % (len(pairs), args.train_n, t_harvest),
           file=sys.stderr, flush=True)
+    # model selection on the training mesh: after every epoch, mixed FGMRES(50) +
+    # the network on the train_n^3 pressure system itself (its assembled rhs, not
+    # a training pair); the epoch with the fewest iterations is kept (the network
+    # quality swings from epoch to epoch at this learning rate)
+    lv = na.LinearSystem(1, na.SolverConfig("fgmres", "learned", tolerance=1e-8, max_iterations=600, kspace=50,
+                                            precision="mixed"))
+    lv.buildElemToNodeGraph(mt)
+    lv.finalizeLinearSystem()
+    lv.zeroSystem()
+    na.assemble_continuity(lv, mt, t(Pt["rho"]), t(Pt["p"]), t(Pt["u"]), t(Pt["gpdx"]), tau=1.0, mode="gather")
+    lv.loadComplete()
+    lv.removeRhsMean()
+    xv = torch.zeros(lv.sizes()[0], dtype=torch.float64, device=devc)
+
+    def validate(nn):
+        lv.set_learned_preconditioner(nn)
+        xv.zero_()
+        lv.solve(xv)
+        return lv.linearSolveIterations() + (10000 if lv.learned_fallback() else 0)
+
     net = na.LearnedPreconditioner(num_layers=4, hidden=32, seed=2)
     t0 = time.perf_counter()
-    hist = net.train(lt, pairs, epochs=args.train_epochs, batch=16, lr=3e-3, seed=1)
+    hist = net.train(lt, pairs, epochs=args.train_epochs, batch=16, lr=3e-3, seed=1, validate=validate)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    print("bench: validation iterations per epoch %s, kept epoch %d" % (net.validation, net.best_epoch),
+          file=sys.stderr, flush=True)
     print("bench: trained %d epochs in %.2f s, losses %s" % (args.train_epochs, t_train, np.round(hist, 4).tolist()),
           file=sys.stderr, flush=True)
     cg_iters = pairs.cg_iterations()
-    del pairs, lt, mt
+    del pairs, lv, lt, mt
 
     ev = lambda: torch.cuda.Event(enable_timing=True)
     res = {}
@@ -179,7 +201,9 @@ def solver_compare(args, na, torch, mesh, P, devc):
         res[name]["pressure_rel_diff_vs_fp64"] = float(torch.linalg.norm(sols[name] - ref) / torch.linalg.norm(ref))
     res["speedup_mixed_learned_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_learned"]["solve_s"]
     res["speedup_mixed_sgs_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_sgs"]["solve_s"]
-    res["learned_training"] = {"train_mesh": "%d^3" % args.train_n, "pairs": 2000, "rhs": 200, "cg_tol": 1e-6,
+    res["learned_training"] = {"train_mesh": "%d^3" % args.train_n, "pairs": 2000, "rhs": 400, "pairs_per_rhs": 5,
+                               "cg_tol": 1e-6,
+                               "validation_fgmres_iters_per_epoch": net.validation, "kept_epoch": net.best_epoch,
                                "harvest_s": t_harvest,
                                "harvest_cg_iters_mean": float(np.mean(cg_iters)), "train_s": t_train,
                                "epochs": args.train_epochs, "batch": 16, "epoch_loss": hist.tolist(),

@@ -311,6 +311,12 @@ int nalu_gnn_data_pair(const nalu_gnn_data *d, int idx, float *r_dev, float *e_d
  * same system; epoch_loss_host[epochs] receives the mean loss per epoch. */
 int nalu_gnn_train(nalu_gnn *g, nalu_linsys *ls, const nalu_gnn_data *d, int epochs, int batch, double lr,
                    uint64_t seed, double *epoch_loss_host);
+/* Epochs [first_epoch, first_epoch + num_epochs) of a run of total_epochs (the
+ * schedule and pair order of the whole run), so that a caller can validate the
+ * network between epochs; epoch_loss_host[num_epochs]. */
+int nalu_gnn_train_epochs(nalu_gnn *g, nalu_linsys *ls, const nalu_gnn_data *d, int first_epoch,
+                          int num_epochs, int total_epochs, int batch, double lr, uint64_t seed,
+                          double *epoch_loss_host);
 
 #ifdef __cplusplus
 }

@@ -115,6 +115,8 @@ def lib():
         L.nalu_assemble_momentum_mass_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
                                                              _dp, C.c_int, C.c_int, _vp]
         L.nalu_linsys_solve_multi.argtypes = [_vp, C.c_int, _vp, _vp, _i32p, _dp]
+        L.nalu_gnn_train_epochs.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                            C.c_uint64, _dp]
         L.nalu_compute_dual_volume.argtypes = [_vp, _vp, _vp]
         L.nalu_compute_nodal_gradient.argtypes = [_vp, _vp, _vp, C.c_int, _vp, _vp]
         # learned preconditioner
@@ -514,11 +516,30 @@ class LearnedPreconditioner:
                                         C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))), "gnn_loss_grad")
         return loss.value, g
 
-    def train(self, ls, pairs, epochs=4, batch=16, lr=3e-3, seed=0):
-        """Adam on the harvested pairs; returns the mean loss of each epoch."""
+    def train(self, ls, pairs, epochs=4, batch=16, lr=3e-3, seed=0, validate=None):
+        """Adam on the harvested pairs; returns the mean loss of each epoch.
+
+        validate: optional callable(net) -> score (lower is better), evaluated
+        after every epoch; the parameters of the best-scoring epoch are kept and
+        the scores are available as self.validation."""
         hist = np.zeros(int(epochs), dtype=np.float64)
-        _check(lib().nalu_gnn_train(self._h, ls.handle, pairs.handle, int(epochs), int(batch), float(lr), int(seed),
-                                    hist.ctypes.data_as(_dp)), "gnn_train")
+        if validate is None:
+            _check(lib().nalu_gnn_train(self._h, ls.handle, pairs.handle, int(epochs), int(batch), float(lr),
+                                        int(seed), hist.ctypes.data_as(_dp)), "gnn_train")
+            return hist
+        self.validation = []
+        best, best_th = None, None
+        for ep in range(int(epochs)):
+            h = np.zeros(1, dtype=np.float64)
+            _check(lib().nalu_gnn_train_epochs(self._h, ls.handle, pairs.handle, ep, 1, int(epochs), int(batch),
+                                               float(lr), int(seed), h.ctypes.data_as(_dp)), "gnn_train")
+            hist[ep] = h[0]
+            score = float(validate(self))
+            self.validation.append(score)
+            if best is None or score < best:
+                best, best_th = score, self.get_params()
+        self.set_params(best_th)
+        self.best_epoch = int(np.argmin(self.validation))
         return hist
 
     def __del__(self):

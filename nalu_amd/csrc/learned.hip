@@ -1847,13 +1847,19 @@ extern "C" int nalu_gnn_data_pair(const nalu_gnn_data* d, int idx, float* r_dev,
 // Adam training on the harvested pairs of the same system; epoch_loss_host[e]
 // receives the mean batch loss of epoch e.  The learning rate decays from lr to
 // lr/10 on a cosine over all steps.
-extern "C" int nalu_gnn_train(nalu_gnn* g, nalu_linsys* ls, const nalu_gnn_data* d, int epochs, int batch,
-                              double lr, uint64_t seed, double* epoch_loss_host) {
+// Epochs [first, first + count) of a run of total_epochs: the learning-rate
+// schedule spans the whole run and the pair order of epoch k is the k-th shuffle
+// of one generator seeded with seed, so that a run split into ranges (with
+// validation in between) follows the same sequence as one call.
+extern "C" int nalu_gnn_train_epochs(nalu_gnn* g, nalu_linsys* ls, const nalu_gnn_data* d, int first_epoch,
+                                     int num_epochs, int total_epochs, int batch, double lr, uint64_t seed,
+                                     double* epoch_loss_host) {
   return guarded([&]() -> int {
     using namespace nalu;
     check_net(g);
-    NALU_REQUIRE(ls && d && epochs >= 1 && batch >= 1 && batch <= 64 && lr > 0, NALU_ERR_ARG,
-                 "gnn_train: bad arguments");
+    NALU_REQUIRE(ls && d && num_epochs >= 1 && first_epoch >= 0 && first_epoch + num_epochs <= total_epochs &&
+                     batch >= 1 && batch <= 64 && lr > 0,
+                 NALU_ERR_ARG, "gnn_train: bad arguments");
     NALU_REQUIRE(d->owner == ls && d->n == ls->nrows, NALU_ERR_ARG, "gnn_train: data harvested from another system");
     NALU_REQUIRE(ls->precond_ready, NALU_ERR_STATE, "gnn_train: set up the preconditioner first");
     ensure_split32(ls);
@@ -1863,7 +1869,7 @@ extern "C" int nalu_gnn_train(nalu_gnn* g, nalu_linsys* ls, const nalu_gnn_data*
     const int64_t n = ls->nrows;
     const int B = std::min(batch, d->npairs);
     const int spe = d->npairs / B;
-    const int64_t total_steps = (int64_t)epochs * spe;
+    const int64_t total_steps = (int64_t)total_epochs * spe;
     const int64_t np = (int64_t)g->theta.n;
     Work w;
     w.alloc(n, B, g->L);
@@ -1873,8 +1879,9 @@ extern "C" int nalu_gnn_train(nalu_gnn* g, nalu_linsys* ls, const nalu_gnn_data*
     std::iota(order.begin(), order.end(), 0);
     std::mt19937_64 rng(seed);
     std::vector<double> lh(spe);
-    int64_t step = 0;
-    for (int ep = 0; ep < epochs; ++ep) {
+    for (int ep = 0; ep < first_epoch; ++ep) std::shuffle(order.begin(), order.end(), rng);
+    int64_t step = (int64_t)first_epoch * spe;
+    for (int ep = 0; ep < num_epochs; ++ep) {
       std::shuffle(order.begin(), order.end(), rng);
       NALU_HIP_CHECK(hipMemcpyAsync(idx.p, order.data(), sizeof(int32_t) * d->npairs, hipMemcpyHostToDevice, s));
       NALU_HIP_CHECK(hipStreamSynchronize(s));
@@ -1902,4 +1909,9 @@ extern "C" int nalu_gnn_train(nalu_gnn* g, nalu_linsys* ls, const nalu_gnn_data*
     }
     return NALU_OK;
   });
+}
+
+extern "C" int nalu_gnn_train(nalu_gnn* g, nalu_linsys* ls, const nalu_gnn_data* d, int epochs, int batch,
+                              double lr, uint64_t seed, double* epoch_loss_host) {
+  return nalu_gnn_train_epochs(g, ls, d, 0, epochs, epochs, batch, lr, seed, epoch_loss_host);
 }
