@@ -222,3 +222,40 @@ def test_nonfinite_network_falls_back_to_sgs():
     st, it, rr = ls.solve_rhs(b, x)
     print("  status %d iters %d resid %.2e fallback %s" % (st, it, rr, ls.learned_fallback()), flush=True)
     assert st == 0 and rr <= 1e-8 and ls.learned_fallback()
+
+
+def test_epoch_ranges_and_validation_selection():
+    """Training split into single epochs (with a validation call in between)
+    follows the one-call run exactly; validation keeps the best epoch."""
+    print("case: epoch-range training and validation selection, n=8", flush=True)
+    cfg = na.SolverConfig("fgmres", "learned", tolerance=1e-8, max_iterations=400, kspace=30, precision="mixed")
+    P, mesh, ls = poisson_system(8, cfg)
+    data = na.HarvestedPairs(ls, num_rhs=8, pairs_per_rhs=4, seed=3, tol=1e-8)
+    a = na.LearnedPreconditioner(seed=5)
+    ha = a.train(ls, data, epochs=3, batch=8, lr=3e-3, seed=2)
+    b = na.LearnedPreconditioner(seed=5)
+    calls = []
+
+    def last_is_best(net):
+        calls.append(1)
+        return -len(calls)
+
+    hb = b.train(ls, data, epochs=3, batch=8, lr=3e-3, seed=2, validate=last_is_best)
+    np.testing.assert_array_equal(ha, hb)
+    np.testing.assert_array_equal(a.get_params(), b.get_params())
+    assert b.best_epoch == 2 and len(b.validation) == 3
+    # real validation: FGMRES iterations on the system itself; the kept network scores the minimum
+    n = ls.sizes()[0]
+    bb = dev(poisson_rhs(n, 17))
+    x = torch.zeros(n, dtype=torch.float64, device=DEV)
+
+    def iters(net):
+        ls.set_learned_preconditioner(net)
+        x.zero_()
+        st, it, rr = ls.solve_rhs(bb, x)
+        return it + (10000 if ls.learned_fallback() else 0)
+
+    c = na.LearnedPreconditioner(seed=6)
+    c.train(ls, data, epochs=4, batch=8, lr=3e-3, seed=4, validate=iters)
+    print("  validation", c.validation, "kept", c.best_epoch, flush=True)
+    assert iters(c) == min(c.validation)
