@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--learned-apply-only", type=int, default=0, metavar="K",
                     help="only time K applications of an untrained learned preconditioner on the n^3 "
                          "continuity system (profiling aid), print one JSON line and exit")
+    ap.add_argument("--stretched-only", action="store_true",
+                    help="only the stretched-mesh pressure comparison (with --stretched-n), as one JSON line")
     ap.add_argument("--stretched-n", type=int, default=0,
                     help="run the stretched-mesh pressure comparison on an n^3 mesh (256: the stretched "
                          "configuration, about 80 s of solves; 0 skips it)")
@@ -93,16 +95,17 @@ def cpu_baseline(n_cpu):
             "pressure_solve_s": t_solve, "pressure_iters": it}
 
 
-def solver_compare(args, na, torch, mesh, P, devc):
-    """Pressure-Poisson solve to 1e-8 FP64 true relative residual on the same
-    assembled continuity system: plain FP64 GMRES(50) + SGS against mixed
-    precision FGMRES(50) with the learned preconditioner (trained on pairs
-    harvested from FP64 CG + SGS on a train_n^3 mesh with lognormal(0, 0.3)
-    density), plus the mixed FGMRES + SGS ablation."""
-    from nalu_amd.problems import lowmach_problem
+def train_network(args, na, torch, devc, Pt, tag=""):
+    """Harvest 2000 (residual, error) pairs from plain FP64 CG + SGS on the
+    pressure system of the training problem Pt (400 seeded Gaussian zero-mean
+    right-hand sides x 5 pairs, CG to 1e-6: pairs are kept while the residual is
+    above 1e-3 of its start, so the error is known to 1e-3), train the network
+    and keep the epoch whose mixed FGMRES(50) solve of the training mesh's own
+    pressure system (its assembled rhs, not a training pair) takes the fewest
+    iterations -- the network quality swings from epoch to epoch at this
+    learning rate.  Returns (net, training record)."""
     t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
-    # training system and pairs
-    Pt = lowmach_problem(args.train_n, rho_sigma=0.3, seed=4321)
+    n_t = round(Pt["conn"].shape[0] ** (1.0 / 3.0))
     mt = na.Mesh(Pt["coords"], Pt["conn"])
     lt = na.LinearSystem(1, na.SolverConfig("cg", "sgs"))
     lt.buildElemToNodeGraph(mt)
@@ -113,17 +116,11 @@ def solver_compare(args, na, torch, mesh, P, devc):
     lt.precond_setup()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    # CG solved to 1e-6; pairs are kept while the residual is above 1e-3 of its
-    # start (the error is then known to 1e-3): 400 right-hand sides x 5 pairs
     pairs = na.HarvestedPairs(lt, num_rhs=400, pairs_per_rhs=5, seed=1, tol=1e-6)
     torch.cuda.synchronize()
     t_harvest = time.perf_counter() - t0
-    print("bench: harvested %d pairs at %d^3 in %.2f s" % (len(pairs), args.train_n, t_harvest),
+    print("bench: %sharvested %d pairs at %d^3 in %.2f s" % (tag, len(pairs), n_t, t_harvest),
           file=sys.stderr, flush=True)
-    # model selection on the training mesh: after every epoch, mixed FGMRES(50) +
-    # the network on the train_n^3 pressure system itself (its assembled rhs, not
-    # a training pair); the epoch with the fewest iterations is kept (the network
-    # quality swings from epoch to epoch at this learning rate)
     lv = na.LinearSystem(1, na.SolverConfig("fgmres", "learned", tolerance=1e-8, max_iterations=600, kspace=50,
                                             precision="mixed"))
     lv.buildElemToNodeGraph(mt)
@@ -145,12 +142,29 @@ def solver_compare(args, na, torch, mesh, P, devc):
     hist = net.train(lt, pairs, epochs=args.train_epochs, batch=16, lr=3e-3, seed=1, validate=validate)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    print("bench: validation iterations per epoch %s, kept epoch %d" % (net.validation, net.best_epoch),
+    print("bench: %svalidation iterations per epoch %s, kept epoch %d" % (tag, net.validation, net.best_epoch),
           file=sys.stderr, flush=True)
-    print("bench: trained %d epochs in %.2f s, losses %s" % (args.train_epochs, t_train, np.round(hist, 4).tolist()),
+    print("bench: %strained %d epochs in %.2f s, losses %s" % (tag, args.train_epochs, t_train,
+                                                              np.round(hist, 4).tolist()),
           file=sys.stderr, flush=True)
-    cg_iters = pairs.cg_iterations()
+    rec = {"train_mesh": "%d^3" % n_t, "pairs": len(pairs), "rhs": 400, "pairs_per_rhs": 5, "cg_tol": 1e-6,
+           "validation_fgmres_iters_per_epoch": net.validation, "kept_epoch": net.best_epoch,
+           "harvest_s": t_harvest, "harvest_cg_iters_mean": float(np.mean(pairs.cg_iterations())),
+           "train_s": t_train, "epochs": args.train_epochs, "batch": 16, "epoch_loss": hist.tolist(),
+           "layers": 4, "hidden": 32, "weights": "bf16", "accumulate": "fp32"}
     del pairs, lv, lt, mt
+    return net, rec
+
+
+def solver_compare(args, na, torch, mesh, P, devc):
+    """Pressure-Poisson solve to 1e-8 FP64 true relative residual on the same
+    assembled continuity system: plain FP64 GMRES(50) + SGS against mixed
+    precision FGMRES(50) with the learned preconditioner (trained on pairs
+    harvested from FP64 CG + SGS on a train_n^3 mesh with lognormal(0, 0.3)
+    density), plus the mixed FGMRES + SGS ablation."""
+    from nalu_amd.problems import lowmach_problem
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
+    net, train_rec = train_network(args, na, torch, devc, lowmach_problem(args.train_n, rho_sigma=0.3, seed=4321))
 
     ev = lambda: torch.cuda.Event(enable_timing=True)
     res = {}
@@ -201,13 +215,7 @@ def solver_compare(args, na, torch, mesh, P, devc):
         res[name]["pressure_rel_diff_vs_fp64"] = float(torch.linalg.norm(sols[name] - ref) / torch.linalg.norm(ref))
     res["speedup_mixed_learned_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_learned"]["solve_s"]
     res["speedup_mixed_sgs_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_sgs"]["solve_s"]
-    res["learned_training"] = {"train_mesh": "%d^3" % args.train_n, "pairs": 2000, "rhs": 400, "pairs_per_rhs": 5,
-                               "cg_tol": 1e-6,
-                               "validation_fgmres_iters_per_epoch": net.validation, "kept_epoch": net.best_epoch,
-                               "harvest_s": t_harvest,
-                               "harvest_cg_iters_mean": float(np.mean(cg_iters)), "train_s": t_train,
-                               "epochs": args.train_epochs, "batch": 16, "epoch_loss": hist.tolist(),
-                               "layers": 4, "hidden": 32, "weights": "bf16", "accumulate": "fp32"}
+    res["learned_training"] = train_rec
     return res, net
 
 
@@ -384,7 +392,7 @@ def momentum_3rhs(args, na, torch, mesh, P, devc, stream, md):
     return out
 
 
-def stretched_compare(args, na, torch, net, devc):
+def stretched_compare(args, na, torch, devc):
     """Stretched-mesh pressure-Poisson: geometric ratio 1.05 from all faces
     (aspect ratio capped at 100), jitter 0.05h, density lognormal(0, 0.3).
     Plain FP64 GMRES(50) + SGS against mixed FGMRES(50) + learned preconditioner
@@ -392,13 +400,21 @@ def stretched_compare(args, na, torch, net, devc):
     from nalu_amd.problems import lowmach_problem
     n = args.stretched_n
     t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
+    # the network for stretched cells is trained on a stretched train_n^3 mesh whose
+    # per-cell ratio reaches the same growth over the same fraction of the box
+    ratio_t = 1.05 ** (n / args.train_n)
+    net, rec = train_network(args, na, torch, devc,
+                             lowmach_problem(args.train_n, jitter=0.05, rho_sigma=0.3, seed=778, stretch=ratio_t,
+                                             max_aspect=100.0), tag="stretched ")
+    rec["stretch_ratio"] = ratio_t
     P = lowmach_problem(n, jitter=0.05, rho_sigma=0.3, seed=777, stretch=1.05, max_aspect=100.0)
     hx = np.diff(P["coords"][:n + 1, 0])  # the boundary line j = k = 0 carries the unjittered axis positions
     mesh = na.Mesh(P["coords"], P["conn"])
     out = {"mesh": "%d^3" % n, "elements": int(P["conn"].shape[0]), "stretch_ratio": 1.05,
-           "max_to_min_spacing": float(hx.max() / hx.min())}
+           "max_to_min_spacing": float(hx.max() / hx.min()), "learned_training": rec}
     for name, method, prec, precision in (("fp64_gmres_sgs", "gmres", "sgs", "fp64"),
-                                          ("mixed_fgmres_learned", "fgmres", "learned", "mixed")):
+                                          ("mixed_fgmres_learned", "fgmres", "learned", "mixed"),
+                                          ("mixed_fgmres_sgs", "fgmres", "sgs", "mixed")):
         ls = na.LinearSystem(1, na.SolverConfig(method, prec, tolerance=1e-8, max_iterations=20000, kspace=50,
                                                 precision=precision))
         ls.buildElemToNodeGraph(mesh)
@@ -421,6 +437,7 @@ def stretched_compare(args, na, torch, net, devc):
         del ls, x
         torch.cuda.empty_cache()
     out["speedup_mixed_learned_vs_fp64"] = out["fp64_gmres_sgs"]["solve_s"] / out["mixed_fgmres_learned"]["solve_s"]
+    out["speedup_mixed_sgs_vs_fp64"] = out["fp64_gmres_sgs"]["solve_s"] / out["mixed_fgmres_sgs"]["solve_s"]
     return out
 
 
@@ -440,6 +457,9 @@ def main():
     n = args.n
     if args.learned_apply_only > 0:
         print(json.dumps(learned_apply_timing(args, na, torch, devc)))
+        return
+    if args.stretched_only:
+        print(json.dumps(stretched_compare(args, na, torch, devc)))
         return
     print("bench: generating %d^3 low-Mach problem" % n, file=sys.stderr, flush=True)
     P = lowmach_problem(n, seed=4321)
@@ -569,11 +589,11 @@ def main():
         mixed = mixed_block(args, na, torch, mesh, P, devc, stream)
 
     stretched = None
-    if args.stretched_n > 0 and net is not None:
+    if args.stretched_n > 0:
         del cont, mom
         torch.cuda.empty_cache()
         print("bench: stretched %d^3 pressure comparison" % args.stretched_n, file=sys.stderr, flush=True)
-        stretched = stretched_compare(args, na, torch, net, devc)
+        stretched = stretched_compare(args, na, torch, devc)
 
     cpu = None
     if not args.no_cpu_baseline:
