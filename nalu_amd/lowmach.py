@@ -1,0 +1,124 @@
+"""Low-Mach time step driver on the HIP kernels (host orchestration only).
+
+Mirrors LowMachEquationSystem::solve_and_update (src/LowMachEquationSystem.C:779-870)
+for one backward-Euler step (gammas 1, -1, 0) with the momentum system in its
+three-right-hand-side form:
+
+  for each Picard iteration:
+    momentum:   mass + advection-diffusion with the current mdot, Dirichlet
+                velocity on the wall nodes, solve for du (3 rhs, one matrix), u += du
+    continuity: advection (projTimeScale = dt / g1) + density time derivative,
+                solve for dp (pure Neumann: zero-mean rhs), p += dp
+    mdot:       recomputed from the new u and p with the previous Gjp
+    projection: Gjp_new = nodal gradient of p;
+                u -= (dt / g1) / rho (Gjp_new - Gjp_old) on the non-wall nodes
+                (LowMachEquationSystem::project_nodal_velocity, :884-975)
+
+The nodal gradient has no exposed-face closure here, which the projection
+tolerates on the interior nodes (wall nodes keep their Dirichlet values).
+Meshes are used without explicit node ids (row = node).
+"""
+import numpy as np
+
+from . import (LinearSystem, NaluError, SolverConfig, assemble_continuity, assemble_continuity_mass,
+               assemble_momentum_mass_segregated, assemble_momentum_segregated, compute_dual_volume, compute_mdot,
+               compute_nodal_gradient)
+
+
+class LowMachStep:
+    """State (torch device tensors): u [N][3], p [N], rho [N] (constant in time),
+    mu [N], Gjp [N][3]; wall_nodes int32 device tensor with wall_u [len][3]."""
+
+    def __init__(self, mesh, u, p, rho, mu, dt, wall_nodes, wall_u, picard=2, tol=1e-8, precision="fp64",
+                 stream=None):
+        import torch
+        self.torch = torch
+        self.mesh, self.dt, self.picard, self.stream = mesh, float(dt), int(picard), stream
+        self.g = (1.0, -1.0, 0.0)
+        self.u, self.p, self.rho, self.mu = u.clone(), p.clone(), rho, mu
+        self.wall = wall_nodes.long()
+        self.wall_i32 = wall_nodes
+        self.wall_u = wall_u
+        self.u[self.wall] = wall_u
+        n_nodes = u.shape[0]
+        dev = u.device
+        self.vol = torch.zeros(n_nodes, dtype=torch.float64, device=dev)
+        compute_dual_volume(mesh, self.vol, stream)
+        self.Gjp = torch.zeros((n_nodes, 3), dtype=torch.float64, device=dev)
+        compute_nodal_gradient(mesh, self.p, self.vol, self.Gjp, stream=stream)
+        self.mdot = torch.zeros((mesh.num_elems, 12), dtype=torch.float64, device=dev)
+        self.interior = torch.ones(n_nodes, dtype=torch.bool, device=dev)
+        self.interior[self.wall] = False
+        method = "fgmres" if precision == "mixed" else "gmres"
+        self.mom = LinearSystem(1, SolverConfig(method, "sgs", tolerance=tol, max_iterations=2000, kspace=50,
+                                                precision=precision))
+        self.cont = LinearSystem(1, SolverConfig(method, "sgs", tolerance=tol, max_iterations=5000, kspace=50,
+                                                 precision=precision))
+        for ls in (self.mom, self.cont):
+            ls.buildElemToNodeGraph(mesh)
+            ls.finalizeLinearSystem()
+            if stream is not None:
+                ls.set_stream(stream)
+        if self.mom.sizes()[0] != n_nodes:
+            raise NaluError("LowMachStep: one row per node expected")
+        self.rhs3 = torch.zeros((3, n_nodes), dtype=torch.float64, device=dev)
+        self.du3 = torch.zeros_like(self.rhs3)
+        self.dp = torch.zeros_like(self.p)
+        self.log = []
+
+    def _mdot(self):
+        compute_mdot(self.mesh, self.rho, self.p, self.u, self.Gjp, self.mdot, tau=self.dt / self.g[0],
+                     stream=self.stream)
+
+    def continuity_residual(self):
+        """|rhs| of the continuity system (mass imbalance incl. the time term) at the current state."""
+        self._mdot()
+        self.cont.zeroSystem()
+        assemble_continuity(self.cont, self.mesh, self.rho, self.p, self.u, self.Gjp, tau=self.dt / self.g[0],
+                            mode="gather")
+        assemble_continuity_mass(self.cont, self.mesh, (self.rho, self.rho, self.rho), self.dt, self.g)
+        self.cont.loadComplete()
+        return float(np.linalg.norm(self.cont.csr()[3]))
+
+    def step(self):
+        """One time step; returns the per-iteration record."""
+        torch = self.torch
+        uN = self.u.clone()
+        rec = []
+        self._mdot()
+        for k in range(self.picard):
+            # momentum: three rhs, one scalar matrix
+            self.mom.zeroSystem()
+            self.rhs3.zero_()
+            assemble_momentum_segregated(self.mom, self.mesh, self.u, self.mu, self.mdot, self.rhs3)
+            assemble_momentum_mass_segregated(self.mom, self.mesh, (self.rho, self.rho, self.rho), (uN, uN, self.u),
+                                              self.Gjp, self.dt, self.rhs3, gamma=self.g)
+            self.mom.applyDirichletBCs(self.wall_i32, self.u[:, 0].contiguous(), self.wall_u[:, 0].contiguous())
+            self.rhs3[:, self.wall] = (self.wall_u - self.u[self.wall]).T
+            self.mom.loadComplete()
+            self.du3.zero_()
+            st_m, its_m, rel_m = self.mom.solve_multi(self.rhs3, self.du3)
+            self.u += self.du3.T
+            # continuity
+            self.cont.zeroSystem()
+            assemble_continuity(self.cont, self.mesh, self.rho, self.p, self.u, self.Gjp, tau=self.dt / self.g[0],
+                                mode="gather")
+            assemble_continuity_mass(self.cont, self.mesh, (self.rho, self.rho, self.rho), self.dt, self.g)
+            self.cont.loadComplete()
+            self.cont.removeRhsMean()
+            self.dp.zero_()
+            st_c = self.cont.solve(self.dp)
+            its_c = self.cont.linearSolveIterations()
+            self.p += self.dp
+            self.p -= self.p.mean()
+            # mdot with the new velocity and pressure, then the nodal-velocity projection
+            self._mdot()
+            g_old = self.Gjp.clone()
+            compute_nodal_gradient(self.mesh, self.p, self.vol, self.Gjp, stream=self.stream)
+            corr = (self.dt / self.g[0]) / self.rho[:, None] * (self.Gjp - g_old)
+            self.u[self.interior] -= corr[self.interior]
+            rec.append({"momentum_status": st_m, "momentum_iters": its_m, "momentum_rel": rel_m,
+                        "continuity_status": st_c, "continuity_iters": its_c,
+                        "du_max": float(self.du3.abs().max()), "dp_max": float(self.dp.abs().max())})
+        self.log.append(rec)
+        return rec

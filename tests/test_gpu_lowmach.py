@@ -1,0 +1,47 @@
+"""GPU integration test of the low-Mach time-step driver (momentum in its
+three-right-hand-side form, continuity, mdot, nodal-velocity projection) built
+on the parity-tested kernels: every linear solve reaches its tolerance and the
+projection lowers the continuity residual (the discrete mass imbalance)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import nalu_amd as na  # noqa: E402
+from nalu_amd.lowmach import LowMachStep  # noqa: E402
+from nalu_amd.problems import lattice_ijk, lowmach_problem  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=DEV)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lowmach_step(precision):
+    n = 8
+    print("case: low-Mach step n=%d precision=%s" % (n, precision), flush=True)
+    P = lowmach_problem(n, seed=11, mu=0.01, noise=0.05)
+    mesh = na.Mesh(P["coords"], P["conn"])
+    i, j, k = lattice_ijk(n)
+    wall = np.nonzero((i == 0) | (i == n) | (j == 0) | (j == n) | (k == 0) | (k == n))[0].astype(np.int32)
+    u0 = dev(P["u"])
+    wall_t = torch.as_tensor(wall, device=DEV)
+    step = LowMachStep(mesh, u0, dev(P["p"]), dev(P["rho"]), dev(P["mu"]), dt=0.25 / n, wall_nodes=wall_t,
+                       wall_u=u0[wall_t.long()].clone(), picard=3, precision=precision)
+    r0 = step.continuity_residual()
+    rec = step.step()
+    r1 = step.continuity_residual()
+    for it in rec:
+        print("  ", {k: (v if not isinstance(v, list) else v) for k, v in it.items()}, flush=True)
+        assert it["momentum_status"] == 0 and it["continuity_status"] == 0
+        assert max(it["momentum_rel"]) <= 1e-8
+    print("  continuity residual %.3e -> %.3e" % (r0, r1), flush=True)
+    assert np.isfinite(r1) and r1 < 0.5 * r0
+    # Picard corrections shrink
+    assert rec[-1]["du_max"] < rec[0]["du_max"]
+    # wall velocities stay at their Dirichlet values
+    np.testing.assert_allclose(step.u[wall_t.long()].cpu().numpy(), P["u"][wall], rtol=0, atol=1e-12)
