@@ -565,7 +565,7 @@ def main():
         "sgs_apply": (t_sgs * (iters + 1), byt["sgs_apply"], 2 * ncol - 1),
         "spmv": (t_spmv * (iters + 1), byt["spmv"], 1),
     }
-    kern_name = {"mdot": "k_mdot", "continuity_assembly": "k_scalar_gather<ContinuityOp>",
+    kern_name = {"mdot": "k_mdot_elem", "continuity_assembly": "k_scalar_gather<ContinuityOp>",
                  "momentum_assembly": "k_momentum_tile", "sgs_apply": "k_sell_fwd + k_sell_bwd",
                  "spmv": "k_sell_spmv"}
     dom = max(per_step, key=lambda k: per_step[k][0])

@@ -195,7 +195,7 @@ int nalu_linsys_solve_multi(nalu_linsys *ls, int k, const double *B_dev, double 
 int nalu_assemble_scalar_diff(nalu_linsys *ls, const nalu_mesh *m, const double *q_dev,
                               const double *diff_coeff_dev, int shifted_grad_op, int mode);
 /* ComputeMdotElemAlgorithm::execute (src/ComputeMdotElemAlgorithm.C:85-250):
- * mdot_dev [E][12]. */
+ * mdot_dev [E][12], 16-byte aligned (NALU_ERR_ARG otherwise). */
 int nalu_compute_mdot(const nalu_mesh *m, const double *rho_dev, const double *p_dev,
                       const double *u_dev, const double *gpdx_dev, double proj_time_scale,
                       int shift_mdot, int shift_poisson, double *mdot_dev, void *stream);

@@ -284,22 +284,22 @@ __device__ __forceinline__ R row_scalar_diff(const R (&x)[8][3], const Gather& g
 }
 
 // ComputeMdotElemAlgorithm for one scs (src/ComputeMdotElemAlgorithm.C:204-232):
-// mdot = (rho u)_ip . A - tau (grad p_ip - Gp_ip) . A, fields streamed per node.
+// mdot = (rho u)_ip . A - tau (grad p_ip - Gp_ip) . A, from an element's nodal
+// fields held in registers.
 template <int IP, bool SHIFT_MDOT, bool SHIFT_POISSON, typename JR, typename R>
-__device__ __forceinline__ R ip_mdot_stream(const IpGeo<JR, R>& G, const Gather& gt, const double* __restrict__ rho,
-                                            const double* __restrict__ p, const double* __restrict__ u,
-                                            const double* __restrict__ gp, R tau) {
+__device__ __forceinline__ R ip_mdot_regs(const IpGeo<JR, R>& G, const double (&rho)[8], const double (&p)[8],
+                                          const double (&u)[8][3], const double (&gp)[8][3], R tau) {
   R ru[3] = {R(0), R(0), R(0)}, gpi[3] = {R(0), R(0), R(0)}, dpa = R(0);
   static_for<8>([&](auto cc) {
     constexpr int c = decltype(cc)::value;
     const R r = R(shape(IP, c, SHIFT_MDOT));
-    const R rr = r * R(gt.s(rho, c));
+    const R rr = r * R(rho[c]);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      ru[j] += rr * R(gt.v(u, c, j));
-      gpi[j] += r * R(gt.v(gp, c, j));
+      ru[j] += rr * R(u[c][j]);
+      gpi[j] += r * R(gp[c][j]);
     }
-    dpa += R(gt.s(p, c)) * dsdot<IP, c, SHIFT_POISSON>(G.g);
+    dpa += R(p[c]) * dsdot<IP, c, SHIFT_POISSON>(G.g);
   });
   const R rua = ru[0] * G.A[0] + ru[1] * G.A[1] + ru[2] * G.A[2];
   const R gpa = gpi[0] * G.A[0] + gpi[1] * G.A[1] + gpi[2] * G.A[2];
