@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--train-n", type=int, default=64, help="mesh of the learned preconditioner's training pairs")
     ap.add_argument("--train-epochs", type=int, default=10)
     ap.add_argument("--solve-reps", type=int, default=3)
+    ap.add_argument("--muelu-degree", type=int, default=2, help="Chebyshev degree of the AMG smoother")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-assembly / mixed-precision block")
     ap.add_argument("--learned-apply-only", type=int, default=0, metavar="K",
                     help="only time K applications of an untrained learned preconditioner on the n^3 "
@@ -161,7 +162,9 @@ def solver_compare(args, na, torch, mesh, P, devc):
     assembled continuity system: plain FP64 GMRES(50) + SGS against mixed
     precision FGMRES(50) with the learned preconditioner (trained on pairs
     harvested from FP64 CG + SGS on a train_n^3 mesh with lognormal(0, 0.3)
-    density), plus the mixed FGMRES + SGS ablation."""
+    density), plus the mixed FGMRES + SGS ablation and the smoothed-aggregation
+    AMG preconditioner ("muelu": FP64 CG and mixed FGMRES; solve_s includes the
+    hierarchy set-up, reported separately as amg.setup_s)."""
     from nalu_amd.problems import lowmach_problem
     t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
     net, train_rec = train_network(args, na, torch, devc, lowmach_problem(args.train_n, rho_sigma=0.3, seed=4321))
@@ -169,11 +172,16 @@ def solver_compare(args, na, torch, mesh, P, devc):
     ev = lambda: torch.cuda.Event(enable_timing=True)
     res = {}
     sols = {}
+    amg_prm = na.MueLuParams(cheb_degree=args.muelu_degree)
     for name, method, prec, precision in (("fp64_gmres_sgs", "gmres", "sgs", "fp64"),
                                           ("mixed_fgmres_learned", "fgmres", "learned", "mixed"),
-                                          ("mixed_fgmres_sgs", "fgmres", "sgs", "mixed")):
+                                          ("mixed_fgmres_sgs", "fgmres", "sgs", "mixed"),
+                                          ("fp64_cg_muelu", "cg", "muelu", "fp64"),
+                                          ("mixed_fgmres_muelu", "fgmres", "muelu", "mixed")):
         ls = na.LinearSystem(1, na.SolverConfig(method, prec, tolerance=1e-8, max_iterations=5000, kspace=50,
                                                 precision=precision))
+        if prec == "muelu":
+            ls.set_muelu_params(amg_prm)
         ls.buildElemToNodeGraph(mesh)
         ls.finalizeLinearSystem()
         ls.set_stream(torch.cuda.current_stream().cuda_stream)
@@ -206,13 +214,17 @@ def solver_compare(args, na, torch, mesh, P, devc):
                      "true_rel_resid": rel, "status": st}
         if prec == "learned":
             res[name]["fell_back_to_sgs"] = ls.learned_fallback()
+        if prec == "muelu":
+            res[name]["amg"] = dict(ls.amg_info(), cheb_degree=args.muelu_degree)
         sols[name] = x - x.mean()
         print("bench: %s %.4f s, %d iters, resid %.2e" % (name, res[name]["solve_s"], res[name]["iters"], rel),
               file=sys.stderr, flush=True)
     res["learned_apply"] = learned_apply_timing(args, na, torch, devc, net=net, mesh=mesh, P=P)
     ref = sols["fp64_gmres_sgs"]
-    for name in ("mixed_fgmres_learned", "mixed_fgmres_sgs"):
+    for name in ("mixed_fgmres_learned", "mixed_fgmres_sgs", "fp64_cg_muelu", "mixed_fgmres_muelu"):
         res[name]["pressure_rel_diff_vs_fp64"] = float(torch.linalg.norm(sols[name] - ref) / torch.linalg.norm(ref))
+    res["speedup_fp64_cg_muelu_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["fp64_cg_muelu"]["solve_s"]
+    res["speedup_mixed_muelu_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_muelu"]["solve_s"]
     res["speedup_mixed_learned_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_learned"]["solve_s"]
     res["speedup_mixed_sgs_vs_fp64"] = res["fp64_gmres_sgs"]["solve_s"] / res["mixed_fgmres_sgs"]["solve_s"]
     res["learned_training"] = train_rec

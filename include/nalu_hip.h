@@ -46,6 +46,7 @@ extern "C" {
 #define NALU_PREC_JACOBI 1
 #define NALU_PREC_SGS 2
 #define NALU_PREC_LEARNED 3 /* learned message-passing net, SGS fallback; no reference counterpart */
+#define NALU_PREC_MUELU 4   /* smoothed-aggregation AMG V-cycle ("muelu", src/LinearSolver.C:124-152) */
 /* Arithmetic of the Krylov inner loop. */
 #define NALU_PRECISION_FP64 0
 #define NALU_PRECISION_MIXED 1 /* FP32 inner Krylov/SpMV/SGS inside FP64 iterative refinement */
@@ -63,6 +64,20 @@ typedef struct nalu_solver_config {
   int precision;          /* NALU_PRECISION_* */
   int final_outer_iter;   /* Realm::isFinalOuterIter_ */
 } nalu_solver_config;
+
+/* The MueLu parameter-list entries the AMG preconditioner honours (the keys of
+ * reg_tests/xml/matches_ml_default.xml and milestone.xml; defaults are that
+ * first file's values plus MueLu's / Ifpack2's own defaults for the rest). */
+typedef struct nalu_muelu_params {
+  int max_coarse_size; /* "coarse: max size" (1000); the coarsest level is solved densely */
+  int max_levels;      /* "max levels" (10) */
+  int cheb_degree;     /* "chebyshev: degree" (4) */
+  double cheb_ratio;   /* "chebyshev: ratio eigenvalue" (20): lambda_min = lambda_max / ratio */
+  double cheb_boost;   /* "chebyshev: boost factor" (1.1) applied to the estimate */
+  int cheb_eig_iters;  /* "chebyshev: eigenvalue max iterations" (30) */
+  double drop_tol;     /* "aggregation: drop tol" (0) */
+  double sa_damping;   /* "sa: damping factor" (4/3): omega = damping / lambda_max(D^-1 A) */
+} nalu_muelu_params;
 
 /* ---------------------------------------------------------------- errors */
 const char *nalu_last_error(void);
@@ -173,6 +188,9 @@ int nalu_linsys_precond_apply(nalu_linsys *ls, const double *r_dev, double *z_de
  * A to vectors in that order (after nalu_linsys_precond_setup). */
 int nalu_linsys_permutation(nalu_linsys *ls, int32_t *perm_host);
 int nalu_linsys_precond_apply_permuted(nalu_linsys *ls, const double *r_dev, double *z_dev);
+/* The same in FP32 arithmetic: the preconditioner as the mixed-precision inner
+ * solve applies it.  No counterpart in the reference. */
+int nalu_linsys_precond_apply_permuted_f32(nalu_linsys *ls, const float *r_dev, float *z_dev);
 int nalu_linsys_spmv_permuted(nalu_linsys *ls, const double *x_dev, double *y_dev);
 /* Solve A x = b for a caller-provided rhs (x in/out is the initial guess). */
 int nalu_linsys_solve_rhs(nalu_linsys *ls, const double *b_dev, double *x_dev, int *iters,
@@ -317,6 +335,31 @@ int nalu_gnn_train(nalu_gnn *g, nalu_linsys *ls, const nalu_gnn_data *d, int epo
 int nalu_gnn_train_epochs(nalu_gnn *g, nalu_linsys *ls, const nalu_gnn_data *d, int first_epoch,
                           int num_epochs, int total_epochs, int batch, double lr, uint64_t seed,
                           double *epoch_loss_host);
+
+/* ------------------------------------------- AMG (MueLu-equivalent) */
+/* Parameters of the AMG preconditioner (NALU_PREC_MUELU) of this system: the
+ * muelu_xml_file_name parameter list of LinearSolverConfig
+ * (src/LinearSolverConfig.C:86-100) given as a struct.  Takes effect at the
+ * next preconditioner set-up. */
+int nalu_linsys_set_muelu_params(nalu_linsys *ls, const nalu_muelu_params *p);
+/* Hierarchy summary after set-up: num_levels; per level (up to max_levels
+ * entries) rows, nonzeros and the lambda_max(D^-1 A) estimate; set-up seconds;
+ * skipped pivots of the dense coarse solve (1 for a pure Neumann operator);
+ * |A_c B_c| / (scale |B_c|) of the coarsest operator along its near-null
+ * vector (<= 1e-5: treated as an exact null vector).  Any output pointer may
+ * be NULL.  No counterpart in the reference (MueLu's verbose hierarchy
+ * printout). */
+int nalu_amg_info(const nalu_linsys *ls, int *num_levels, int64_t *rows, int64_t *nnz, double *lmax,
+                  int max_levels, double *setup_s, int *coarse_free, double *coarse_null_rel);
+/* Host copy of level `level`'s operator (which = 0) or prolongator (which = 1)
+ * as CSR in the solver's row order (level 0 = the color permutation of
+ * nalu_linsys_permutation); rp [n+1], ci / v [nnz]; n, m, nnz are returned
+ * (call with NULL arrays first to size them).  Test and inspection aid, no
+ * counterpart in the reference. */
+int nalu_amg_level_csr(const nalu_linsys *ls, int level, int which, int32_t *rp, int32_t *ci, double *v,
+                       int64_t *n, int64_t *m, int64_t *nnz);
+/* Aggregate id of every row of level `level` (not the coarsest). */
+int nalu_amg_aggregates(const nalu_linsys *ls, int level, int32_t *agg);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,7 @@ inline int check(int rc, const char* what) {
 
 struct SolverConfig {
   std::string method = "gmres";          // gmres | cg | fgmres
-  std::string preconditioner = "sgs";    // none | jacobi | sgs | mt_sgs | learned
+  std::string preconditioner = "sgs";    // none | jacobi | sgs | mt_sgs | learned | muelu
   double tolerance = 1e-8;
   double finalTolerance = 1e-8;
   int maxIterations = 1000;
@@ -41,6 +41,7 @@ struct SolverConfig {
     else if (preconditioner == "jacobi" || preconditioner == "default") c.precond = NALU_PREC_JACOBI;
     else if (preconditioner == "sgs" || preconditioner == "mt_sgs") c.precond = NALU_PREC_SGS;
     else if (preconditioner == "learned") c.precond = NALU_PREC_LEARNED;
+    else if (preconditioner == "muelu") c.precond = NALU_PREC_MUELU;
     else throw std::runtime_error("unsupported preconditioner: " + preconditioner);
     c.tolerance = tolerance;
     c.final_tolerance = finalTolerance;
@@ -117,6 +118,11 @@ class LinearSystem {
   void setWriteMatrixFiles(const std::string& baseName) {
     check(nalu_linsys_set_write_matrix_files(ls_, baseName.empty() ? nullptr : baseName.c_str()),
           "setWriteMatrixFiles");
+  }
+
+  // MueLu parameter list (muelu_xml_file_name, src/LinearSolverConfig.C:86-100) as a struct
+  void setMueLuParams(const nalu_muelu_params& p) {
+    check(nalu_linsys_set_muelu_params(ls_, &p), "setMueLuParams");
   }
 
   // solve; writes the delta solution (device array of numRows doubles)

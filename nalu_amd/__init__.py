@@ -26,14 +26,14 @@ import numpy as np
 __all__ = [
     "lib", "available", "NaluError", "SolverConfig", "Mesh", "LinearSystem", "lattice", "normal_field",
     "lognormal_field", "assemble_scalar_diff", "compute_mdot", "assemble_continuity", "assemble_momentum",
-    "METHOD", "PRECOND", "PRECISION", "LearnedPreconditioner", "HarvestedPairs",
+    "METHOD", "PRECOND", "PRECISION", "LearnedPreconditioner", "HarvestedPairs", "MueLuParams",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnalu_hip.so")
 
 METHOD = {"gmres": 0, "cg": 1, "fgmres": 2}
-PRECOND = {"none": 0, "jacobi": 1, "sgs": 2, "mt_sgs": 2, "learned": 3}
+PRECOND = {"none": 0, "jacobi": 1, "sgs": 2, "mt_sgs": 2, "learned": 3, "muelu": 4}
 PRECISION = {"fp64": 0, "mixed": 1}
 
 _i64 = C.c_int64
@@ -45,6 +45,12 @@ _vp = C.c_void_p
 
 class NaluError(RuntimeError):
     pass
+
+
+class _MueLuParams(C.Structure):
+    _fields_ = [("max_coarse_size", C.c_int), ("max_levels", C.c_int), ("cheb_degree", C.c_int),
+                ("cheb_ratio", C.c_double), ("cheb_boost", C.c_double), ("cheb_eig_iters", C.c_int),
+                ("drop_tol", C.c_double), ("sa_damping", C.c_double)]
 
 
 class _Config(C.Structure):
@@ -99,8 +105,14 @@ def lib():
         L.nalu_linsys_precond_setup.argtypes = [_vp]
         L.nalu_linsys_precond_apply.argtypes = [_vp, _vp, _vp]
         L.nalu_linsys_precond_apply_permuted.argtypes = [_vp, _vp, _vp]
+        L.nalu_linsys_precond_apply_permuted_f32.argtypes = [_vp, _vp, _vp]
         L.nalu_linsys_spmv_permuted.argtypes = [_vp, _vp, _vp]
         L.nalu_linsys_permutation.argtypes = [_vp, _i32p]
+        L.nalu_linsys_set_muelu_params.argtypes = [_vp, C.POINTER(_MueLuParams)]
+        L.nalu_amg_info.argtypes = [_vp, C.POINTER(C.c_int), _i64p, _i64p, _dp, C.c_int, _dp, C.POINTER(C.c_int),
+                                    _dp]
+        L.nalu_amg_level_csr.argtypes = [_vp, C.c_int, C.c_int, _i32p, _i32p, _dp, _i64p, _i64p, _i64p]
+        L.nalu_amg_aggregates.argtypes = [_vp, C.c_int, _i32p]
         L.nalu_linsys_solve_rhs.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_int), _dp]
         L.nalu_assemble_scalar_diff.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int]
         L.nalu_compute_mdot.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp]
@@ -254,6 +266,28 @@ class SolverConfig:
         return _Config(METHOD[self.method], PRECOND[self.preconditioner], float(self.tolerance),
                        float(self.final_tolerance), int(self.max_iterations), int(self.kspace),
                        PRECISION[self.precision], int(bool(self.final_outer_iter)))
+
+
+class MueLuParams:
+    """The MueLu parameter-list keys the AMG preconditioner honours; defaults are
+    reg_tests/xml/matches_ml_default.xml (milestone.xml: cheb_degree=2,
+    cheb_eig_iters=15, drop_tol=0.02)."""
+
+    def __init__(self, max_coarse_size=1000, max_levels=10, cheb_degree=4, cheb_ratio=20.0, cheb_boost=1.1,
+                 cheb_eig_iters=30, drop_tol=0.0, sa_damping=4.0 / 3.0):
+        self.max_coarse_size = max_coarse_size
+        self.max_levels = max_levels
+        self.cheb_degree = cheb_degree
+        self.cheb_ratio = cheb_ratio
+        self.cheb_boost = cheb_boost
+        self.cheb_eig_iters = cheb_eig_iters
+        self.drop_tol = drop_tol
+        self.sa_damping = sa_damping
+
+    def _struct(self):
+        return _MueLuParams(int(self.max_coarse_size), int(self.max_levels), int(self.cheb_degree),
+                            float(self.cheb_ratio), float(self.cheb_boost), int(self.cheb_eig_iters),
+                            float(self.drop_tol), float(self.sa_damping))
 
 
 # ------------------------------------------------------------------ linear system
@@ -414,8 +448,51 @@ class LinearSystem:
         _check(lib().nalu_linsys_permutation(self._h, perm.ctypes.data_as(_i32p)), "permutation")
         return perm
 
+    def set_muelu_params(self, params):
+        """AMG parameters (a MueLuParams) for the next preconditioner set-up."""
+        _check(lib().nalu_linsys_set_muelu_params(self._h, C.byref(params._struct())), "set_muelu_params")
+
+    def amg_info(self):
+        """{'rows', 'nnz', 'lmax' per level, 'setup_s', 'coarse_free', 'coarse_null_rel'} of the AMG hierarchy."""
+        nl = C.c_int(0)
+        cap = 32
+        rows = np.zeros(cap, dtype=np.int64)
+        nnz = np.zeros(cap, dtype=np.int64)
+        lmax = np.zeros(cap, dtype=np.float64)
+        st = C.c_double(0.0)
+        nf = C.c_int(0)
+        nr = C.c_double(0.0)
+        _check(lib().nalu_amg_info(self._h, C.byref(nl), rows.ctypes.data_as(_i64p), nnz.ctypes.data_as(_i64p),
+                                   lmax.ctypes.data_as(_dp), cap, C.byref(st), C.byref(nf), C.byref(nr)), "amg_info")
+        k = nl.value
+        return {"rows": rows[:k].tolist(), "nnz": nnz[:k].tolist(), "lmax": lmax[:k].tolist(),
+                "setup_s": st.value, "coarse_free": nf.value, "coarse_null_rel": nr.value}
+
+    def amg_level_csr(self, level, which="A"):
+        """(row_ptr, col_idx, values, n, m) of a level's operator ("A") or prolongator ("P"), solver order."""
+        w = {"A": 0, "P": 1}[which]
+        n, m, nz = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        _check(lib().nalu_amg_level_csr(self._h, int(level), w, None, None, None, C.byref(n), C.byref(m),
+                                        C.byref(nz)), "amg_level_csr")
+        rp = np.zeros(n.value + 1, dtype=np.int32)
+        ci = np.zeros(max(nz.value, 1), dtype=np.int32)
+        v = np.zeros(max(nz.value, 1), dtype=np.float64)
+        _check(lib().nalu_amg_level_csr(self._h, int(level), w, rp.ctypes.data_as(_i32p), ci.ctypes.data_as(_i32p),
+                                        v.ctypes.data_as(_dp), C.byref(n), C.byref(m), C.byref(nz)), "amg_level_csr")
+        return rp, ci[:nz.value], v[:nz.value], n.value, m.value
+
+    def amg_aggregates(self, level):
+        n = self.amg_level_csr(level, "A")[3]
+        agg = np.zeros(n, dtype=np.int32)
+        _check(lib().nalu_amg_aggregates(self._h, int(level), agg.ctypes.data_as(_i32p)), "amg_aggregates")
+        return agg
+
     def precond_apply_permuted(self, r, z):
         _check(lib().nalu_linsys_precond_apply_permuted(self._h, _ptr(r), _ptr(z)), "precond_apply_permuted")
+
+    def precond_apply_permuted_f32(self, r, z):
+        """M^-1 r in FP32 (float32 device tensors, solver order)."""
+        _check(lib().nalu_linsys_precond_apply_permuted_f32(self._h, _ptr(r), _ptr(z)), "precond_apply_permuted_f32")
 
     def spmv_permuted(self, x, y):
         _check(lib().nalu_linsys_spmv_permuted(self._h, _ptr(x), _ptr(y)), "spmv_permuted")

@@ -171,6 +171,9 @@ struct nalu_linsys {
 
   // learned preconditioner (optional)
   void* learned = nullptr;
+  // AMG preconditioner: parameters and hierarchy (amg.hip)
+  nalu_muelu_params muelu{1000, 10, 4, 20.0, 1.1, 30, 0.0, 4.0 / 3.0};
+  void* amg = nullptr;
 
   // matrix-file output around solve() (LinearSolverConfig "write_matrix_files")
   std::string write_base;  // empty: off
@@ -244,4 +247,12 @@ void learned_set_fallback(nalu_linsys* ls, bool on);
 void learned_apply_f32(nalu_linsys* ls, const float* r, float* z);
 void learned_apply_f64(nalu_linsys* ls, const double* r, double* z);
 void learned_release(nalu_linsys* ls);
+// amg.hip
+void amg_setup(nalu_linsys* ls);
+template <typename T>
+void amg_apply(nalu_linsys* ls, const T* r, T* z);
+void amg_release(nalu_linsys* ls);
+// krylov.hip: r = b - A x in solver order (SELL-64 copy)
+template <typename T>
+void perm_residual(nalu_linsys* ls, const T* x, T* r, const T* b);
 }  // namespace nalu

@@ -968,6 +968,9 @@ void pprec(nalu_linsys* ls, const T* r, T* z) {
     case NALU_PREC_SGS:
       psgs<T>(ls, r, z);
       break;
+    case NALU_PREC_MUELU:
+      amg_apply<T>(ls, r, z);
+      break;
     default:
       hipLaunchKernelGGL((k_copy<T, T>), dim3(nblocks(n)), dim3(kThreads), 0, ls->stream, r, z, n);
   }
@@ -1217,6 +1220,12 @@ void perm_sgs64(nalu_linsys* ls, const double* r, double* z) { psgs<double>(ls, 
 void perm_spmv64(nalu_linsys* ls, const double* x, double* y) {
   pspmv<double, double>(ls, x, y, (const double*)nullptr, 0);
 }
+template <typename T>
+void perm_residual(nalu_linsys* ls, const T* x, T* r, const T* b) {
+  pspmv<T, T>(ls, x, r, b, 1);
+}
+template void perm_residual<double>(nalu_linsys*, const double*, double*, const double*);
+template void perm_residual<float>(nalu_linsys*, const float*, float*, const float*);
 
 // FP32 copy of the permuted split values (mixed-precision inner solve, and the
 // SGS input feature of the learned preconditioner)
@@ -1283,6 +1292,8 @@ void precond_setup(nalu_linsys* ls) {
   ls->split32_ready = false;
   if (mixed(ls) || ls->learned) ensure_split32(ls);
   learned_setup(ls);
+  if (ls->cfg.precond == NALU_PREC_MUELU) amg_setup(ls);
+  else amg_release(ls);
 }
 
 // M^-1 r in the original row order (permutes in and out through two
@@ -1399,6 +1410,17 @@ extern "C" int nalu_linsys_precond_apply_permuted(nalu_linsys* ls, const double*
     NALU_REQUIRE(ls && ls->finalized && r && z, NALU_ERR_ARG, "precond_apply_permuted: bad arguments");
     NALU_REQUIRE(ls->precond_ready, NALU_ERR_STATE, "precond_apply_permuted: preconditioner not set up");
     nalu::pprec<double>(ls, r, z);
+    NALU_HIP_CHECK(hipGetLastError());
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_linsys_precond_apply_permuted_f32(nalu_linsys* ls, const float* r, float* z) {
+  return nalu::guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized && r && z, NALU_ERR_ARG, "precond_apply_permuted_f32: bad arguments");
+    NALU_REQUIRE(ls->precond_ready, NALU_ERR_STATE, "precond_apply_permuted_f32: preconditioner not set up");
+    nalu::ensure_split32(ls);
+    nalu::pprec<float>(ls, r, z);
     NALU_HIP_CHECK(hipGetLastError());
     return NALU_OK;
   });

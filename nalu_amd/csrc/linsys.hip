@@ -112,7 +112,7 @@ extern "C" int nalu_linsys_create(int num_dof, const nalu_solver_config* cfg, na
     NALU_REQUIRE(num_dof >= 1 && num_dof <= 8, NALU_ERR_ARG, "linsys_create: numDof out of range");
     NALU_REQUIRE(cfg->method >= NALU_METHOD_GMRES && cfg->method <= NALU_METHOD_FGMRES, NALU_ERR_ARG,
                  "linsys_create: unknown method");
-    NALU_REQUIRE(cfg->precond >= NALU_PREC_NONE && cfg->precond <= NALU_PREC_LEARNED, NALU_ERR_ARG,
+    NALU_REQUIRE(cfg->precond >= NALU_PREC_NONE && cfg->precond <= NALU_PREC_MUELU, NALU_ERR_ARG,
                  "linsys_create: unknown preconditioner");
     NALU_REQUIRE(cfg->precision == NALU_PRECISION_FP64 || cfg->precision == NALU_PRECISION_MIXED, NALU_ERR_ARG,
                  "linsys_create: unknown precision");
@@ -130,6 +130,7 @@ extern "C" int nalu_linsys_destroy(nalu_linsys* ls) {
   else (void)hipDeviceSynchronize();
   flush_timers(ls);
   learned_release(ls);
+  amg_release(ls);
   delete ls;
   return NALU_OK;
 }
