@@ -408,7 +408,8 @@ def stretched_compare(args, na, torch, devc):
     """Stretched-mesh pressure-Poisson: geometric ratio 1.05 from all faces
     (aspect ratio capped at 100), jitter 0.05h, density lognormal(0, 0.3).
     Plain FP64 GMRES(50) + SGS against mixed FGMRES(50) + learned preconditioner
-    to the same 1e-8 FP64 relative residual (one timed solve each)."""
+    to the same 1e-8 FP64 relative residual (one timed solve each), plus mixed
+    FGMRES + SGS and the AMG preconditioner (FP64 CG and mixed FGMRES)."""
     from nalu_amd.problems import lowmach_problem
     n = args.stretched_n
     t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
@@ -426,9 +427,13 @@ def stretched_compare(args, na, torch, devc):
            "max_to_min_spacing": float(hx.max() / hx.min()), "learned_training": rec}
     for name, method, prec, precision in (("fp64_gmres_sgs", "gmres", "sgs", "fp64"),
                                           ("mixed_fgmres_learned", "fgmres", "learned", "mixed"),
-                                          ("mixed_fgmres_sgs", "fgmres", "sgs", "mixed")):
+                                          ("mixed_fgmres_sgs", "fgmres", "sgs", "mixed"),
+                                          ("fp64_cg_muelu", "cg", "muelu", "fp64"),
+                                          ("mixed_fgmres_muelu", "fgmres", "muelu", "mixed")):
         ls = na.LinearSystem(1, na.SolverConfig(method, prec, tolerance=1e-8, max_iterations=20000, kspace=50,
                                                 precision=precision))
+        if prec == "muelu":
+            ls.set_muelu_params(na.MueLuParams(cheb_degree=args.muelu_degree))
         ls.buildElemToNodeGraph(mesh)
         ls.finalizeLinearSystem()
         ls.set_stream(torch.cuda.current_stream().cuda_stream)
@@ -444,12 +449,16 @@ def stretched_compare(args, na, torch, devc):
         out[name] = {"solve_s": dt, "iters": ls.linearSolveIterations(), "true_rel_resid": rel}
         if prec == "learned":
             out[name]["fell_back_to_sgs"] = ls.learned_fallback()
+        if prec == "muelu":
+            out[name]["amg"] = ls.amg_info()
         print("bench: stretched %s %.3f s, %d iters, resid %.2e" % (name, dt, out[name]["iters"], rel),
               file=sys.stderr, flush=True)
         del ls, x
         torch.cuda.empty_cache()
     out["speedup_mixed_learned_vs_fp64"] = out["fp64_gmres_sgs"]["solve_s"] / out["mixed_fgmres_learned"]["solve_s"]
     out["speedup_mixed_sgs_vs_fp64"] = out["fp64_gmres_sgs"]["solve_s"] / out["mixed_fgmres_sgs"]["solve_s"]
+    out["speedup_fp64_cg_muelu_vs_fp64"] = out["fp64_gmres_sgs"]["solve_s"] / out["fp64_cg_muelu"]["solve_s"]
+    out["speedup_mixed_muelu_vs_fp64"] = out["fp64_gmres_sgs"]["solve_s"] / out["mixed_fgmres_muelu"]["solve_s"]
     return out
 
 

@@ -139,6 +139,10 @@ double dot(const double* a, const double* b, int64_t n, DevBuf<double>& scratch,
 }
 
 // ------------------------------------------------------ finest-level operator
+// Row kernels of the set-up run kL lanes per row (coalesced walks of the row's
+// entries, shuffle reductions within the lane group).
+constexpr int kL = 8;
+
 // Solver-order CSR of the split system: row k = its L part, its diagonal, its
 // U part (row_ptr[k] = lrp[k] + urp[k] + k because lrp/urp are prefix sums).
 __global__ void k_fine_csr(const int32_t* __restrict__ lrp, const int32_t* __restrict__ lci,
@@ -146,21 +150,26 @@ __global__ void k_fine_csr(const int32_t* __restrict__ lrp, const int32_t* __res
                            const int32_t* __restrict__ uci, const double* __restrict__ uv,
                            const double* __restrict__ dv, int64_t n, int32_t* __restrict__ rp,
                            int32_t* __restrict__ ci, double* __restrict__ v) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = t / kL;
+  const int lane = (int)(t % kL);
   if (k > n) return;
-  rp[k] = lrp[k] + urp[k] + (int32_t)k;
+  const int32_t o0 = lrp[k] + urp[k] + (int32_t)k;
+  if (lane == 0) rp[k] = o0;
   if (k == n) return;
-  int32_t o = lrp[k] + urp[k] + (int32_t)k;
-  for (int32_t q = lrp[k]; q < lrp[k + 1]; ++q, ++o) {
-    ci[o] = lci[q];
-    v[o] = lv[q];
+  const int32_t l0 = lrp[k], nl = lrp[k + 1] - l0;
+  for (int32_t q = lane; q < nl; q += kL) {
+    ci[o0 + q] = lci[l0 + q];
+    v[o0 + q] = lv[l0 + q];
   }
-  ci[o] = (int32_t)k;
-  v[o] = dv[k];
-  ++o;
-  for (int32_t q = urp[k]; q < urp[k + 1]; ++q, ++o) {
-    ci[o] = uci[q];
-    v[o] = uv[q];
+  if (lane == 0) {
+    ci[o0 + nl] = (int32_t)k;
+    v[o0 + nl] = dv[k];
+  }
+  const int32_t u0 = urp[k], nu = urp[k + 1] - u0;
+  for (int32_t q = lane; q < nu; q += kL) {
+    ci[o0 + nl + 1 + q] = uci[u0 + q];
+    v[o0 + nl + 1 + q] = uv[u0 + q];
   }
 }
 
@@ -168,15 +177,23 @@ __global__ void k_fine_csr(const int32_t* __restrict__ lrp, const int32_t* __res
 __global__ void k_diag(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                        const double* __restrict__ v, int64_t n, double* __restrict__ d, double* __restrict__ invd,
                        int* __restrict__ bad) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / kL;
+  const int lane = (int)(t % kL);
   double a = 0.0;
-  bool found = false;
-  for (int32_t q = rp[i]; q < rp[i + 1]; ++q)
-    if (ci[q] == (int32_t)i) {
-      a += v[q];
-      found = true;
-    }
+  int found = 0;
+  if (i < n)
+    for (int32_t q = rp[i] + lane; q < rp[i + 1]; q += kL)
+      if (ci[q] == (int32_t)i) {
+        a += v[q];
+        found = 1;
+      }
+#pragma unroll
+  for (int o = kL / 2; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, kL);
+    found |= __shfl_xor(found, o, kL);
+  }
+  if (i >= n || lane != 0) return;
   if (!found || a == 0.0 || !isfinite(a)) {
     atomicOr(bad, 1);
     a = 1.0;
@@ -233,52 +250,91 @@ __global__ void k_hash_vec(double* __restrict__ x, int64_t n) {
   x[i] = 0.5 + (double)(h & 0xffffff) / (double)0x1000000;  // (0.5, 1.5): no zero entries
 }
 
-__global__ void k_scale_diag(const double* __restrict__ invd, const double* __restrict__ y, double sc, int64_t n,
-                             double* __restrict__ out) {
+// out = invd * y / sqrt(*nx2)
+__global__ void k_scale_diag(const double* __restrict__ invd, const double* __restrict__ y,
+                             const double* __restrict__ nx2, int64_t n, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = sc * invd[i] * y[i];
+  if (i < n) out[i] = invd[i] * y[i] / sqrt(*nx2);
+}
+
+void dot_dev(const double* a, const double* b, int64_t n, double* part, double* out, hipStream_t s) {
+  constexpr int kParts = 512;
+  const unsigned np = std::min<unsigned>(kParts, nbl(n));
+  hipLaunchKernelGGL(k_dot_part, dim3(np), dim3(kT), 0, s, a, b, n, part);
+  hipLaunchKernelGGL(k_sum_part, dim3(1), dim3(kT), 0, s, part, (int)np, out);
 }
 
 // lambda_max(D^-1 A) by the power method from a fixed pseudo-random start:
 // x <- x / |x|, y = D^-1 A x, lambda = x . y (the Rayleigh-quotient form of the
-// estimate Ifpack2's Chebyshev setup uses).
+// estimate Ifpack2's Chebyshev setup uses).  The scalars stay on the device;
+// one host read at the end.
 double power_lmax(const Csr& A, const double* invd, int iters, hipStream_t s) {
   const int64_t n = A.n;
-  DevBuf<double> x(n), y(n), t(n), scratch;
+  const int it_n = std::max(1, iters);
+  DevBuf<double> x(n), y(n), t(n), part(512), sc(2 * it_n);
   hipLaunchKernelGGL(k_hash_vec, dim3(nbl(n)), dim3(kT), 0, s, x.p, n);
-  double lam = 0.0;
-  for (int it = 0; it < std::max(1, iters); ++it) {
-    const double nx = std::sqrt(dot(x.p, x.p, n, scratch, s));
-    if (!(nx > 0.0)) break;
+  for (int it = 0; it < it_n; ++it) {
+    double* nx2 = sc.p + 2 * it;
+    dot_dev(x.p, x.p, n, part.p, nx2, s);
     csr_spmv<double>(A, A.v.p, x.p, t.p, (const double*)nullptr, 0, s);
-    hipLaunchKernelGGL(k_scale_diag, dim3(nbl(n)), dim3(kT), 0, s, invd, t.p, 1.0 / nx, n, y.p);
-    lam = dot(x.p, y.p, n, scratch, s) / nx;
+    hipLaunchKernelGGL(k_scale_diag, dim3(nbl(n)), dim3(kT), 0, s, invd, t.p, nx2, n, y.p);
+    dot_dev(x.p, y.p, n, part.p, nx2 + 1, s);
     std::swap(x.p, y.p);
+  }
+  NALU_HIP_CHECK(hipGetLastError());
+  std::vector<double> h(2 * it_n);
+  NALU_HIP_CHECK(hipMemcpyAsync(h.data(), sc.p, sizeof(double) * 2 * it_n, hipMemcpyDeviceToHost, s));
+  NALU_HIP_CHECK(hipStreamSynchronize(s));
+  // lambda = (x . y) / |x| of the last iteration whose |x| was positive
+  double lam = 0.0;
+  for (int it = 0; it < it_n; ++it) {
+    if (!(h[2 * it] > 0.0) || !std::isfinite(h[2 * it])) break;
+    lam = h[2 * it + 1] / std::sqrt(h[2 * it]);
   }
   return lam;
 }
 
 // ------------------------------------------------------------ aggregation
-// strong[q] = 1 when entry q is a strong off-diagonal connection; lump[i] =
-// sum of the weak off-diagonal entries of row i (folded into the diagonal of
-// the filtered operator).
+// strong[q] = 1 when entry q is a strong off-diagonal connection, and the
+// values of S = I - omega D_f^-1 A_f on A's pattern: weak entries 0, the
+// filtered diagonal d_f = a_ii + the weak off-diagonal entries (lumping).  A row
+// without strong entries, or whose d_f is below 1e-12 |a_ii|, keeps the
+// identity row (its tentative prolongator row is not smoothed).
 __global__ void k_strength(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                            const double* __restrict__ v, const double* __restrict__ d, int64_t n, double tol,
-                           uint8_t* __restrict__ strong, double* __restrict__ lump) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                           double omega, uint8_t* __restrict__ strong, double* __restrict__ sv) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / kL;
+  const int lane = (int)(t % kL);
   double l = 0.0;
-  for (int32_t q = rp[i]; q < rp[i + 1]; ++q) {
-    const int32_t j = ci[q];
-    const double a = v[q];
-    bool st = false;
-    if (j != (int32_t)i) {
-      st = (a != 0.0) && (tol == 0.0 || a * a > tol * tol * fabs(d[i] * d[j]));
-      if (!st) l += a;
+  int has = 0;
+  if (i < n) {
+    const double di = d[i];
+    for (int32_t q = rp[i] + lane; q < rp[i + 1]; q += kL) {
+      const int32_t j = ci[q];
+      const double a = v[q];
+      bool st = false;
+      if (j != (int32_t)i) {
+        st = (a != 0.0) && (tol == 0.0 || a * a > tol * tol * fabs(di * d[j]));
+        if (!st) l += a;
+      }
+      strong[q] = st ? 1 : 0;
+      has |= st ? 1 : 0;
     }
-    strong[q] = st ? 1 : 0;
   }
-  lump[i] = l;
+#pragma unroll
+  for (int o = kL / 2; o > 0; o >>= 1) {
+    l += __shfl_xor(l, o, kL);
+    has |= __shfl_xor(has, o, kL);
+  }
+  if (i >= n) return;
+  const double di = d[i];
+  const double df = di + l;
+  const double w = (has && fabs(df) > 1e-12 * fabs(di)) ? omega / df : 0.0;
+  for (int32_t q = rp[i] + lane; q < rp[i + 1]; q += kL) {
+    if (ci[q] == (int32_t)i) sv[q] = 1.0 - w * df;
+    else sv[q] = strong[q] ? -w * v[q] : 0.0;
+  }
 }
 
 __device__ __forceinline__ uint32_t prio_hash(uint32_t i) {
@@ -303,28 +359,53 @@ __global__ void k_mis_keys(const int8_t* __restrict__ state, int64_t n, uint64_t
 __global__ void k_mis_max(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                           const uint8_t* __restrict__ strong, const uint64_t* __restrict__ in, int64_t n,
                           uint64_t* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint64_t m = in[i];
-  for (int32_t q = rp[i]; q < rp[i + 1]; ++q)
-    if (strong[q]) m = max(m, in[ci[q]]);
-  out[i] = m;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / kL;
+  const int lane = (int)(t % kL);
+  uint64_t m = 0;
+  if (i < n) {
+    if (lane == 0) m = in[i];
+    for (int32_t q = rp[i] + lane; q < rp[i + 1]; q += kL)
+      if (strong[q]) m = max(m, in[ci[q]]);
+  }
+#pragma unroll
+  for (int o = kL / 2; o > 0; o >>= 1) m = max(m, (uint64_t)__shfl_xor((unsigned long long)m, o, kL));
+  if (i < n && lane == 0) out[i] = m;
 }
 
 // undecided row: root when it holds the largest key of its 2-neighbourhood,
-// covered when a root lies within two steps; counts the rows still undecided
+// covered when a root lies within two steps; *undecided is set to 1 (one
+// store per wave) while rows remain undecided
 __global__ void k_mis_update(const uint64_t* __restrict__ key, const uint64_t* __restrict__ m2, int64_t n,
                              int8_t* __restrict__ state, int* __restrict__ undecided) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || state[i] != 0) return;
-  if (m2[i] == key[i]) state[i] = 1;
-  else if ((m2[i] >> 62) == 3u) state[i] = 2;
-  else atomicAdd(undecided, 1);
+  bool left = false;
+  if (i < n && state[i] == 0) {
+    if (m2[i] == key[i]) state[i] = 1;
+    else if ((m2[i] >> 62) == 3u) state[i] = 2;
+    else left = true;
+  }
+  if (__ballot(left) != 0 && (threadIdx.x % 64) == 0) *undecided = 1;
 }
 
 __global__ void k_root_flags(const int8_t* __restrict__ state, int64_t n, int32_t* __restrict__ flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i <= n) flag[i] = (i < n && state[i] == 1) ? 1 : 0;
+}
+
+// best (|a|, key) candidate of a lane group; all lanes end with the group's best
+__device__ __forceinline__ void group_best(double& best, uint64_t& bk, int32_t& a) {
+#pragma unroll
+  for (int o = kL / 2; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, kL);
+    const uint64_t ok = (uint64_t)__shfl_xor((unsigned long long)bk, o, kL);
+    const int32_t oa = __shfl_xor(a, o, kL);
+    if (ob > best || (ob == best && ok > bk)) {
+      best = ob;
+      bk = ok;
+      a = oa;
+    }
+  }
 }
 
 // pass 1: roots get their own aggregate; rows adjacent to roots join the root
@@ -333,28 +414,29 @@ __global__ void k_agg_pass1(const int32_t* __restrict__ rp, const int32_t* __res
                             const double* __restrict__ v, const uint8_t* __restrict__ strong,
                             const int8_t* __restrict__ state, const int32_t* __restrict__ root_id, int64_t n,
                             int32_t* __restrict__ agg) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (state[i] == 1) {
-    agg[i] = root_id[i];
-    return;
-  }
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / kL;
+  const int lane = (int)(t % kL);
+  const bool live = i < n && state[i] != 1;
   double best = -1.0;
   uint64_t bk = 0;
   int32_t a = -1;
-  for (int32_t q = rp[i]; q < rp[i + 1]; ++q) {
-    if (!strong[q]) continue;
-    const int32_t j = ci[q];
-    if (state[j] != 1) continue;
-    const double w = fabs(v[q]);
-    const uint64_t kj = mis_key(1, (uint32_t)j);
-    if (w > best || (w == best && kj > bk)) {
-      best = w;
-      bk = kj;
-      a = root_id[j];
+  if (live)
+    for (int32_t q = rp[i] + lane; q < rp[i + 1]; q += kL) {
+      if (!strong[q]) continue;
+      const int32_t j = ci[q];
+      if (state[j] != 1) continue;
+      const double w = fabs(v[q]);
+      const uint64_t kj = mis_key(1, (uint32_t)j);
+      if (w > best || (w == best && kj > bk)) {
+        best = w;
+        bk = kj;
+        a = root_id[j];
+      }
     }
-  }
-  agg[i] = a;
+  group_best(best, bk, a);
+  if (i >= n || lane != 0) return;
+  agg[i] = live ? a : root_id[i];
 }
 
 // pass 2: the remaining rows join the aggregate of their strongest neighbour
@@ -363,26 +445,31 @@ __global__ void k_agg_pass2(const int32_t* __restrict__ rp, const int32_t* __res
                             const double* __restrict__ v, const uint8_t* __restrict__ strong,
                             const int32_t* __restrict__ agg1, int64_t n, int32_t* __restrict__ agg,
                             int* __restrict__ orphan) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (agg1[i] >= 0) {
-    agg[i] = agg1[i];
-    return;
-  }
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / kL;
+  const int lane = (int)(t % kL);
+  const bool live = i < n && agg1[i] < 0;
   double best = -1.0;
   uint64_t bk = 0;
   int32_t a = -1;
-  for (int32_t q = rp[i]; q < rp[i + 1]; ++q) {
-    if (!strong[q]) continue;
-    const int32_t j = ci[q];
-    if (agg1[j] < 0) continue;
-    const double w = fabs(v[q]);
-    const uint64_t kj = mis_key(2, (uint32_t)j);
-    if (w > best || (w == best && kj > bk)) {
-      best = w;
-      bk = kj;
-      a = agg1[j];
+  if (live)
+    for (int32_t q = rp[i] + lane; q < rp[i + 1]; q += kL) {
+      if (!strong[q]) continue;
+      const int32_t j = ci[q];
+      if (agg1[j] < 0) continue;
+      const double w = fabs(v[q]);
+      const uint64_t kj = mis_key(2, (uint32_t)j);
+      if (w > best || (w == best && kj > bk)) {
+        best = w;
+        bk = kj;
+        a = agg1[j];
+      }
     }
+  group_best(best, bk, a);
+  if (i >= n || lane != 0) return;
+  if (!live) {
+    agg[i] = agg1[i];
+    return;
   }
   if (a < 0) atomicOr(orphan, 1);
   agg[i] = a;
@@ -412,26 +499,6 @@ __global__ void k_tentative(const int32_t* __restrict__ agg, const double* __res
   ci[i] = a;
   const double nb = Bc[a];
   v[i] = nb > 0.0 ? B[i] / nb : 0.0;
-}
-
-// values of S = I - omega D_f^-1 A_f on A's pattern (weak entries: 0; the
-// filtered diagonal is a_ii + the lumped weak entries).  A row without strong
-// entries, or whose filtered diagonal is below 1e-12 |a_ii|, keeps the identity
-// row (its tentative prolongator row is not smoothed).
-__global__ void k_smoother_vals(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                const double* __restrict__ v, const uint8_t* __restrict__ strong,
-                                const double* __restrict__ d, const double* __restrict__ lump, double omega,
-                                int64_t n, double* __restrict__ sv) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double df = d[i] + lump[i];
-  bool has = false;
-  for (int32_t q = rp[i]; q < rp[i + 1]; ++q) has = has || strong[q];
-  const double w = (has && fabs(df) > 1e-12 * fabs(d[i])) ? omega / df : 0.0;
-  for (int32_t q = rp[i]; q < rp[i + 1]; ++q) {
-    if (ci[q] == (int32_t)i) sv[q] = 1.0 - w * df;
-    else sv[q] = strong[q] ? -w * v[q] : 0.0;
-  }
 }
 
 // ------------------------------------------------------------------ SpGEMM
@@ -502,11 +569,15 @@ __global__ __launch_bounds__(64) void k_spgemm(const int32_t* __restrict__ arp, 
 
 __global__ void k_row_bound(const int32_t* __restrict__ arp, const int32_t* __restrict__ aci,
                             const int32_t* __restrict__ brp, int64_t n, int32_t* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int64_t s = 0;
-  for (int32_t q = arp[i]; q < arp[i + 1]; ++q) s += brp[aci[q] + 1] - brp[aci[q]];
-  out[i] = (int32_t)std::min<int64_t>(s, 1 << 30);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / kL;
+  const int lane = (int)(t % kL);
+  int64_t sum = 0;
+  if (i < n)
+    for (int32_t q = arp[i] + lane; q < arp[i + 1]; q += kL) sum += brp[aci[q] + 1] - brp[aci[q]];
+#pragma unroll
+  for (int o = kL / 2; o > 0; o >>= 1) sum += __shfl_xor((long long)sum, o, kL);
+  if (i < n && lane == 0) out[i] = (int32_t)std::min<int64_t>(sum, 1 << 30);
 }
 
 int32_t dev_max(const int32_t* a, int64_t n, hipStream_t s) {
@@ -518,6 +589,7 @@ int32_t dev_max(const int32_t* a, int64_t n, hipStream_t s) {
   return to_host(mx.p, s);
 }
 
+constexpr int kFirstHash = 1024;
 constexpr int kMaxHash = 4096;
 
 Csr spgemm(const Csr& A, const double* av, const Csr& B, hipStream_t s) {
@@ -530,20 +602,28 @@ Csr spgemm(const Csr& A, const double* av, const Csr& B, hipStream_t s) {
     return C;
   }
   DevBuf<int32_t> cnt(A.n + 1);
-  hipLaunchKernelGGL(k_row_bound, dim3(nbl(A.n)), dim3(kT), 0, s, A.rp.p, A.ci.p, B.rp.p, A.n, cnt.p);
+  hipLaunchKernelGGL(k_row_bound, dim3(nbl(A.n * kL)), dim3(kT), 0, s, A.rp.p, A.ci.p, B.rp.p, A.n, cnt.p);
   const int64_t bound = std::min<int64_t>(dev_max(cnt.p, A.n, s), B.m);
+  // the bound (sum of the B rows met) overestimates the distinct columns of a
+  // product row several-fold: start with at most kFirstHash slots and grow to
+  // kMaxHash only when a row overflows
   int H = 64;
-  while (H < 2 * bound && H < kMaxHash) H <<= 1;
-  const size_t lds = (size_t)H * (sizeof(double) + sizeof(int32_t));
+  while (H < 2 * bound && H < kFirstHash) H <<= 1;
   DevBuf<int> ovf(1);
-  NALU_HIP_CHECK(hipMemsetAsync(ovf.p, 0, sizeof(int), s));
-  NALU_HIP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (A.n + 1), s));
   const unsigned grid = (unsigned)std::min<int64_t>(A.n, 1 << 20);
-  hipLaunchKernelGGL(k_spgemm<0>, dim3(grid), dim3(64), lds, s, A.rp.p, A.ci.p, av, B.rp.p, B.ci.p, B.v.p, A.n, H,
-                     (const int32_t*)nullptr, cnt.p, (int32_t*)nullptr, (double*)nullptr, ovf.p);
-  NALU_HIP_CHECK(hipGetLastError());
-  NALU_REQUIRE(to_host(ovf.p, s) == 0, NALU_ERR_UNSUPPORTED,
-               "amg: a product row has more than " + std::to_string(H) + " distinct columns");
+  for (;;) {
+    NALU_HIP_CHECK(hipMemsetAsync(ovf.p, 0, sizeof(int), s));
+    NALU_HIP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (A.n + 1), s));
+    const size_t lds = (size_t)H * (sizeof(double) + sizeof(int32_t));
+    hipLaunchKernelGGL(k_spgemm<0>, dim3(grid), dim3(64), lds, s, A.rp.p, A.ci.p, av, B.rp.p, B.ci.p, B.v.p, A.n,
+                       H, (const int32_t*)nullptr, cnt.p, (int32_t*)nullptr, (double*)nullptr, ovf.p);
+    NALU_HIP_CHECK(hipGetLastError());
+    if (to_host(ovf.p, s) == 0) break;
+    NALU_REQUIRE(H < kMaxHash && H < 2 * bound, NALU_ERR_UNSUPPORTED,
+                 "amg: a product row has more than " + std::to_string(H) + " distinct columns");
+    while (H < 2 * bound && H < kMaxHash) H <<= 1;
+  }
+  const size_t lds = (size_t)H * (sizeof(double) + sizeof(int32_t));
   scan_excl(cnt.p, C.rp.p, A.n + 1, s);
   C.nnz = to_host(C.rp.p + A.n, s);
   C.max_row = dev_max(cnt.p, A.n, s);
@@ -767,7 +847,7 @@ void amg_setup(nalu_linsys* ls) {
   L0.A.rp.alloc(n0 + 1);
   L0.A.ci.alloc(L0.A.nnz);
   L0.A.v.alloc(L0.A.nnz);
-  hipLaunchKernelGGL(k_fine_csr, dim3(nbl(n0 + 1)), dim3(kT), 0, s, ls->lrp.p, ls->lci.p, ls->lval.p, ls->urp.p,
+  hipLaunchKernelGGL(k_fine_csr, dim3(nbl((n0 + 1) * kL)), dim3(kT), 0, s, ls->lrp.p, ls->lci.p, ls->lval.p, ls->urp.p,
                      ls->uci.p, ls->uval.p, ls->dval.p, n0, L0.A.rp.p, L0.A.ci.p, L0.A.v.p);
   NALU_HIP_CHECK(hipGetLastError());
   L0.A.max_row = ls->max_l_len + ls->max_u_len + 1;
@@ -783,7 +863,7 @@ void amg_setup(nalu_linsys* ls) {
     L.diag.alloc(n);
     L.invd.alloc(n);
     NALU_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_diag, dim3(nbl(n)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, L.A.v.p, n, L.diag.p, L.invd.p,
+    hipLaunchKernelGGL(k_diag, dim3(nbl(n * kL)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, L.A.v.p, n, L.diag.p, L.invd.p,
                        flag.p);
     NALU_REQUIRE(to_host(flag.p, s) == 0, NALU_ERR_NUMERIC,
                  "amg: level " + std::to_string(l) + " has a zero, missing or non-finite diagonal");
@@ -796,9 +876,10 @@ void amg_setup(nalu_linsys* ls) {
 
     // strength, MIS-2 roots, aggregates
     DevBuf<uint8_t> strong(std::max<int64_t>(L.A.nnz, 1));
-    DevBuf<double> lump(n);
-    hipLaunchKernelGGL(k_strength, dim3(nbl(n)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, L.A.v.p, L.diag.p, n,
-                       pr.drop_tol, strong.p, lump.p);
+    DevBuf<double> sv(std::max<int64_t>(L.A.nnz, 1));  // values of I - omega D_f^-1 A_f
+    const double omega = pr.sa_damping / L.lmax;
+    hipLaunchKernelGGL(k_strength, dim3(nbl(n * kL)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, L.A.v.p, L.diag.p, n,
+                       pr.drop_tol, omega, strong.p, sv.p);
     DevBuf<int8_t> state(n);
     NALU_HIP_CHECK(hipMemsetAsync(state.p, 0, n, s));
     DevBuf<uint64_t> key(n), m1(n), m2(n);
@@ -807,8 +888,8 @@ void amg_setup(nalu_linsys* ls) {
     for (;; ++rounds) {
       NALU_REQUIRE(rounds <= 4096, NALU_ERR_NUMERIC, "amg: aggregation did not settle");
       hipLaunchKernelGGL(k_mis_keys, dim3(nbl(n)), dim3(kT), 0, s, state.p, n, key.p);
-      hipLaunchKernelGGL(k_mis_max, dim3(nbl(n)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, strong.p, key.p, n, m1.p);
-      hipLaunchKernelGGL(k_mis_max, dim3(nbl(n)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, strong.p, m1.p, n, m2.p);
+      hipLaunchKernelGGL(k_mis_max, dim3(nbl(n * kL)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, strong.p, key.p, n, m1.p);
+      hipLaunchKernelGGL(k_mis_max, dim3(nbl(n * kL)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, strong.p, m1.p, n, m2.p);
       NALU_HIP_CHECK(hipMemsetAsync(und.p, 0, sizeof(int), s));
       hipLaunchKernelGGL(k_mis_update, dim3(nbl(n)), dim3(kT), 0, s, key.p, m2.p, n, state.p, und.p);
       if (to_host(und.p, s) == 0) break;
@@ -820,10 +901,10 @@ void amg_setup(nalu_linsys* ls) {
     NALU_REQUIRE(nc > 0, NALU_ERR_NUMERIC, "amg: no aggregates");
     DevBuf<int32_t> agg1(n);
     L.agg.alloc(n);
-    hipLaunchKernelGGL(k_agg_pass1, dim3(nbl(n)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, L.A.v.p, strong.p, state.p,
+    hipLaunchKernelGGL(k_agg_pass1, dim3(nbl(n * kL)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, L.A.v.p, strong.p, state.p,
                        rid.p, n, agg1.p);
     NALU_HIP_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_agg_pass2, dim3(nbl(n)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, L.A.v.p, strong.p, agg1.p, n,
+    hipLaunchKernelGGL(k_agg_pass2, dim3(nbl(n * kL)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, L.A.v.p, strong.p, agg1.p, n,
                        L.agg.p, flag.p);
     NALU_REQUIRE(to_host(flag.p, s) == 0, NALU_ERR_NUMERIC, "amg: a row was left without an aggregate");
 
@@ -842,10 +923,6 @@ void amg_setup(nalu_linsys* ls) {
     Pt.v.alloc(n);
     hipLaunchKernelGGL(k_tentative, dim3(nbl(n + 1)), dim3(kT), 0, s, L.agg.p, B.p, Bc.p, n, Pt.rp.p, Pt.ci.p,
                        Pt.v.p);
-    DevBuf<double> sv(std::max<int64_t>(L.A.nnz, 1));
-    const double omega = pr.sa_damping / L.lmax;
-    hipLaunchKernelGGL(k_smoother_vals, dim3(nbl(n)), dim3(kT), 0, s, L.A.rp.p, L.A.ci.p, L.A.v.p, strong.p,
-                       L.diag.p, lump.p, omega, n, sv.p);
     L.P = spgemm(L.A, sv.p, Pt, s);
     L.R = transpose(L.P, s);
     Csr AP = spgemm(L.A, L.A.v.p, L.P, s);
