@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--learned-apply-only", type=int, default=0, metavar="K",
                     help="only time K applications of an untrained learned preconditioner on the n^3 "
                          "continuity system (profiling aid), print one JSON line and exit")
+    ap.add_argument("--amg-only", type=int, default=0, metavar="K",
+                    help="only run K AMG set-ups + FP64 CG solves on the n^3 continuity system (lognormal(0, 0.3) "
+                         "density; profiling aid), print one JSON line and exit")
     ap.add_argument("--stretched-only", action="store_true",
                     help="only the stretched-mesh pressure comparison (with --stretched-n), as one JSON line")
     ap.add_argument("--stretched-n", type=int, default=0,
@@ -404,6 +407,32 @@ def momentum_3rhs(args, na, torch, mesh, P, devc, stream, md):
     return out
 
 
+def amg_timing(args, na, torch, devc):
+    """K rounds of: assemble the n^3 continuity system, AMG set-up, FP64 CG + AMG
+    solve to 1e-8 (set-up and solve timed apart)."""
+    from nalu_amd.problems import lowmach_problem
+    P = lowmach_problem(args.n, rho_sigma=0.3, seed=4321)
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
+    mesh = na.Mesh(P["coords"], P["conn"])
+    ls = na.LinearSystem(1, na.SolverConfig("cg", "muelu", tolerance=1e-8, max_iterations=500))
+    ls.set_muelu_params(na.MueLuParams(cheb_degree=args.muelu_degree))
+    ls.buildElemToNodeGraph(mesh)
+    ls.finalizeLinearSystem()
+    ls.set_stream(torch.cuda.current_stream().cuda_stream)
+    x = torch.zeros(ls.sizes()[0], dtype=torch.float64, device=devc)
+    setup, total = [], []
+    for _ in range(args.amg_only):
+        x.zero_()
+        ls.zeroSystem()
+        na.assemble_continuity(ls, mesh, t(P["rho"]), t(P["p"]), t(P["u"]), t(P["gpdx"]), tau=1.0, mode="gather")
+        ls.loadComplete()
+        ls.removeRhsMean()
+        total.append(timed(torch, lambda: ls.solve(x)))
+        setup.append(ls.amg_info()["setup_s"])
+    return {"amg_setup_s": setup, "cg_amg_solve_s_incl_setup": total, "iters": ls.linearSolveIterations(),
+            "hierarchy": ls.amg_info()}
+
+
 def stretched_compare(args, na, torch, devc):
     """Stretched-mesh pressure-Poisson: geometric ratio 1.05 from all faces
     (aspect ratio capped at 100), jitter 0.05h, density lognormal(0, 0.3).
@@ -481,6 +510,9 @@ def main():
         return
     if args.stretched_only:
         print(json.dumps(stretched_compare(args, na, torch, devc)))
+        return
+    if args.amg_only > 0:
+        print(json.dumps(amg_timing(args, na, torch, devc)))
         return
     print("bench: generating %d^3 low-Mach problem" % n, file=sys.stderr, flush=True)
     P = lowmach_problem(n, seed=4321)
