@@ -636,7 +636,8 @@ __global__ void k_take_sell(const int32_t* __restrict__ src, const double* __res
   if (q < n) out[q] = src[q] >= 0 ? (T)v[src[q]] : T(0);
 }
 
-// y = (D + L + U) x  (mode 0)  or  y = b - (D + L + U) x  (mode 1)
+// y = (D + L + U) x  (mode 0)  or  y = b - (D + L + U) x  (mode 1); the matrix
+// entries stream through once (non-temporal loads), the gathered x stays cached
 template <typename T, typename TY>
 __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restrict__ ch, int64_t nch,
                                                         const int32_t* __restrict__ lcol, const T* __restrict__ lval,
@@ -653,31 +654,48 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restr
   const int32_t* lc = lcol + k.loff + lane;
   const T* lv = lval + k.loff + lane;
 #pragma unroll 4
-  for (int q = 0; q < k.lw; ++q) s += lv[q * 64] * x[lc[q * 64]];
+  for (int q = 0; q < k.lw; ++q)
+    s += __builtin_nontemporal_load(&lv[q * 64]) * x[__builtin_nontemporal_load(&lc[q * 64])];
   const int32_t* uc = ucol + k.uoff + lane;
   const T* uv = uval + k.uoff + lane;
 #pragma unroll 4
-  for (int q = 0; q < k.uw; ++q) s += uv[q * 64] * x[uc[q * 64]];
+  for (int q = 0; q < k.uw; ++q)
+    s += __builtin_nontemporal_load(&uv[q * 64]) * x[__builtin_nontemporal_load(&uc[q * 64])];
   if (act) y[row] = mode ? (TY)((T)b[row] - s) : (TY)s;
 }
 
-// forward sweep over the chunks [c0, c1) of one color: z = (r - L z) / d
+// forward sweep over the chunks [c0, c1) of one color: z = (r - L z) / d.
+// Two lanes per row (128 threads per 64-row chunk, even / odd entries, the
+// halves combined through LDS) double the loads in flight per chunk; the matrix
+// entries are read once per sweep, so they are loaded non-temporally and the
+// gathered z entries keep the cache.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
                                                        const int32_t* __restrict__ lcol, const T* __restrict__ lval,
                                                        const T* __restrict__ invd, const T* __restrict__ r,
                                                        T* __restrict__ z) {
-  const int64_t c = c0 + (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  __shared__ T part[kThreads / 2];
+  const int64_t c = c0 + (int64_t)blockIdx.x * (kThreads / 128) + (threadIdx.x >> 7);
   const int lane = threadIdx.x & 63;
-  if (c >= c1) return;
-  const SellChunk k = ch[c];
-  if (lane >= k.nrows) return;
-  const int row = k.row0 + lane;
+  const int half = (threadIdx.x >> 6) & 1;
+  const int slot = (threadIdx.x >> 7) * 64 + lane;
+  const bool ok = c < c1;
+  SellChunk k{};
+  if (ok) k = ch[c];
+  const bool act = ok && lane < k.nrows;
   T s = T(0);
-  const int32_t* lc = lcol + k.loff + lane;
-  const T* lv = lval + k.loff + lane;
+  if (act) {
+    const int32_t* lc = lcol + k.loff + lane;
+    const T* lv = lval + k.loff + lane;
 #pragma unroll 4
-  for (int q = 0; q < k.lw; ++q) s += lv[q * 64] * z[lc[q * 64]];
+    for (int q = half; q < k.lw; q += 2)
+      s += __builtin_nontemporal_load(&lv[q * 64]) * z[__builtin_nontemporal_load(&lc[q * 64])];
+  }
+  if (half == 1) part[slot] = s;
+  __syncthreads();
+  if (!act || half == 1) return;
+  const int row = k.row0 + lane;
+  s += part[slot];
   z[row] = (r[row] - s) * invd[row];
 }
 
@@ -686,17 +704,28 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
                                                        const int32_t* __restrict__ ucol, const T* __restrict__ uval,
                                                        const T* __restrict__ invd, T* __restrict__ z) {
-  const int64_t c = c0 + (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  __shared__ T part[kThreads / 2];
+  const int64_t c = c0 + (int64_t)blockIdx.x * (kThreads / 128) + (threadIdx.x >> 7);
   const int lane = threadIdx.x & 63;
-  if (c >= c1) return;
-  const SellChunk k = ch[c];
-  if (lane >= k.nrows) return;
-  const int row = k.row0 + lane;
+  const int half = (threadIdx.x >> 6) & 1;
+  const int slot = (threadIdx.x >> 7) * 64 + lane;
+  const bool ok = c < c1;
+  SellChunk k{};
+  if (ok) k = ch[c];
+  const bool act = ok && lane < k.nrows;
   T s = T(0);
-  const int32_t* uc = ucol + k.uoff + lane;
-  const T* uv = uval + k.uoff + lane;
+  if (act) {
+    const int32_t* uc = ucol + k.uoff + lane;
+    const T* uv = uval + k.uoff + lane;
 #pragma unroll 4
-  for (int q = 0; q < k.uw; ++q) s += uv[q * 64] * z[uc[q * 64]];
+    for (int q = half; q < k.uw; q += 2)
+      s += __builtin_nontemporal_load(&uv[q * 64]) * z[__builtin_nontemporal_load(&uc[q * 64])];
+  }
+  if (half == 1) part[slot] = s;
+  __syncthreads();
+  if (!act || half == 1) return;
+  const int row = k.row0 + lane;
+  s += part[slot];
   z[row] = z[row] - s * invd[row];
 }
 
@@ -899,13 +928,13 @@ void psgs(nalu_linsys* ls, const T* r, T* z) {
     for (int c = 0; c < ls->num_colors; ++c) {
       const int64_t c0 = ccp[c], c1 = ccp[c + 1];
       if (c1 > c0)
-        hipLaunchKernelGGL(k_sell_fwd<T>, dim3((unsigned)((c1 - c0 + 3) / 4)), dim3(kThreads), 0, s, ch, c0, c1,
+        hipLaunchKernelGGL(k_sell_fwd<T>, dim3((unsigned)((c1 - c0 + 1) / 2)), dim3(kThreads), 0, s, ch, c0, c1,
                            ls->sl_col.p, sell_vals<T>(ls, false), A.invd, r, z);
     }
     for (int c = ls->num_colors - 2; c >= 0; --c) {  // the last color has no U entries
       const int64_t c0 = ccp[c], c1 = ccp[c + 1];
       if (c1 > c0)
-        hipLaunchKernelGGL(k_sell_bwd<T>, dim3((unsigned)((c1 - c0 + 3) / 4)), dim3(kThreads), 0, s, ch, c0, c1,
+        hipLaunchKernelGGL(k_sell_bwd<T>, dim3((unsigned)((c1 - c0 + 1) / 2)), dim3(kThreads), 0, s, ch, c0, c1,
                            ls->su_col.p, sell_vals<T>(ls, true), A.invd, z);
     }
     return;

@@ -27,10 +27,14 @@ from . import (LinearSystem, NaluError, SolverConfig, assemble_continuity, assem
 
 class LowMachStep:
     """State (torch device tensors): u [N][3], p [N], rho [N] (constant in time),
-    mu [N], Gjp [N][3]; wall_nodes int32 device tensor with wall_u [len][3]."""
+    mu [N], Gjp [N][3]; wall_nodes int32 device tensor with wall_u [len][3].
+    pressure_precond: "sgs", or "muelu" for the AMG preconditioner the reference
+    configures for the continuity system (CG + MueLu in its regression inputs,
+    e.g. reg_tests/test_files/elemClosedDomain/elemClosedDomain.i:15-25); with
+    "muelu" the FP64 pressure solve is CG, the mixed one FGMRES."""
 
     def __init__(self, mesh, u, p, rho, mu, dt, wall_nodes, wall_u, picard=2, tol=1e-8, precision="fp64",
-                 stream=None):
+                 stream=None, pressure_precond="sgs", muelu_params=None):
         import torch
         self.torch = torch
         self.mesh, self.dt, self.picard, self.stream = mesh, float(dt), int(picard), stream
@@ -52,8 +56,11 @@ class LowMachStep:
         method = "fgmres" if precision == "mixed" else "gmres"
         self.mom = LinearSystem(1, SolverConfig(method, "sgs", tolerance=tol, max_iterations=2000, kspace=50,
                                                 precision=precision))
-        self.cont = LinearSystem(1, SolverConfig(method, "sgs", tolerance=tol, max_iterations=5000, kspace=50,
-                                                 precision=precision))
+        cont_method = "cg" if (pressure_precond == "muelu" and precision == "fp64") else method
+        self.cont = LinearSystem(1, SolverConfig(cont_method, pressure_precond, tolerance=tol, max_iterations=5000,
+                                                 kspace=50, precision=precision))
+        if pressure_precond == "muelu" and muelu_params is not None:
+            self.cont.set_muelu_params(muelu_params)
         for ls in (self.mom, self.cont):
             ls.buildElemToNodeGraph(mesh)
             ls.finalizeLinearSystem()

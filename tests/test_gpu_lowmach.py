@@ -20,10 +20,10 @@ def dev(a):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=DEV)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_lowmach_step(precision):
+@pytest.mark.parametrize("precision,pprec", [("fp64", "sgs"), ("mixed", "sgs"), ("fp64", "muelu"), ("mixed", "muelu")])
+def test_lowmach_step(precision, pprec):
     n = 8
-    print("case: low-Mach step n=%d precision=%s" % (n, precision), flush=True)
+    print("case: low-Mach step n=%d precision=%s pressure %s" % (n, precision, pprec), flush=True)
     P = lowmach_problem(n, seed=11, mu=0.01, noise=0.05)
     mesh = na.Mesh(P["coords"], P["conn"])
     i, j, k = lattice_ijk(n)
@@ -31,7 +31,8 @@ def test_lowmach_step(precision):
     u0 = dev(P["u"])
     wall_t = torch.as_tensor(wall, device=DEV)
     step = LowMachStep(mesh, u0, dev(P["p"]), dev(P["rho"]), dev(P["mu"]), dt=0.25 / n, wall_nodes=wall_t,
-                       wall_u=u0[wall_t.long()].clone(), picard=3, precision=precision)
+                       wall_u=u0[wall_t.long()].clone(), picard=3, precision=precision, pressure_precond=pprec,
+                       muelu_params=na.MueLuParams(max_coarse_size=50, cheb_degree=2))
     r0 = step.continuity_residual()
     rec = step.step()
     r1 = step.continuity_residual()
