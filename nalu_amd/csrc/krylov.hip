@@ -668,14 +668,17 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restr
 // Two lanes per row (128 threads per 64-row chunk, even / odd entries, the
 // halves combined through LDS) double the loads in flight per chunk; the matrix
 // entries are read once per sweep, so they are loaded non-temporally and the
-// gathered z entries keep the cache.
+// gathered z entries keep the cache.  Each XCD sweeps a contiguous range of the
+// color's chunks, so neighbouring rows gather z through the same L2.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
                                                        const int32_t* __restrict__ lcol, const T* __restrict__ lval,
                                                        const T* __restrict__ invd, const T* __restrict__ r,
                                                        T* __restrict__ z) {
   __shared__ T part[kThreads / 2];
-  const int64_t c = c0 + (int64_t)blockIdx.x * (kThreads / 128) + (threadIdx.x >> 7);
+  const int64_t nb = (c1 - c0 + 1) / 2;  // blocks of two chunks, XCD-contiguous ranges
+  const int64_t pos = xcd_block(blockIdx.x, nb);
+  const int64_t c = pos < 0 ? c1 : c0 + pos * (kThreads / 128) + (threadIdx.x >> 7);
   const int lane = threadIdx.x & 63;
   const int half = (threadIdx.x >> 6) & 1;
   const int slot = (threadIdx.x >> 7) * 64 + lane;
@@ -705,7 +708,9 @@ __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restri
                                                        const int32_t* __restrict__ ucol, const T* __restrict__ uval,
                                                        const T* __restrict__ invd, T* __restrict__ z) {
   __shared__ T part[kThreads / 2];
-  const int64_t c = c0 + (int64_t)blockIdx.x * (kThreads / 128) + (threadIdx.x >> 7);
+  const int64_t nb = (c1 - c0 + 1) / 2;  // blocks of two chunks, XCD-contiguous ranges
+  const int64_t pos = xcd_block(blockIdx.x, nb);
+  const int64_t c = pos < 0 ? c1 : c0 + pos * (kThreads / 128) + (threadIdx.x >> 7);
   const int lane = threadIdx.x & 63;
   const int half = (threadIdx.x >> 6) & 1;
   const int slot = (threadIdx.x >> 7) * 64 + lane;
@@ -928,14 +933,14 @@ void psgs(nalu_linsys* ls, const T* r, T* z) {
     for (int c = 0; c < ls->num_colors; ++c) {
       const int64_t c0 = ccp[c], c1 = ccp[c + 1];
       if (c1 > c0)
-        hipLaunchKernelGGL(k_sell_fwd<T>, dim3((unsigned)((c1 - c0 + 1) / 2)), dim3(kThreads), 0, s, ch, c0, c1,
-                           ls->sl_col.p, sell_vals<T>(ls, false), A.invd, r, z);
+        hipLaunchKernelGGL(k_sell_fwd<T>, dim3((unsigned)xcd_grid((c1 - c0 + 1) / 2)), dim3(kThreads), 0, s, ch,
+                           c0, c1, ls->sl_col.p, sell_vals<T>(ls, false), A.invd, r, z);
     }
     for (int c = ls->num_colors - 2; c >= 0; --c) {  // the last color has no U entries
       const int64_t c0 = ccp[c], c1 = ccp[c + 1];
       if (c1 > c0)
-        hipLaunchKernelGGL(k_sell_bwd<T>, dim3((unsigned)((c1 - c0 + 1) / 2)), dim3(kThreads), 0, s, ch, c0, c1,
-                           ls->su_col.p, sell_vals<T>(ls, true), A.invd, z);
+        hipLaunchKernelGGL(k_sell_bwd<T>, dim3((unsigned)xcd_grid((c1 - c0 + 1) / 2)), dim3(kThreads), 0, s, ch,
+                           c0, c1, ls->su_col.p, sell_vals<T>(ls, true), A.invd, z);
     }
     return;
   }
