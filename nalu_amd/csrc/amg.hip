@@ -129,15 +129,6 @@ __global__ void k_sum_part(const double* __restrict__ part, int np, double* __re
   if (threadIdx.x == 0) *out = sh[0];
 }
 
-double dot(const double* a, const double* b, int64_t n, DevBuf<double>& scratch, hipStream_t s) {
-  constexpr int kParts = 512;
-  if (scratch.n < kParts + 1) scratch.alloc(kParts + 1);
-  const unsigned np = std::min<unsigned>(kParts, nbl(n));
-  hipLaunchKernelGGL(k_dot_part, dim3(np), dim3(kT), 0, s, a, b, n, scratch.p);
-  hipLaunchKernelGGL(k_sum_part, dim3(1), dim3(kT), 0, s, scratch.p, (int)np, scratch.p + kParts);
-  return to_host(scratch.p + kParts, s);
-}
-
 // ------------------------------------------------------ finest-level operator
 // Row kernels of the set-up run kL lanes per row (coalesced walks of the row's
 // entries, shuffle reductions within the lane group).
@@ -267,8 +258,9 @@ void dot_dev(const double* a, const double* b, int64_t n, double* part, double* 
 // lambda_max(D^-1 A) by the power method from a fixed pseudo-random start:
 // x <- x / |x|, y = D^-1 A x, lambda = x . y (the Rayleigh-quotient form of the
 // estimate Ifpack2's Chebyshev setup uses).  The scalars stay on the device;
-// one host read at the end.
-double power_lmax(const Csr& A, const double* invd, int iters, hipStream_t s) {
+// one host read at the end.  ls (finest level only): the product runs on the
+// solver's SELL-64 copy of the same matrix.
+double power_lmax(const Csr& A, const double* invd, int iters, hipStream_t s, nalu_linsys* ls = nullptr) {
   const int64_t n = A.n;
   const int it_n = std::max(1, iters);
   DevBuf<double> x(n), y(n), t(n), part(512), sc(2 * it_n);
@@ -276,7 +268,8 @@ double power_lmax(const Csr& A, const double* invd, int iters, hipStream_t s) {
   for (int it = 0; it < it_n; ++it) {
     double* nx2 = sc.p + 2 * it;
     dot_dev(x.p, x.p, n, part.p, nx2, s);
-    csr_spmv<double>(A, A.v.p, x.p, t.p, (const double*)nullptr, 0, s);
+    if (ls) perm_spmv64(ls, x.p, t.p);
+    else csr_spmv<double>(A, A.v.p, x.p, t.p, (const double*)nullptr, 0, s);
     hipLaunchKernelGGL(k_scale_diag, dim3(nbl(n)), dim3(kT), 0, s, invd, t.p, nx2, n, y.p);
     dot_dev(x.p, y.p, n, part.p, nx2 + 1, s);
     std::swap(x.p, y.p);
@@ -867,7 +860,7 @@ void amg_setup(nalu_linsys* ls) {
                        flag.p);
     NALU_REQUIRE(to_host(flag.p, s) == 0, NALU_ERR_NUMERIC,
                  "amg: level " + std::to_string(l) + " has a zero, missing or non-finite diagonal");
-    L.lmax = power_lmax(L.A, L.invd.p, pr.cheb_eig_iters, s);
+    L.lmax = power_lmax(L.A, L.invd.p, pr.cheb_eig_iters, s, l == 0 ? ls : nullptr);
     NALU_REQUIRE(L.lmax > 0.0 && std::isfinite(L.lmax), NALU_ERR_NUMERIC,
                  "amg: non-positive eigenvalue estimate on level " + std::to_string(l));
     L.work.alloc(4 * n);
@@ -925,6 +918,9 @@ void amg_setup(nalu_linsys* ls) {
                        Pt.v.p);
     L.P = spgemm(L.A, sv.p, Pt, s);
     L.R = transpose(L.P, s);
+    // A_c = R (A P); the other association, (R A) P, forms fewer partial
+    // products but its ~500-column R A rows load the LDS hash tables to half
+    // full and measured 2.2x slower
     Csr AP = spgemm(L.A, L.A.v.p, L.P, s);
     Level next;
     next.A = spgemm(L.R, L.R.v.p, AP, s);

@@ -111,7 +111,10 @@ def test_hierarchy_matches_oracle(drop_tol):
         assert sparse_maxdiff(Pg, Po) <= 1e-12 * np.abs(Po.data).max()
         Ac = (Pg.T @ (A @ Pg)).tocsr()
         Gc = level(ls, l + 1)
-        assert sparse_maxdiff(Gc, Ac) <= 1e-12 * np.abs(Ac.data).max()
+        # against the non-cancelling magnitude |P|^T |A| |P|: the Galerkin product
+        # of a singular operator can cancel to round-off (a one-aggregate level)
+        mag = (abs(Pg).T @ (abs(A) @ abs(Pg))).max()
+        assert sparse_maxdiff(Gc, Ac) <= 1e-12 * mag
         B = Bc
     Ac = level(ls, L - 1)
     lam = M.power_lmax(Ac, prm.cheb_eig_iters)
