@@ -391,6 +391,19 @@ def momentum_3rhs(args, na, torch, mesh, P, devc, stream, md):
         out["iters_" + prec] = its
         out["true_rel_resid_" + prec] = rel
         sols[prec] = X3.clone()
+        # the same three rhs as one stacked block-diagonal system (one matrix
+        # read per SpMV / SGS sweep for the three components)
+        XS = torch.zeros_like(rhs3)
+
+        def solve_stacked():
+            XS.zero_()
+            info["s"] = ls.solve_stacked(rhs3, XS)
+
+        out["stacked_solve_s_" + prec] = timed(torch, solve_stacked, reps=3)
+        st_s, its_s, rel_s = info["s"]
+        out["stacked_iters_" + prec] = its_s
+        out["stacked_true_rel_resid_" + prec] = rel_s
+        out["stacked_vs_separate_rel_l2_" + prec] = float(torch.linalg.norm(XS - X3) / torch.linalg.norm(X3))
         csr[prec] = (ls.csr(), rhs3.cpu().numpy())
     err = out["errors_vs_fp64"]
     (_, _, vA, _), bA = csr["fp64"]
@@ -401,6 +414,11 @@ def momentum_3rhs(args, na, torch, mesh, P, devc, stream, md):
         err["delta_%s_rel_l2" % c] = float(torch.linalg.norm(sols["mixed"][i] - sols["fp64"][i]) /
                                            torch.linalg.norm(sols["fp64"][i]))
     out["speedup_mixed_vs_fp64"] = out["solve_s_fp64"] / out["solve_s_mixed"]
+    out["speedup_stacked_vs_separate_fp64"] = out["solve_s_fp64"] / out["stacked_solve_s_fp64"]
+    out["speedup_stacked_mixed_vs_separate_fp64"] = out["solve_s_fp64"] / out["stacked_solve_s_mixed"]
+    print("bench: momentum 3 rhs stacked: fp64 %.4f s %d it., mixed %.4f s %d it." %
+          (out["stacked_solve_s_fp64"], out["stacked_iters_fp64"], out["stacked_solve_s_mixed"],
+           out["stacked_iters_mixed"]), file=sys.stderr, flush=True)
     print("bench: momentum 3 rhs: fp64 %.4f s %s, mixed %.4f s %s" % (out["solve_s_fp64"], out["iters_fp64"],
                                                                      out["solve_s_mixed"], out["iters_mixed"]),
           file=sys.stderr, flush=True)

@@ -201,6 +201,15 @@ int nalu_linsys_solve_rhs(nalu_linsys *ls, const double *b_dev, double *x_dev, i
  * Returns the worst status (0 converged, 1 iteration cap). */
 int nalu_linsys_solve_multi(nalu_linsys *ls, int k, const double *B_dev, double *X_dev, int *iters,
                             double *rel);
+/* The same right-hand sides solved as one block-diagonal stacked system
+ * diag(A, ..., A) [X_0; ...; X_{k-1}] = [B_0; ...; B_{k-1}] in a single Krylov
+ * space: every SpMV and SGS sweep reads the matrix once for all k vectors
+ * (batched kernels for k <= 4), and convergence is measured on the stacked
+ * residual norm -- the way the reference measures its numDof 3 momentum
+ * system over all three components.  iters: Krylov iterations of the one
+ * solve; rel [k] (optional): each right-hand side's FP64 true relative
+ * residual.  No counterpart in the reference (its momentum system is coupled). */
+int nalu_linsys_solve_stacked(nalu_linsys *ls, int k, const double *B, double *X, int *iters, double *rel);
 
 /* ------------------------------------------------------ assembly kernels */
 /* Each fused kernel performs, for every element: gather, Hex8 SCS geometry,

@@ -127,6 +127,7 @@ def lib():
         L.nalu_assemble_momentum_mass_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
                                                              _dp, C.c_int, C.c_int, _vp]
         L.nalu_linsys_solve_multi.argtypes = [_vp, C.c_int, _vp, _vp, _i32p, _dp]
+        L.nalu_linsys_solve_stacked.argtypes = [_vp, C.c_int, _vp, _vp, C.POINTER(C.c_int), _dp]
         L.nalu_gnn_train_epochs.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                             C.c_uint64, _dp]
         L.nalu_compute_dual_volume.argtypes = [_vp, _vp, _vp]
@@ -373,6 +374,16 @@ class LinearSystem:
         rel = (C.c_double * k)()
         st = _check(lib().nalu_linsys_solve_multi(self._h, k, _ptr(B), _ptr(X), its, rel), "solve_multi")
         return st, list(its), list(rel)
+
+    def solve_stacked(self, B, X):
+        """The rows of B [k][n] solved as one block-diagonal stacked system in one
+        Krylov space (one matrix read per SpMV / SGS sweep for all k).  Returns
+        (status, Krylov iterations, FP64 true relative residual per rhs)."""
+        k = int(B.shape[0])
+        it = C.c_int(0)
+        rel = (C.c_double * k)()
+        st = _check(lib().nalu_linsys_solve_stacked(self._h, k, _ptr(B), _ptr(X), C.byref(it), rel), "solve_stacked")
+        return st, it.value, list(rel)
 
     def solve_rhs(self, b, x):
         it = C.c_int(0)

@@ -233,6 +233,8 @@ void precond_setup(nalu_linsys* ls);
 void precond_apply(nalu_linsys* ls, const double* r, double* z);
 void spmv(nalu_linsys* ls, const double* x, double* y);
 int krylov_solve(nalu_linsys* ls, const double* b, double* x, int* iters, double* final_rel_resid);
+int krylov_solve_k(nalu_linsys* ls, int nrhs, const double* b, double* x, int* iters, double* final_rel_resid,
+                   double* rel_each);
 double norm2(nalu_linsys* ls, const double* x, int64_t n);
 // solver-order (color-permuted) building blocks, krylov.hip
 void perm_sgs32(nalu_linsys* ls, const float* r, float* z);

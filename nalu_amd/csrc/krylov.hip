@@ -734,6 +734,95 @@ __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restri
   z[row] = z[row] - s * invd[row];
 }
 
+// Batched forms for NR right-hand sides stacked [NR][n] (one matrix read for
+// all of them): SpMV and the two sweeps, with the layouts of the kernels above.
+template <typename T, typename TY, int NR>
+__global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __restrict__ ch, int64_t nch,
+                                                          const int32_t* __restrict__ lcol, const T* __restrict__ lval,
+                                                          const int32_t* __restrict__ ucol, const T* __restrict__ uval,
+                                                          const T* __restrict__ dv, const T* __restrict__ x,
+                                                          TY* __restrict__ y, const TY* __restrict__ b, int mode,
+                                                          int64_t n) {
+  const int64_t c = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= nch) return;
+  const SellChunk k = ch[c];
+  const bool act = lane < k.nrows;
+  const int row = k.row0 + (act ? lane : 0);
+  T s[NR];
+  const T d = dv[row];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) s[r] = d * x[r * n + row];
+  const int32_t* lc = lcol + k.loff + lane;
+  const T* lv = lval + k.loff + lane;
+#pragma unroll 2
+  for (int q = 0; q < k.lw; ++q) {
+    const T a = __builtin_nontemporal_load(&lv[q * 64]);
+    const int64_t j = __builtin_nontemporal_load(&lc[q * 64]);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[r] += a * x[r * n + j];
+  }
+  const int32_t* uc = ucol + k.uoff + lane;
+  const T* uv = uval + k.uoff + lane;
+#pragma unroll 2
+  for (int q = 0; q < k.uw; ++q) {
+    const T a = __builtin_nontemporal_load(&uv[q * 64]);
+    const int64_t j = __builtin_nontemporal_load(&uc[q * 64]);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[r] += a * x[r * n + j];
+  }
+  if (!act) return;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) y[r * n + row] = mode ? (TY)((T)b[r * n + row] - s[r]) : (TY)s[r];
+}
+
+// UPPER = false: forward sweep z = (r - L z) / d; true: backward z -= (U z) / d
+template <typename T, int NR, bool UPPER>
+__global__ __launch_bounds__(kThreads) void k_sell_sweep_m(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
+                                                           const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                           const T* __restrict__ invd, const T* __restrict__ rr,
+                                                           T* __restrict__ z, int64_t n) {
+  __shared__ T part[NR][kThreads / 2];
+  const int64_t nb = (c1 - c0 + 1) / 2;  // blocks of two chunks, XCD-contiguous ranges
+  const int64_t pos = xcd_block(blockIdx.x, nb);
+  const int64_t c = pos < 0 ? c1 : c0 + pos * (kThreads / 128) + (threadIdx.x >> 7);
+  const int lane = threadIdx.x & 63;
+  const int half = (threadIdx.x >> 6) & 1;
+  const int slot = (threadIdx.x >> 7) * 64 + lane;
+  const bool ok = c < c1;
+  SellChunk k{};
+  if (ok) k = ch[c];
+  const bool act = ok && lane < k.nrows;
+  T s[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) s[r] = T(0);
+  if (act) {
+    const int w = UPPER ? k.uw : k.lw;
+    const int32_t* cc = col + (UPPER ? k.uoff : k.loff) + lane;
+    const T* vv = val + (UPPER ? k.uoff : k.loff) + lane;
+#pragma unroll 2
+    for (int q = half; q < w; q += 2) {
+      const T a = __builtin_nontemporal_load(&vv[q * 64]);
+      const int64_t j = __builtin_nontemporal_load(&cc[q * 64]);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) s[r] += a * z[r * n + j];
+    }
+  }
+  if (half == 1)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) part[r][slot] = s[r];
+  __syncthreads();
+  if (!act || half == 1) return;
+  const int row = k.row0 + lane;
+  const T di = invd[row];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const T t = s[r] + part[r][slot];
+    if (UPPER) z[r * n + row] = z[r * n + row] - t * di;
+    else z[r * n + row] = (rr[r * n + row] - t) * di;
+  }
+}
+
 // Sparse kernel family used by the solvers: 64 = SELL-64 (default), 0 =
 // CSR-stream (LDS staged), 4 / 8 = G lanes per CSR row.
 // NALU_SPARSE_KERNEL=sell|stream|vector4|vector8 overrides.
@@ -981,6 +1070,66 @@ void psgs(nalu_linsys* ls, const T* r, T* z) {
   }
 }
 
+// Operators on nrhs stacked vectors [nrhs][n] (solver order): the batched
+// SELL kernels for up to 4 right-hand sides, else one slice at a time.
+template <typename T, typename TY, int NR>
+void spmv_nr(nalu_linsys* ls, const T* x, TY* y, const TY* b, int mode) {
+  const Split<T> A = split_of<T>(ls);
+  const int64_t nch = ls->sell_nchunks;
+  hipLaunchKernelGGL((k_sell_spmv_m<T, TY, NR>), dim3((unsigned)std::max<int64_t>(1, (nch + 3) / 4)), dim3(kThreads),
+                     0, ls->stream, reinterpret_cast<const SellChunk*>(ls->sell_chunks.p), nch, ls->sl_col.p,
+                     sell_vals<T>(ls, false), ls->su_col.p, sell_vals<T>(ls, true), A.dv, x, y, b, mode,
+                     (int64_t)ls->nrows);
+}
+
+template <typename T, typename TY>
+void spmv_m(nalu_linsys* ls, int nrhs, const T* x, TY* y, const TY* b, int mode) {
+  const int64_t n = ls->nrows;
+  if (nrhs > 1 && nrhs <= 4 && sparse_kernel_choice() == 64) {
+    if (nrhs == 2) spmv_nr<T, TY, 2>(ls, x, y, b, mode);
+    else if (nrhs == 3) spmv_nr<T, TY, 3>(ls, x, y, b, mode);
+    else spmv_nr<T, TY, 4>(ls, x, y, b, mode);
+    return;
+  }
+  for (int r = 0; r < nrhs; ++r) pspmv<T, TY>(ls, x + r * n, y + r * n, b ? b + r * n : b, mode);
+}
+
+template <typename T, int NR>
+void sgs_nr(nalu_linsys* ls, const T* r, T* z) {
+  const Split<T> A = split_of<T>(ls);
+  hipStream_t s = ls->stream;
+  const int64_t n = ls->nrows;
+  const SellChunk* ch = reinterpret_cast<const SellChunk*>(ls->sell_chunks.p);
+  const auto& ccp = ls->sell_color_ptr;
+  for (int c = 0; c < ls->num_colors; ++c) {
+    const int64_t c0 = ccp[c], c1 = ccp[c + 1];
+    if (c1 > c0)
+      hipLaunchKernelGGL((k_sell_sweep_m<T, NR, false>), dim3((unsigned)xcd_grid((c1 - c0 + 1) / 2)), dim3(kThreads),
+                         0, s, ch, c0, c1, ls->sl_col.p, sell_vals<T>(ls, false), A.invd, r, z, n);
+  }
+  for (int c = ls->num_colors - 2; c >= 0; --c) {
+    const int64_t c0 = ccp[c], c1 = ccp[c + 1];
+    if (c1 > c0)
+      hipLaunchKernelGGL((k_sell_sweep_m<T, NR, true>), dim3((unsigned)xcd_grid((c1 - c0 + 1) / 2)), dim3(kThreads),
+                         0, s, ch, c0, c1, ls->su_col.p, sell_vals<T>(ls, true), A.invd, (const T*)nullptr, z, n);
+  }
+}
+
+template <typename T>
+void pprec(nalu_linsys* ls, const T* r, T* z);
+
+template <typename T>
+void prec_m(nalu_linsys* ls, int nrhs, const T* r, T* z) {
+  const int64_t n = ls->nrows;
+  if (nrhs > 1 && nrhs <= 4 && ls->cfg.precond == NALU_PREC_SGS && sparse_kernel_choice() == 64) {
+    if (nrhs == 2) sgs_nr<T, 2>(ls, r, z);
+    else if (nrhs == 3) sgs_nr<T, 3>(ls, r, z);
+    else sgs_nr<T, 4>(ls, r, z);
+    return;
+  }
+  for (int q = 0; q < nrhs; ++q) pprec<T>(ls, r + q * n, z + q * n);
+}
+
 bool mixed(const nalu_linsys* ls) { return ls->cfg.precision == NALU_PRECISION_MIXED; }
 
 // preconditioner in the permuted space
@@ -1063,9 +1212,9 @@ struct PinnedDoubles {
 // flexible: store Z_k = M^-1 v_k (FGMRES).  x (FP64) holds the initial guess.
 template <typename TV>
 int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, double tol, bool flexible,
-            double* implicit_rel) {
+            double* implicit_rel, int nrhs = 1) {
   hipStream_t s = ls->stream;
-  const int64_t n = ls->nrows;
+  const int64_t n = ls->nrows * nrhs;  // nrhs > 1: the block-diagonal stacked system
   Red red(ls);
   // basis columns padded to a multiple of 64 entries (16-byte aligned columns)
   const int64_t ldv = (n + 63) / 64 * 64;
@@ -1106,7 +1255,7 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
   double r0 = -1.0, rel = 1.0;
   while (iters < maxit) {
     // r = b - A x in FP64, v0 = r / ||r||
-    pspmv<double, double>(ls, x, r64.p, b, 1);
+    spmv_m<double, double>(ls, nrhs, x, r64.p, b, 1);
     const double beta = std::sqrt(red.dot(r64.p, r64.p, n));
     if (r0 < 0) r0 = beta;
     if (r0 == 0.0 || beta <= tol * r0) {
@@ -1125,8 +1274,8 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
       TV* vk = V + (size_t)ldv * k;
       TV* vk1 = V + (size_t)ldv * (k + 1);
       TV* zk = flexible ? Z + (size_t)ldv * k : Z;
-      pprec<TV>(ls, vk, zk);
-      pspmv<TV, TV>(ls, zk, vk1, (const TV*)nullptr, 0);
+      prec_m<TV>(ls, nrhs, vk, zk);
+      spmv_m<TV, TV>(ls, nrhs, zk, vk1, (const TV*)nullptr, 0);
       const int cnt = k + 1;
       const size_t hbytes = sizeof(double) * (2 * (m + 2) + 1);
       hipLaunchKernelGGL(k_gs_dots<TV>, dim3(nb_dots), dim3(kThreads), lds_dots, s, V, ldv, cnt, always2 ? 0 : 1, vk1,
@@ -1187,7 +1336,7 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
       hipLaunchKernelGGL((k_axpy<TV, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, 1.0, tmp, x, n);
     } else {
       hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, V, ldv, k, hdev.p, tmp, n, 1.0);
-      pprec<TV>(ls, tmp, Z);
+      prec_m<TV>(ls, nrhs, tmp, Z);
       hipLaunchKernelGGL((k_axpy<TV, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, 1.0, Z, x, n);
     }
     NALU_HIP_CHECK(hipStreamSynchronize(s));  // hpin is reused by the next cycle
@@ -1201,16 +1350,16 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
 
 // Preconditioned CG in the permuted space; scalars stay on the device, one
 // host read per iteration for the convergence test.
-int cg(nalu_linsys* ls, const double* b, double* x, int maxit, double tol) {
+int cg(nalu_linsys* ls, const double* b, double* x, int maxit, double tol, int nrhs = 1) {
   hipStream_t s = ls->stream;
-  const int64_t n = ls->nrows;
+  const int64_t n = ls->nrows * nrhs;  // nrhs > 1: the block-diagonal stacked system
   Red red(ls);
   DevBuf<double> r(n), z(n), p(n), q(n), sc(8);
-  pspmv<double, double>(ls, x, r.p, b, 1);
+  spmv_m<double, double>(ls, nrhs, x, r.p, b, 1);
   const double r0 = std::sqrt(red.dot(r.p, r.p, n));
   if (r0 == 0.0) return 0;
   NALU_REQUIRE(std::isfinite(r0), NALU_ERR_NUMERIC, "cg: non-finite residual");
-  pprec<double>(ls, r.p, z.p);
+  prec_m<double>(ls, nrhs, r.p, z.p);
   red.dot_dev(r.p, z.p, n, sc.p + 0);  // rz (slot alternates 0/3)
   hipLaunchKernelGGL((k_copy<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, z.p, p.p, n);
   PinnedDoubles pin(1);
@@ -1218,7 +1367,7 @@ int cg(nalu_linsys* ls, const double* b, double* x, int maxit, double tol) {
   int cur = 0;
   while (it < maxit) {
     ++it;
-    pspmv<double, double>(ls, p.p, q.p, (const double*)nullptr, 0);
+    spmv_m<double, double>(ls, nrhs, p.p, q.p, (const double*)nullptr, 0);
     red.dot_dev(p.p, q.p, n, sc.p + 1);
     const unsigned nb = red_blocks(n);
     hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, x, r.p, p.p, q.p, sc.p + cur, sc.p + 1, n,
@@ -1229,7 +1378,7 @@ int cg(nalu_linsys* ls, const double* b, double* x, int maxit, double tol) {
     const double rn = std::sqrt(*pin.p);
     NALU_REQUIRE(std::isfinite(rn), NALU_ERR_NUMERIC, "cg: non-finite residual");
     if (rn <= tol * r0) break;
-    pprec<double>(ls, r.p, z.p);
+    prec_m<double>(ls, nrhs, r.p, z.p);
     const int nxt = cur == 0 ? 3 : 0;
     red.dot_dev(r.p, z.p, n, sc.p + nxt);
     hipLaunchKernelGGL(k_cg_p, dim3(nblocks(n)), dim3(kThreads), 0, s, p.p, z.p, sc.p + nxt, sc.p + cur, n);
@@ -1346,30 +1495,41 @@ void precond_apply(nalu_linsys* ls, const double* r, double* z) {
 }
 
 // Solve with the configured method; x holds the initial guess (original order).
-int krylov_solve(nalu_linsys* ls, const double* b, double* x, int* iters_out, double* rel_out) {
+// nrhs > 1: B and X are [nrhs][rows] and the block-diagonal stacked system is
+// solved in one Krylov space (the right-hand sides share every matrix read);
+// convergence is measured on the stacked residual norm, as the reference
+// measures a numDof 3 system over all of its components.  rel_each (optional,
+// [nrhs]): each right-hand side's FP64 true relative residual.
+int krylov_solve_k(nalu_linsys* ls, int nrhs, const double* b, double* x, int* iters_out, double* rel_out,
+                   double* rel_each) {
   const auto& c = ls->cfg;
   const double tol = c.final_outer_iter ? c.final_tolerance : c.tolerance;
   const int maxit = std::max(1, c.max_iterations);
   const int m = std::max(1, c.kspace);
-  const int64_t n = ls->nrows;
+  const int64_t n1 = ls->nrows;
+  const int64_t n = n1 * nrhs;
   hipStream_t s = ls->stream;
   Red red(ls);
   DevBuf<double> bp(n), xp(n), r(n);
-  hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, b, ls->perm.p, n, bp.p);
-  hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, x, ls->perm.p, n, xp.p);
+  for (int q = 0; q < nrhs; ++q) {
+    hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n1)), dim3(kThreads), 0, s, b + q * n1, ls->perm.p,
+                       n1, bp.p + q * n1);
+    hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n1)), dim3(kThreads), 0, s, x + q * n1, ls->perm.p,
+                       n1, xp.p + q * n1);
+  }
   const double bn = std::sqrt(red.dot(bp.p, bp.p, n));
   NALU_REQUIRE(std::isfinite(bn), NALU_ERR_NUMERIC, "solve: non-finite right-hand side");
   learned_set_fallback(ls, false);
   int iters = 0;
   if (!mixed(ls)) {
-    if (c.method == NALU_METHOD_CG) iters = cg(ls, bp.p, xp.p, maxit, tol);
-    else iters = gmres_t<double>(ls, bp.p, xp.p, m, maxit, tol, c.method == NALU_METHOD_FGMRES, nullptr);
+    if (c.method == NALU_METHOD_CG) iters = cg(ls, bp.p, xp.p, maxit, tol, nrhs);
+    else iters = gmres_t<double>(ls, bp.p, xp.p, m, maxit, tol, c.method == NALU_METHOD_FGMRES, nullptr, nrhs);
   } else {
     // FP64 iterative refinement around an FP32 inner (F)GMRES: r = b - A x in
     // FP64, d ~ A^-1 r in FP32 arithmetic with FP64 Arnoldi inner products and
     // least squares, x += d in FP64.
     DevBuf<double> d(n);
-    pspmv<double, double>(ls, xp.p, r.p, bp.p, 1);
+    spmv_m<double, double>(ls, nrhs, xp.p, r.p, bp.p, 1);
     double rn = std::sqrt(red.dot(r.p, r.p, n));
     const double target = tol * (bn > 0 ? bn : 1.0);
     int outer = 0;
@@ -1377,22 +1537,35 @@ int krylov_solve(nalu_linsys* ls, const double* b, double* x, int* iters_out, do
       ++outer;
       hipLaunchKernelGGL((k_fill<double>), dim3(nblocks(n)), dim3(kThreads), 0, s, d.p, n, 0.0);
       const double inner_tol = std::min(1e-1, std::max(1e-4, 0.5 * target / rn));
-      iters += gmres_t<float>(ls, r.p, d.p, m, maxit - iters, inner_tol, c.method == NALU_METHOD_FGMRES, nullptr);
+      iters += gmres_t<float>(ls, r.p, d.p, m, maxit - iters, inner_tol, c.method == NALU_METHOD_FGMRES, nullptr,
+                              nrhs);
       hipLaunchKernelGGL((k_axpy<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, 1.0, d.p, xp.p, n);
-      pspmv<double, double>(ls, xp.p, r.p, bp.p, 1);
+      spmv_m<double, double>(ls, nrhs, xp.p, r.p, bp.p, 1);
       rn = std::sqrt(red.dot(r.p, r.p, n));
       NALU_REQUIRE(std::isfinite(rn), NALU_ERR_NUMERIC, "mixed precision: non-finite residual");
     }
   }
   // the FP64 true residual decides convergence
-  pspmv<double, double>(ls, xp.p, r.p, bp.p, 1);
+  spmv_m<double, double>(ls, nrhs, xp.p, r.p, bp.p, 1);
   const double rn = std::sqrt(red.dot(r.p, r.p, n));
-  hipLaunchKernelGGL((k_scatter<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, xp.p, ls->perm.p, n, x);
+  if (rel_each)
+    for (int q = 0; q < nrhs; ++q) {
+      const double bq = std::sqrt(red.dot(bp.p + q * n1, bp.p + q * n1, n1));
+      const double rq = std::sqrt(red.dot(r.p + q * n1, r.p + q * n1, n1));
+      rel_each[q] = rq / (bq > 0 ? bq : 1.0);
+    }
+  for (int q = 0; q < nrhs; ++q)
+    hipLaunchKernelGGL((k_scatter<double, double>), dim3(nblocks(n1)), dim3(kThreads), 0, s, xp.p + q * n1,
+                       ls->perm.p, n1, x + q * n1);
   NALU_HIP_CHECK(hipStreamSynchronize(s));
   if (iters_out) *iters_out = iters;
   if (rel_out) *rel_out = rn / (bn > 0 ? bn : 1.0);
   ls->linear_residual = rn;
   return (rn <= tol * (bn > 0 ? bn : 1.0) * (1.0 + 1e-12) || bn == 0.0) ? NALU_OK : NALU_NOT_CONVERGED;
+}
+
+int krylov_solve(nalu_linsys* ls, const double* b, double* x, int* iters_out, double* rel_out) {
+  return krylov_solve_k(ls, 1, b, x, iters_out, rel_out, nullptr);
 }
 
 }  // namespace nalu

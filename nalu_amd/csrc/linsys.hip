@@ -392,6 +392,24 @@ extern "C" int nalu_linsys_solve_multi(nalu_linsys* ls, int k, const double* B, 
   });
 }
 
+extern "C" int nalu_linsys_solve_stacked(nalu_linsys* ls, int k, const double* B, double* X, int* iters,
+                                         double* rel) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized && B && X && k >= 1, NALU_ERR_ARG, "solve_stacked: bad arguments");
+    if (!ls->precond_ready) {
+      PhaseTimer t(ls, PH_PRECOND);
+      precond_setup(ls);
+    }
+    PhaseTimer t(ls, PH_SOLVE);
+    int it = 0;
+    double r = 0.0;
+    const int status = krylov_solve_k(ls, k, B, X, &it, &r, rel);
+    ls->iters = it;
+    if (iters) *iters = it;
+    return status;
+  });
+}
+
 extern "C" int nalu_linsys_iterations(const nalu_linsys* ls) { return ls ? ls->iters : -1; }
 extern "C" double nalu_linsys_linear_residual(const nalu_linsys* ls) { return ls ? ls->linear_residual : NAN; }
 extern "C" double nalu_linsys_nonlinear_residual(const nalu_linsys* ls) { return ls ? ls->nonlinear_residual : NAN; }
