@@ -6,7 +6,8 @@ three-right-hand-side form:
 
   for each Picard iteration:
     momentum:   mass + advection-diffusion with the current mdot, Dirichlet
-                velocity on the wall nodes, solve for du (3 rhs, one matrix), u += du
+                velocity on the wall nodes, solve for du (3 rhs, one matrix, one
+                stacked Krylov solve), u += du
     continuity: advection (projTimeScale = dt / g1) + density time derivative,
                 solve for dp (pure Neumann: zero-mean rhs), p += dp
     mdot:       recomputed from the new u and p with the previous Gjp
@@ -31,13 +32,19 @@ class LowMachStep:
     pressure_precond: "sgs", or "muelu" for the AMG preconditioner the reference
     configures for the continuity system (CG + MueLu in its regression inputs,
     e.g. reg_tests/test_files/elemClosedDomain/elemClosedDomain.i:15-25); with
-    "muelu" the FP64 pressure solve is CG, the mixed one FGMRES."""
+    "muelu" the FP64 pressure solve is CG, the mixed one FGMRES.
+    momentum_solve: "stacked" (the three components in one Krylov space,
+    converged on their stacked residual norm like the reference's coupled
+    momentum system) or "separate" (three solves, each to the tolerance)."""
 
     def __init__(self, mesh, u, p, rho, mu, dt, wall_nodes, wall_u, picard=2, tol=1e-8, precision="fp64",
-                 stream=None, pressure_precond="sgs", muelu_params=None):
+                 stream=None, pressure_precond="sgs", muelu_params=None, momentum_solve="stacked"):
         import torch
         self.torch = torch
         self.mesh, self.dt, self.picard, self.stream = mesh, float(dt), int(picard), stream
+        if momentum_solve not in ("stacked", "separate"):
+            raise NaluError("LowMachStep: momentum_solve is 'stacked' or 'separate'")
+        self.momentum_solve = momentum_solve
         self.g = (1.0, -1.0, 0.0)
         self.u, self.p, self.rho, self.mu = u.clone(), p.clone(), rho, mu
         self.wall = wall_nodes.long()
@@ -104,7 +111,10 @@ class LowMachStep:
             self.rhs3[:, self.wall] = (self.wall_u - self.u[self.wall]).T
             self.mom.loadComplete()
             self.du3.zero_()
-            st_m, its_m, rel_m = self.mom.solve_multi(self.rhs3, self.du3)
+            if self.momentum_solve == "stacked":
+                st_m, its_m, rel_m = self.mom.solve_stacked(self.rhs3, self.du3)
+            else:
+                st_m, its_m, rel_m = self.mom.solve_multi(self.rhs3, self.du3)
             self.u += self.du3.T
             # continuity
             self.cont.zeroSystem()
