@@ -88,6 +88,7 @@ Amg* amg_of(const nalu_linsys* ls) { return reinterpret_cast<Amg*>(ls->amg); }
 
 // ------------------------------------------------------------------ helpers
 void scan_excl(const int32_t* in, int32_t* out, int64_t n, hipStream_t s) {
+  NALU_REQUIRE(n < (int64_t)INT32_MAX, NALU_ERR_UNSUPPORTED, "amg: more than 2^31 items in a scan");
   size_t tmp = 0;
   NALU_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
   DevBuf<unsigned char> t(std::max<size_t>(tmp, 1));
@@ -560,6 +561,21 @@ __global__ __launch_bounds__(64) void k_spgemm(const int32_t* __restrict__ arp, 
   }
 }
 
+// 64-bit total of int32 counts (block sums, one atomic per block)
+__global__ void k_count_total(const int32_t* __restrict__ cnt, int64_t n, unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long sh[kT];
+  unsigned long long acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    acc += (unsigned long long)cnt[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(total, sh[0]);
+}
+
 __global__ void k_row_bound(const int32_t* __restrict__ arp, const int32_t* __restrict__ aci,
                             const int32_t* __restrict__ brp, int64_t n, int32_t* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -574,6 +590,7 @@ __global__ void k_row_bound(const int32_t* __restrict__ arp, const int32_t* __re
 }
 
 int32_t dev_max(const int32_t* a, int64_t n, hipStream_t s) {
+  NALU_REQUIRE(n > 0 && n < (int64_t)INT32_MAX, NALU_ERR_UNSUPPORTED, "amg: reduction size out of range");
   DevBuf<int32_t> mx(1);
   size_t tmp = 0;
   NALU_HIP_CHECK(hipcub::DeviceReduce::Max(nullptr, tmp, a, mx.p, (int)n, s));
@@ -617,6 +634,13 @@ Csr spgemm(const Csr& A, const double* av, const Csr& B, hipStream_t s) {
     while (H < 2 * bound && H < kMaxHash) H <<= 1;
   }
   const size_t lds = (size_t)H * (sizeof(double) + sizeof(int32_t));
+  {
+    DevBuf<unsigned long long> tot(1);
+    NALU_HIP_CHECK(hipMemsetAsync(tot.p, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_count_total, dim3(std::min<unsigned>(512, nbl(A.n))), dim3(kT), 0, s, cnt.p, A.n, tot.p);
+    NALU_REQUIRE(to_host(tot.p, s) < (unsigned long long)INT32_MAX, NALU_ERR_UNSUPPORTED,
+                 "amg: a product has more than 2^31 entries (32-bit row pointers)");
+  }
   scan_excl(cnt.p, C.rp.p, A.n + 1, s);
   C.nnz = to_host(C.rp.p + A.n, s);
   C.max_row = dev_max(cnt.p, A.n, s);
@@ -650,6 +674,7 @@ __global__ void k_tr_rows(const uint64_t* __restrict__ key, int64_t nnz, int64_t
 // R = P^T, rows sorted by column (deterministic): one radix sort of
 // (column, row) keys carrying the values
 Csr transpose(const Csr& P, hipStream_t s) {
+  NALU_REQUIRE(P.nnz < (int64_t)INT32_MAX, NALU_ERR_UNSUPPORTED, "amg: prolongator has more than 2^31 entries");
   Csr R;
   R.n = P.m;
   R.m = P.n;
@@ -824,6 +849,7 @@ void amg_release(nalu_linsys* ls) {
 
 void amg_setup(nalu_linsys* ls) {
   NALU_REQUIRE(ls->num_dof == 1, NALU_ERR_UNSUPPORTED, "amg preconditioner: scalar systems (numDof 1) only");
+  NALU_REQUIRE(ls->nrows > 0, NALU_ERR_STATE, "amg preconditioner: empty system");
   const nalu_muelu_params& pr = ls->muelu;
   hipStream_t s = ls->stream;
   NALU_HIP_CHECK(hipStreamSynchronize(s));
