@@ -168,6 +168,9 @@ struct nalu_linsys {
   nalu::DevBuf<double> sl_val, su_val;
   nalu::DevBuf<float> sl_val32, su_val32;
   nalu::DevBuf<double> pwork;  // permuted in/out vectors of precond_apply
+  // layout of the vectors of a stacked multi-rhs solve while it runs: false =
+  // [nrhs][rows] slices, true = interleaved [rows][nrhs] (batched kernels)
+  bool stack_interleaved = false;
 
   // learned preconditioner (optional)
   void* learned = nullptr;

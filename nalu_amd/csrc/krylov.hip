@@ -457,6 +457,22 @@ __global__ void k_gather(const TA* __restrict__ x, const int32_t* __restrict__ p
   if (k < n) xp[k] = (TB)x[perm[k]];
 }
 
+// interleaved forms: out[k * stride] = x[perm[k]], x[perm[k]] = in[k * stride]
+__global__ void k_gather_strided(const double* __restrict__ x, const int32_t* __restrict__ perm, int64_t n,
+                                 int stride, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k * stride] = x[perm[k]];
+}
+__global__ void k_scatter_strided(const double* __restrict__ in, const int32_t* __restrict__ perm, int64_t n,
+                                  int stride, double* __restrict__ x) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[perm[k]] = in[k * stride];
+}
+__global__ void k_take_strided(const double* __restrict__ in, int64_t n, int stride, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[k * stride];
+}
+
 template <typename TA, typename TB>
 __global__ void k_scatter(const TA* __restrict__ xp, const int32_t* __restrict__ perm, int64_t n, TB* __restrict__ x) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -734,8 +750,9 @@ __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restri
   z[row] = z[row] - s * invd[row];
 }
 
-// Batched forms for NR right-hand sides stacked [NR][n] (one matrix read for
-// all of them): SpMV and the two sweeps, with the layouts of the kernels above.
+// Batched forms for NR right-hand sides interleaved [n][NR] (one matrix read
+// and one contiguous gather of NR values per entry): SpMV and the two sweeps,
+// with the matrix layouts of the kernels above.
 template <typename T, typename TY, int NR>
 __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __restrict__ ch, int64_t nch,
                                                           const int32_t* __restrict__ lcol, const T* __restrict__ lval,
@@ -752,7 +769,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __res
   T s[NR];
   const T d = dv[row];
 #pragma unroll
-  for (int r = 0; r < NR; ++r) s[r] = d * x[r * n + row];
+  for (int r = 0; r < NR; ++r) s[r] = d * x[row * NR + r];
   const int32_t* lc = lcol + k.loff + lane;
   const T* lv = lval + k.loff + lane;
 #pragma unroll 2
@@ -760,7 +777,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __res
     const T a = __builtin_nontemporal_load(&lv[q * 64]);
     const int64_t j = __builtin_nontemporal_load(&lc[q * 64]);
 #pragma unroll
-    for (int r = 0; r < NR; ++r) s[r] += a * x[r * n + j];
+    for (int r = 0; r < NR; ++r) s[r] += a * x[j * NR + r];
   }
   const int32_t* uc = ucol + k.uoff + lane;
   const T* uv = uval + k.uoff + lane;
@@ -769,11 +786,11 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __res
     const T a = __builtin_nontemporal_load(&uv[q * 64]);
     const int64_t j = __builtin_nontemporal_load(&uc[q * 64]);
 #pragma unroll
-    for (int r = 0; r < NR; ++r) s[r] += a * x[r * n + j];
+    for (int r = 0; r < NR; ++r) s[r] += a * x[j * NR + r];
   }
   if (!act) return;
 #pragma unroll
-  for (int r = 0; r < NR; ++r) y[r * n + row] = mode ? (TY)((T)b[r * n + row] - s[r]) : (TY)s[r];
+  for (int r = 0; r < NR; ++r) y[row * NR + r] = mode ? (TY)((T)b[row * NR + r] - s[r]) : (TY)s[r];
 }
 
 // UPPER = false: forward sweep z = (r - L z) / d; true: backward z -= (U z) / d
@@ -805,7 +822,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_sweep_m(const SellChunk* __re
       const T a = __builtin_nontemporal_load(&vv[q * 64]);
       const int64_t j = __builtin_nontemporal_load(&cc[q * 64]);
 #pragma unroll
-      for (int r = 0; r < NR; ++r) s[r] += a * z[r * n + j];
+      for (int r = 0; r < NR; ++r) s[r] += a * z[j * NR + r];
     }
   }
   if (half == 1)
@@ -818,8 +835,8 @@ __global__ __launch_bounds__(kThreads) void k_sell_sweep_m(const SellChunk* __re
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const T t = s[r] + part[r][slot];
-    if (UPPER) z[r * n + row] = z[r * n + row] - t * di;
-    else z[r * n + row] = (rr[r * n + row] - t) * di;
+    if (UPPER) z[row * NR + r] = z[row * NR + r] - t * di;
+    else z[row * NR + r] = (rr[row * NR + r] - t) * di;
   }
 }
 
@@ -1082,10 +1099,16 @@ void spmv_nr(nalu_linsys* ls, const T* x, TY* y, const TY* b, int mode) {
                      (int64_t)ls->nrows);
 }
 
+// The stacked vectors are interleaved [rows][nrhs] exactly when the batched
+// kernels serve both operators (stacked_interleaved()), [nrhs][rows] otherwise.
+bool stacked_interleaved(const nalu_linsys* ls, int nrhs) {
+  return nrhs > 1 && nrhs <= 4 && sparse_kernel_choice() == 64 && ls->cfg.precond == NALU_PREC_SGS;
+}
+
 template <typename T, typename TY>
 void spmv_m(nalu_linsys* ls, int nrhs, const T* x, TY* y, const TY* b, int mode) {
   const int64_t n = ls->nrows;
-  if (nrhs > 1 && nrhs <= 4 && sparse_kernel_choice() == 64) {
+  if (nrhs > 1 && ls->stack_interleaved) {
     if (nrhs == 2) spmv_nr<T, TY, 2>(ls, x, y, b, mode);
     else if (nrhs == 3) spmv_nr<T, TY, 3>(ls, x, y, b, mode);
     else spmv_nr<T, TY, 4>(ls, x, y, b, mode);
@@ -1121,7 +1144,7 @@ void pprec(nalu_linsys* ls, const T* r, T* z);
 template <typename T>
 void prec_m(nalu_linsys* ls, int nrhs, const T* r, T* z) {
   const int64_t n = ls->nrows;
-  if (nrhs > 1 && nrhs <= 4 && ls->cfg.precond == NALU_PREC_SGS && sparse_kernel_choice() == 64) {
+  if (nrhs > 1 && ls->stack_interleaved) {
     if (nrhs == 2) sgs_nr<T, 2>(ls, r, z);
     else if (nrhs == 3) sgs_nr<T, 3>(ls, r, z);
     else sgs_nr<T, 4>(ls, r, z);
@@ -1511,11 +1534,24 @@ int krylov_solve_k(nalu_linsys* ls, int nrhs, const double* b, double* x, int* i
   hipStream_t s = ls->stream;
   Red red(ls);
   DevBuf<double> bp(n), xp(n), r(n);
+  const bool inter = stacked_interleaved(ls, nrhs);
+  struct LayoutReset {  // the flag describes this solve's vectors only
+    nalu_linsys* ls;
+    ~LayoutReset() { ls->stack_interleaved = false; }
+  } reset{ls};
+  ls->stack_interleaved = inter;
   for (int q = 0; q < nrhs; ++q) {
-    hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n1)), dim3(kThreads), 0, s, b + q * n1, ls->perm.p,
-                       n1, bp.p + q * n1);
-    hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n1)), dim3(kThreads), 0, s, x + q * n1, ls->perm.p,
-                       n1, xp.p + q * n1);
+    if (inter) {
+      hipLaunchKernelGGL(k_gather_strided, dim3(nblocks(n1)), dim3(kThreads), 0, s, b + q * n1, ls->perm.p, n1,
+                         nrhs, bp.p + q);
+      hipLaunchKernelGGL(k_gather_strided, dim3(nblocks(n1)), dim3(kThreads), 0, s, x + q * n1, ls->perm.p, n1,
+                         nrhs, xp.p + q);
+    } else {
+      hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n1)), dim3(kThreads), 0, s, b + q * n1,
+                         ls->perm.p, n1, bp.p + q * n1);
+      hipLaunchKernelGGL((k_gather<double, double>), dim3(nblocks(n1)), dim3(kThreads), 0, s, x + q * n1,
+                         ls->perm.p, n1, xp.p + q * n1);
+    }
   }
   const double bn = std::sqrt(red.dot(bp.p, bp.p, n));
   NALU_REQUIRE(std::isfinite(bn), NALU_ERR_NUMERIC, "solve: non-finite right-hand side");
@@ -1548,15 +1584,29 @@ int krylov_solve_k(nalu_linsys* ls, int nrhs, const double* b, double* x, int* i
   // the FP64 true residual decides convergence
   spmv_m<double, double>(ls, nrhs, xp.p, r.p, bp.p, 1);
   const double rn = std::sqrt(red.dot(r.p, r.p, n));
-  if (rel_each)
+  if (rel_each) {
+    DevBuf<double> sl(inter ? 2 * n1 : 0);
     for (int q = 0; q < nrhs; ++q) {
-      const double bq = std::sqrt(red.dot(bp.p + q * n1, bp.p + q * n1, n1));
-      const double rq = std::sqrt(red.dot(r.p + q * n1, r.p + q * n1, n1));
+      const double *bq_p = bp.p + q * n1, *rq_p = r.p + q * n1;
+      if (inter) {  // contiguous copies of component q
+        hipLaunchKernelGGL(k_take_strided, dim3(nblocks(n1)), dim3(kThreads), 0, s, bp.p + q, n1, nrhs, sl.p);
+        hipLaunchKernelGGL(k_take_strided, dim3(nblocks(n1)), dim3(kThreads), 0, s, r.p + q, n1, nrhs, sl.p + n1);
+        bq_p = sl.p;
+        rq_p = sl.p + n1;
+      }
+      const double bq = std::sqrt(red.dot(bq_p, bq_p, n1));
+      const double rq = std::sqrt(red.dot(rq_p, rq_p, n1));
       rel_each[q] = rq / (bq > 0 ? bq : 1.0);
     }
-  for (int q = 0; q < nrhs; ++q)
-    hipLaunchKernelGGL((k_scatter<double, double>), dim3(nblocks(n1)), dim3(kThreads), 0, s, xp.p + q * n1,
-                       ls->perm.p, n1, x + q * n1);
+  }
+  for (int q = 0; q < nrhs; ++q) {
+    if (inter)
+      hipLaunchKernelGGL(k_scatter_strided, dim3(nblocks(n1)), dim3(kThreads), 0, s, xp.p + q, ls->perm.p, n1, nrhs,
+                         x + q * n1);
+    else
+      hipLaunchKernelGGL((k_scatter<double, double>), dim3(nblocks(n1)), dim3(kThreads), 0, s, xp.p + q * n1,
+                         ls->perm.p, n1, x + q * n1);
+  }
   NALU_HIP_CHECK(hipStreamSynchronize(s));
   if (iters_out) *iters_out = iters;
   if (rel_out) *rel_out = rn / (bn > 0 ? bn : 1.0);
