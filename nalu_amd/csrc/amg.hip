@@ -714,57 +714,6 @@ __global__ void k_dense_fill(const int32_t* __restrict__ rp, const int32_t* __re
   for (int32_t q = rp[i]; q < rp[i + 1]; ++q) row[ci[q]] += v[q];
 }
 
-// pivot search, row swap, pivot-row scaling and the copy of column k for the
-// elimination; piv[k] = 0 marks a skipped (singular) pivot.  One block.
-__global__ void k_gj_pivot(double* __restrict__ M, int64_t n, int64_t k, double tiny, double* __restrict__ fcol,
-                           int8_t* __restrict__ piv) {
-  __shared__ double bv[kT];
-  __shared__ int64_t bi[kT];
-  __shared__ double pivot;
-  const int64_t w = 2 * n;
-  double best = -1.0;
-  int64_t arg = k;
-  for (int64_t i = k + threadIdx.x; i < n; i += kT) {
-    const double a = fabs(M[i * w + k]);
-    if (a > best) {
-      best = a;
-      arg = i;
-    }
-  }
-  bv[threadIdx.x] = best;
-  bi[threadIdx.x] = arg;
-  __syncthreads();
-  for (int s = kT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const double o = bv[threadIdx.x + s];
-      const int64_t oi = bi[threadIdx.x + s];
-      if (o > bv[threadIdx.x] || (o == bv[threadIdx.x] && oi < bi[threadIdx.x])) {
-        bv[threadIdx.x] = o;
-        bi[threadIdx.x] = oi;
-      }
-    }
-    __syncthreads();
-  }
-  const int64_t p = bi[0];
-  const bool ok = bv[0] > tiny;
-  if (ok && p != k)
-    for (int64_t j = threadIdx.x; j < w; j += kT) {
-      const double t = M[k * w + j];
-      M[k * w + j] = M[p * w + j];
-      M[p * w + j] = t;
-    }
-  __syncthreads();
-  if (threadIdx.x == 0) pivot = M[k * w + k];  // read once: the scaling below overwrites it
-  __syncthreads();
-  if (ok) {
-    const double inv = 1.0 / pivot;
-    for (int64_t j = threadIdx.x; j < w; j += kT) M[k * w + j] *= inv;
-  }
-  __syncthreads();
-  for (int64_t i = threadIdx.x; i < n; i += kT) fcol[i] = (ok && i != k) ? M[i * w + k] : 0.0;
-  if (threadIdx.x == 0) piv[k] = ok ? 1 : 0;
-}
-
 // M[:, :n] -= c y B^T (the residual rank-one part of a near-singular operator)
 __global__ void k_rank1_sub(double* __restrict__ M, const double* __restrict__ y, const double* __restrict__ B,
                             int64_t n, double c) {
@@ -774,13 +723,56 @@ __global__ void k_rank1_sub(double* __restrict__ M, const double* __restrict__ y
   M[i * 2 * n + j] -= c * y[i] * B[j];
 }
 
-__global__ void k_gj_elim(double* __restrict__ M, int64_t n, int64_t k, const double* __restrict__ fcol) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// One Gauss-Jordan step as a single launch, out of place (Min -> Mout, no
+// block reads what another writes): every block (one row each) finds the same
+// pivot p of column k (largest |value| in rows >= k, ties to the smaller row),
+// row k of Mout is row p / pivot, row p takes the old row k, and every other
+// row eliminates column k with the normalized pivot row.  A pivot at or below
+// tiny leaves Min unchanged in Mout and is recorded as skipped (piv[k] = 0).
+__global__ void k_gj_step(const double* __restrict__ Min, double* __restrict__ Mout, int64_t n, int64_t k,
+                          double tiny, int8_t* __restrict__ piv) {
+  __shared__ double bv[kT];
+  __shared__ int64_t bi[kT];
   const int64_t w = 2 * n;
-  if (t >= n * w) return;
-  const int64_t i = t / w, j = t % w;
-  const double f = fcol[i];
-  if (f != 0.0) M[i * w + j] -= f * M[k * w + j];
+  const int64_t i = blockIdx.x;
+  double best = -1.0;
+  int64_t arg = k;
+  for (int64_t r = k + threadIdx.x; r < n; r += kT) {
+    const double a = fabs(Min[r * w + k]);
+    if (a > best) {
+      best = a;
+      arg = r;
+    }
+  }
+  bv[threadIdx.x] = best;
+  bi[threadIdx.x] = arg;
+  __syncthreads();
+  for (int sft = kT / 2; sft > 0; sft >>= 1) {
+    if ((int)threadIdx.x < sft) {
+      const double o = bv[threadIdx.x + sft];
+      const int64_t oi = bi[threadIdx.x + sft];
+      if (o > bv[threadIdx.x] || (o == bv[threadIdx.x] && oi < bi[threadIdx.x])) {
+        bv[threadIdx.x] = o;
+        bi[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const int64_t p = bi[0];
+  const bool ok = bv[0] > tiny;
+  if (i == 0 && threadIdx.x == 0) piv[k] = ok ? 1 : 0;
+  if (!ok) {
+    for (int64_t j = threadIdx.x; j < w; j += kT) Mout[i * w + j] = Min[i * w + j];
+    return;
+  }
+  const double pv = Min[p * w + k];
+  if (i == k) {
+    for (int64_t j = threadIdx.x; j < w; j += kT) Mout[k * w + j] = Min[p * w + j] / pv;
+    return;
+  }
+  const int64_t src = (i == p) ? k : i;
+  const double f = Min[src * w + k];
+  for (int64_t j = threadIdx.x; j < w; j += kT) Mout[i * w + j] = Min[src * w + j] - f * (Min[p * w + j] / pv);
 }
 
 __global__ void k_gj_extract(const double* __restrict__ M, const int8_t* __restrict__ piv, int64_t n,
@@ -963,7 +955,7 @@ void amg_setup(nalu_linsys* ls) {
                "amg: coarsest level has " + std::to_string(nc) + " rows (dense solve limit 8192)");
   amg->nc = nc;
   {
-    DevBuf<double> M(2 * nc * nc), fcol(nc);
+    DevBuf<double> M(2 * nc * nc);
     DevBuf<int8_t> piv(nc);
     hipLaunchKernelGGL(k_dense_fill, dim3(nbl(nc)), dim3(kT), 0, s, Lc.A.rp.p, Lc.A.ci.p, Lc.A.v.p, nc, M.p);
     auto host_absmax = [&](const double* d, int64_t m) {
@@ -1000,13 +992,17 @@ void amg_setup(nalu_linsys* ls) {
       if (amg->deflated)
         hipLaunchKernelGGL(k_rank1_sub, dim3(nbl(nc * nc)), dim3(kT), 0, s, M.p, y.p, B.p, nc, 1.0 / bb);
     }
+    DevBuf<double> M2(2 * nc * nc);
+    double* cur = M.p;
+    double* nxt = M2.p;
     for (int64_t k = 0; k < nc; ++k) {
-      hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(kT), 0, s, M.p, nc, k, tiny, fcol.p, piv.p);
-      hipLaunchKernelGGL(k_gj_elim, dim3(nbl(2 * nc * nc)), dim3(kT), 0, s, M.p, nc, k, fcol.p);
+      hipLaunchKernelGGL(k_gj_step, dim3((unsigned)nc), dim3(kT), 0, s, cur, nxt, nc, k, tiny, piv.p);
+      std::swap(cur, nxt);
     }
+    NALU_HIP_CHECK(hipGetLastError());
     amg->cinv.alloc(nc * nc);
     amg->cinv32.alloc(nc * nc);
-    hipLaunchKernelGGL(k_gj_extract, dim3(nbl(nc * nc)), dim3(kT), 0, s, M.p, piv.p, nc, amg->cinv.p,
+    hipLaunchKernelGGL(k_gj_extract, dim3(nbl(nc * nc)), dim3(kT), 0, s, cur, piv.p, nc, amg->cinv.p,
                        amg->cinv32.p);
     std::vector<int8_t> ph(nc);
     NALU_HIP_CHECK(hipMemcpyAsync(ph.data(), piv.p, nc, hipMemcpyDeviceToHost, s));
