@@ -106,58 +106,11 @@ def train_network(args, na, torch, devc, Pt, tag=""):
     above 1e-3 of its start, so the error is known to 1e-3), train the network
     and keep the epoch whose mixed FGMRES(50) solve of the training mesh's own
     pressure system (its assembled rhs, not a training pair) takes the fewest
-    iterations -- the network quality swings from epoch to epoch at this
-    learning rate.  Returns (net, training record)."""
-    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
-    n_t = round(Pt["conn"].shape[0] ** (1.0 / 3.0))
-    mt = na.Mesh(Pt["coords"], Pt["conn"])
-    lt = na.LinearSystem(1, na.SolverConfig("cg", "sgs"))
-    lt.buildElemToNodeGraph(mt)
-    lt.finalizeLinearSystem()
-    lt.zeroSystem()
-    na.assemble_continuity(lt, mt, t(Pt["rho"]), t(Pt["p"]), t(Pt["u"]), t(Pt["gpdx"]), tau=1.0, mode="gather")
-    lt.loadComplete()
-    lt.precond_setup()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    pairs = na.HarvestedPairs(lt, num_rhs=400, pairs_per_rhs=5, seed=1, tol=1e-6)
-    torch.cuda.synchronize()
-    t_harvest = time.perf_counter() - t0
-    print("bench: %sharvested %d pairs at %d^3 in %.2f s" % (tag, len(pairs), n_t, t_harvest),
-          file=sys.stderr, flush=True)
-    lv = na.LinearSystem(1, na.SolverConfig("fgmres", "learned", tolerance=1e-8, max_iterations=600, kspace=50,
-                                            precision="mixed"))
-    lv.buildElemToNodeGraph(mt)
-    lv.finalizeLinearSystem()
-    lv.zeroSystem()
-    na.assemble_continuity(lv, mt, t(Pt["rho"]), t(Pt["p"]), t(Pt["u"]), t(Pt["gpdx"]), tau=1.0, mode="gather")
-    lv.loadComplete()
-    lv.removeRhsMean()
-    xv = torch.zeros(lv.sizes()[0], dtype=torch.float64, device=devc)
-
-    def validate(nn):
-        lv.set_learned_preconditioner(nn)
-        xv.zero_()
-        lv.solve(xv)
-        return lv.linearSolveIterations() + (10000 if lv.learned_fallback() else 0)
-
-    net = na.LearnedPreconditioner(num_layers=4, hidden=32, seed=2)
-    t0 = time.perf_counter()
-    hist = net.train(lt, pairs, epochs=args.train_epochs, batch=16, lr=3e-3, seed=1, validate=validate)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    print("bench: %svalidation iterations per epoch %s, kept epoch %d" % (tag, net.validation, net.best_epoch),
-          file=sys.stderr, flush=True)
-    print("bench: %strained %d epochs in %.2f s, losses %s" % (tag, args.train_epochs, t_train,
-                                                              np.round(hist, 4).tolist()),
-          file=sys.stderr, flush=True)
-    rec = {"train_mesh": "%d^3" % n_t, "pairs": len(pairs), "rhs": 400, "pairs_per_rhs": 5, "cg_tol": 1e-6,
-           "validation_fgmres_iters_per_epoch": net.validation, "kept_epoch": net.best_epoch,
-           "harvest_s": t_harvest, "harvest_cg_iters_mean": float(np.mean(pairs.cg_iterations())),
-           "train_s": t_train, "epochs": args.train_epochs, "batch": 16, "epoch_loss": hist.tolist(),
-           "layers": 4, "hidden": 32, "weights": "bf16", "accumulate": "fp32"}
-    del pairs, lv, lt, mt
-    return net, rec
+    iterations (nalu_amd.training).  Returns (net, training record)."""
+    from nalu_amd.training import train_pressure_preconditioner
+    return train_pressure_preconditioner(
+        Pt, num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=args.train_epochs, batch=16, lr=3e-3, seed=1,
+        net_seed=2, device=devc, log=lambda m: print("bench: %s%s" % (tag, m), file=sys.stderr, flush=True))
 
 
 def solver_compare(args, na, torch, mesh, P, devc):

@@ -172,16 +172,32 @@ def _check(rc, what=""):
     return rc
 
 
-def _ptr(t):
-    """Device pointer of a torch CUDA tensor (or an int address)."""
+_DTYPE_NAMES = {"f64": "float64", "f32": "float32", "i32": "int32"}
+
+
+def _dev(t, what, dt, numel=None, shape=None, optional=False):
+    """Device pointer of a torch tensor after the checks the kernels rely on:
+    element type dt ("f64", "f32" or "i32"), the number of entries (numel) or
+    the exact shape, contiguous layout, device memory.  Every wrapper validates
+    its arrays here before any launch, so that a caller mistake raises
+    NaluError instead of letting a kernel index past a buffer."""
     if t is None:
-        return None
+        if optional:
+            return None
+        raise NaluError("%s: array missing" % what)
     if isinstance(t, int):
-        return t
-    if not t.is_cuda:
-        raise NaluError("expected a device (cuda) tensor")
+        raise NaluError("%s: a torch tensor is expected, not a raw address" % what)
+    name = getattr(getattr(t, "dtype", None), "__repr__", lambda: "")()
+    if not name.endswith(_DTYPE_NAMES[dt]):
+        raise NaluError("%s: dtype %s, expected torch.%s" % (what, getattr(t, "dtype", type(t)), _DTYPE_NAMES[dt]))
+    if shape is not None and tuple(t.shape) != tuple(int(s) for s in shape):
+        raise NaluError("%s: shape %s, expected %s" % (what, tuple(t.shape), tuple(int(s) for s in shape)))
+    if numel is not None and t.numel() != int(numel):
+        raise NaluError("%s: %d entries, expected %d" % (what, t.numel(), int(numel)))
     if not t.is_contiguous():
-        raise NaluError("expected a contiguous tensor")
+        raise NaluError("%s: expected a contiguous tensor" % what)
+    if not t.is_cuda:
+        raise NaluError("%s: expected a device (cuda) tensor" % what)
     return t.data_ptr()
 
 
@@ -224,6 +240,7 @@ class Mesh:
         self.num_nodes = coords.shape[0]
         self.num_elems = conn.shape[0]
         ids = None if node_ids is None else _np(node_ids, np.int64)
+        self.has_ids = ids is not None
         h = _vp()
         _check(lib().nalu_mesh_create(self.num_nodes, self.num_elems, conn.ctypes.data_as(_i32p),
                                       coords.ctypes.data_as(_dp),
@@ -314,6 +331,18 @@ class LinearSystem:
             _L.nalu_linsys_destroy(h)
             self._h = None
 
+    def _rows(self):
+        """Rows of the finalized system (cached)."""
+        n = getattr(self, "_nrows", None)
+        if n is None:
+            n = self._nrows = self.sizes()[0]
+        return n
+
+    def _nodes(self):
+        if self.mesh is None:
+            raise NaluError("no graph was built")
+        return self.mesh.num_nodes
+
     # --- construction
     def buildElemToNodeGraph(self, mesh):
         self.mesh = mesh
@@ -337,18 +366,26 @@ class LinearSystem:
     def sumIntoBatch(self, lhs, rhs, num_elems=None):
         """Kokkos overload batched over the graph mesh's elements: lhs [E,R,R], rhs [E,R] device tensors."""
         _, conn = self.mesh.device_ptrs()
-        ne = self.mesh.num_elems if num_elems is None else num_elems
-        _check(lib().nalu_linsys_sum_into_batch(self._h, ne, conn, _ptr(lhs), _ptr(rhs)), "sumIntoBatch")
+        ne = self.mesh.num_elems if num_elems is None else int(num_elems)
+        if not 0 < ne <= self.mesh.num_elems:
+            raise NaluError("sumIntoBatch: num_elems must be in [1, %d]" % self.mesh.num_elems)
+        R = 8 * self.num_dof
+        _check(lib().nalu_linsys_sum_into_batch(self._h, ne, conn, _dev(lhs, "sumIntoBatch lhs", "f64", shape=(ne, R, R)),
+                                                _dev(rhs, "sumIntoBatch rhs", "f64", shape=(ne, R))), "sumIntoBatch")
 
     def applyDirichletBCs(self, bc_nodes, solution, bc_values, begin_pos=0, end_pos=None):
         end_pos = self.num_dof if end_pos is None else end_pos
-        _check(lib().nalu_linsys_apply_dirichlet(self._h, int(bc_nodes.numel()), _ptr(bc_nodes), _ptr(solution),
-                                                 _ptr(bc_values), begin_pos, end_pos), "applyDirichletBCs")
+        nf = self._nodes() * self.num_dof
+        _check(lib().nalu_linsys_apply_dirichlet(self._h, int(bc_nodes.numel()),
+                                                 _dev(bc_nodes, "applyDirichletBCs nodes", "i32"),
+                                                 _dev(solution, "applyDirichletBCs solution", "f64", numel=nf),
+                                                 _dev(bc_values, "applyDirichletBCs bc values", "f64", numel=nf),
+                                                 begin_pos, end_pos), "applyDirichletBCs")
 
     def resetRows(self, nodes, begin_pos=0, end_pos=None, diag_value=1.0, rhs_residual=0.0):
         end_pos = self.num_dof if end_pos is None else end_pos
-        _check(lib().nalu_linsys_reset_rows(self._h, int(nodes.numel()), _ptr(nodes), begin_pos, end_pos,
-                                            float(diag_value), float(rhs_residual)), "resetRows")
+        _check(lib().nalu_linsys_reset_rows(self._h, int(nodes.numel()), _dev(nodes, "resetRows nodes", "i32"),
+                                            begin_pos, end_pos, float(diag_value), float(rhs_residual)), "resetRows")
 
     def loadComplete(self):
         _check(lib().nalu_linsys_load_complete(self._h), "loadComplete")
@@ -364,31 +401,44 @@ class LinearSystem:
     # --- solve
     def solve(self, delta_out=None):
         """Returns 0 when converged, 1 when max_iterations was reached."""
-        return _check(lib().nalu_linsys_solve(self._h, _ptr(delta_out)), "solve")
+        d = _dev(delta_out, "solve delta", "f64", numel=self._rows(), optional=True)
+        return _check(lib().nalu_linsys_solve(self._h, d), "solve")
 
     def solve_multi(self, B, X):
         """k right-hand sides (rows of B [k][n]) sharing the matrix; X [k][n] holds the
         initial guesses.  Returns (worst status, iterations per rhs, rel. residual per rhs)."""
-        k = int(B.shape[0])
+        k = int(B.shape[0]) if B.dim() == 2 else 0
+        if k < 1:
+            raise NaluError("solve_multi: B must be [k][rows]")
+        shp = (k, self._rows())
         its = (C.c_int32 * k)()
         rel = (C.c_double * k)()
-        st = _check(lib().nalu_linsys_solve_multi(self._h, k, _ptr(B), _ptr(X), its, rel), "solve_multi")
+        st = _check(lib().nalu_linsys_solve_multi(self._h, k, _dev(B, "solve_multi B", "f64", shape=shp),
+                                                  _dev(X, "solve_multi X", "f64", shape=shp), its, rel), "solve_multi")
         return st, list(its), list(rel)
 
     def solve_stacked(self, B, X):
         """The rows of B [k][n] solved as one block-diagonal stacked system in one
         Krylov space (one matrix read per SpMV / SGS sweep for all k).  Returns
         (status, Krylov iterations, FP64 true relative residual per rhs)."""
-        k = int(B.shape[0])
+        k = int(B.shape[0]) if B.dim() == 2 else 0
+        if k < 1:
+            raise NaluError("solve_stacked: B must be [k][rows]")
+        shp = (k, self._rows())
         it = C.c_int(0)
         rel = (C.c_double * k)()
-        st = _check(lib().nalu_linsys_solve_stacked(self._h, k, _ptr(B), _ptr(X), C.byref(it), rel), "solve_stacked")
+        st = _check(lib().nalu_linsys_solve_stacked(self._h, k, _dev(B, "solve_stacked B", "f64", shape=shp),
+                                                    _dev(X, "solve_stacked X", "f64", shape=shp), C.byref(it), rel),
+                    "solve_stacked")
         return st, it.value, list(rel)
 
     def solve_rhs(self, b, x):
         it = C.c_int(0)
         rel = C.c_double(0)
-        st = _check(lib().nalu_linsys_solve_rhs(self._h, _ptr(b), _ptr(x), C.byref(it), C.byref(rel)), "solve_rhs")
+        n = self._rows()
+        st = _check(lib().nalu_linsys_solve_rhs(self._h, _dev(b, "solve_rhs b", "f64", numel=n),
+                                                _dev(x, "solve_rhs x", "f64", numel=n), C.byref(it), C.byref(rel)),
+                    "solve_rhs")
         return st, it.value, rel.value
 
     def linearSolveIterations(self):
@@ -441,16 +491,22 @@ class LinearSystem:
         return dict(zip(["init", "zero", "assemble", "dirichlet", "load_complete", "precond", "solve"], t))
 
     def spmv(self, x, y):
-        _check(lib().nalu_linsys_spmv(self._h, _ptr(x), _ptr(y)), "spmv")
+        n = self._rows()
+        _check(lib().nalu_linsys_spmv(self._h, _dev(x, "spmv x", "f64", numel=n), _dev(y, "spmv y", "f64", numel=n)),
+               "spmv")
 
     def spmv_f32(self, x, y):
-        _check(lib().nalu_linsys_spmv_f32(self._h, _ptr(x), _ptr(y)), "spmv_f32")
+        n = self._rows()
+        _check(lib().nalu_linsys_spmv_f32(self._h, _dev(x, "spmv_f32 x", "f32", numel=n),
+                                          _dev(y, "spmv_f32 y", "f32", numel=n)), "spmv_f32")
 
     def precond_setup(self):
         _check(lib().nalu_linsys_precond_setup(self._h), "precond_setup")
 
     def precond_apply(self, r, z):
-        _check(lib().nalu_linsys_precond_apply(self._h, _ptr(r), _ptr(z)), "precond_apply")
+        n = self._rows()
+        _check(lib().nalu_linsys_precond_apply(self._h, _dev(r, "precond_apply r", "f64", numel=n),
+                                               _dev(z, "precond_apply z", "f64", numel=n)), "precond_apply")
 
     def permutation(self):
         """Solver row order: new row k is original row perm[k]."""
@@ -499,14 +555,23 @@ class LinearSystem:
         return agg
 
     def precond_apply_permuted(self, r, z):
-        _check(lib().nalu_linsys_precond_apply_permuted(self._h, _ptr(r), _ptr(z)), "precond_apply_permuted")
+        n = self._rows()
+        _check(lib().nalu_linsys_precond_apply_permuted(self._h, _dev(r, "precond_apply_permuted r", "f64", numel=n),
+                                                        _dev(z, "precond_apply_permuted z", "f64", numel=n)),
+               "precond_apply_permuted")
 
     def precond_apply_permuted_f32(self, r, z):
         """M^-1 r in FP32 (float32 device tensors, solver order)."""
-        _check(lib().nalu_linsys_precond_apply_permuted_f32(self._h, _ptr(r), _ptr(z)), "precond_apply_permuted_f32")
+        n = self._rows()
+        _check(lib().nalu_linsys_precond_apply_permuted_f32(self._h,
+                                                            _dev(r, "precond_apply_permuted_f32 r", "f32", numel=n),
+                                                            _dev(z, "precond_apply_permuted_f32 z", "f32", numel=n)),
+               "precond_apply_permuted_f32")
 
     def spmv_permuted(self, x, y):
-        _check(lib().nalu_linsys_spmv_permuted(self._h, _ptr(x), _ptr(y)), "spmv_permuted")
+        n = self._rows()
+        _check(lib().nalu_linsys_spmv_permuted(self._h, _dev(x, "spmv_permuted x", "f64", numel=n),
+                                               _dev(y, "spmv_permuted y", "f64", numel=n)), "spmv_permuted")
 
     def set_l2_scaling(self, s):
         _check(lib().nalu_linsys_set_l2_scaling(self._h, float(s)), "set_l2_scaling")
@@ -554,7 +619,11 @@ class HarvestedPairs:
 
     def pair(self, idx, r, e):
         """Copy pair idx (original row order) into float32 device tensors r, e."""
-        _check(lib().nalu_gnn_data_pair(self._h, int(idx), _ptr(r), _ptr(e)), "gnn_data_pair")
+        n = self.ls._rows()
+        if not 0 <= int(idx) < len(self):
+            raise NaluError("pair: index out of range")
+        _check(lib().nalu_gnn_data_pair(self._h, int(idx), _dev(r, "pair r", "f32", numel=n),
+                                        _dev(e, "pair e", "f32", numel=n)), "gnn_data_pair")
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -593,14 +662,20 @@ class LearnedPreconditioner:
 
     def apply(self, ls, r, z):
         """z = network(r) for float32 device tensors in the original row order."""
-        _check(lib().nalu_gnn_apply(self._h, ls.handle, _ptr(r), _ptr(z)), "gnn_apply")
+        n = ls._rows()
+        _check(lib().nalu_gnn_apply(self._h, ls.handle, _dev(r, "gnn apply r", "f32", numel=n),
+                                    _dev(z, "gnn apply z", "f32", numel=n)), "gnn_apply")
 
     def loss_grad(self, ls, r, e, zs=None, out=None):
         """Loss and parameter gradient of one batch (r, e: float32 device tensors [B, n])."""
         batch = int(r.shape[0]) if r.dim() == 2 else 1
+        nb = batch * ls._rows()
         loss = C.c_double(0)
         g = np.zeros(self.num_params(), dtype=np.float32)
-        _check(lib().nalu_gnn_loss_grad(self._h, ls.handle, batch, _ptr(r), _ptr(e), _ptr(zs), _ptr(out),
+        _check(lib().nalu_gnn_loss_grad(self._h, ls.handle, batch, _dev(r, "loss_grad r", "f32", numel=nb),
+                                        _dev(e, "loss_grad e", "f32", numel=nb),
+                                        _dev(zs, "loss_grad zs", "f32", numel=nb, optional=True),
+                                        _dev(out, "loss_grad out", "f32", numel=nb, optional=True),
                                         C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))), "gnn_loss_grad")
         return loss.value, g
 
@@ -644,7 +719,9 @@ MODE = {"gather": 0, "atomic": 1, "tile": 2}
 
 
 def assemble_scalar_diff(ls, mesh, q, k, shifted=False, mode="tile"):
-    _check(lib().nalu_assemble_scalar_diff(ls.handle, mesh.handle, _ptr(q), _ptr(k), int(shifted), MODE[mode]),
+    N = mesh.num_nodes
+    _check(lib().nalu_assemble_scalar_diff(ls.handle, mesh.handle, _dev(q, "scalar_diff q", "f64", numel=N),
+                                           _dev(k, "scalar_diff k", "f64", numel=N), int(shifted), MODE[mode]),
            "assemble_scalar_diff")
 
 
@@ -653,37 +730,50 @@ def compute_mdot(mesh, rho, p, u, gpdx, mdot, tau=1.0, shift_mdot=False, shift_p
     """mdot at the 12 scs of every element; precision "fp32" evaluates with FP32
     arithmetic (FP64 Jacobians), as the mixed-precision path does."""
     fn = lib().nalu_compute_mdot_f32 if precision == "fp32" else lib().nalu_compute_mdot
-    _check(fn(mesh.handle, _ptr(rho), _ptr(p), _ptr(u), _ptr(gpdx), float(tau), int(shift_mdot), int(shift_poisson),
-              _ptr(mdot), stream), "compute_mdot")
+    N, E = mesh.num_nodes, mesh.num_elems
+    _check(fn(mesh.handle, _dev(rho, "mdot rho", "f64", numel=N), _dev(p, "mdot p", "f64", numel=N),
+              _dev(u, "mdot u", "f64", numel=3 * N), _dev(gpdx, "mdot Gpdx", "f64", numel=3 * N), float(tau),
+              int(shift_mdot), int(shift_poisson), _dev(mdot, "mdot", "f64", numel=12 * E), stream), "compute_mdot")
 
 
 def compute_dual_volume(mesh, dual_vol, stream=None):
     """Dual nodal volume [N] (sum of the scv volumes around each node)."""
-    _check(lib().nalu_compute_dual_volume(mesh.handle, _ptr(dual_vol), stream), "compute_dual_volume")
+    _check(lib().nalu_compute_dual_volume(mesh.handle, _dev(dual_vol, "dual volume", "f64", numel=mesh.num_nodes),
+                                          stream), "compute_dual_volume")
 
 
 def compute_nodal_gradient(mesh, q, dual_vol, dqdx, shifted=False, stream=None):
     """Interior Green-Gauss nodal gradient [N][3] of the nodal field q."""
-    _check(lib().nalu_compute_nodal_gradient(mesh.handle, _ptr(q), _ptr(dual_vol), int(shifted), _ptr(dqdx), stream),
+    N = mesh.num_nodes
+    _check(lib().nalu_compute_nodal_gradient(mesh.handle, _dev(q, "nodal gradient q", "f64", numel=N),
+                                             _dev(dual_vol, "nodal gradient dual volume", "f64", numel=N), int(shifted),
+                                             _dev(dqdx, "nodal gradient dqdx", "f64", numel=3 * N), stream),
            "compute_nodal_gradient")
 
 
 def assemble_continuity(ls, mesh, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False,
                         reduced_sens=False, mode="tile"):
-    _check(lib().nalu_assemble_continuity(ls.handle, mesh.handle, _ptr(rho), _ptr(p), _ptr(u), _ptr(gpdx),
+    N = mesh.num_nodes
+    _check(lib().nalu_assemble_continuity(ls.handle, mesh.handle, _dev(rho, "continuity rho", "f64", numel=N),
+                                          _dev(p, "continuity p", "f64", numel=N),
+                                          _dev(u, "continuity u", "f64", numel=3 * N),
+                                          _dev(gpdx, "continuity Gpdx", "f64", numel=3 * N),
                                           float(tau), int(shift_mdot), int(shift_poisson), int(reduced_sens),
                                           MODE[mode]), "assemble_continuity")
 
 
 def assemble_momentum(ls, mesh, u, mu, mdot, include_divu=0.0, skew=False, shifted=False, mode="tile"):
-    _check(lib().nalu_assemble_momentum(ls.handle, mesh.handle, _ptr(u), _ptr(mu), _ptr(mdot), float(include_divu),
+    N, E = mesh.num_nodes, mesh.num_elems
+    _check(lib().nalu_assemble_momentum(ls.handle, mesh.handle, _dev(u, "momentum u", "f64", numel=3 * N),
+                                        _dev(mu, "momentum mu", "f64", numel=N),
+                                        _dev(mdot, "momentum mdot", "f64", numel=12 * E), float(include_divu),
                                         int(skew), int(shifted), MODE[mode]), "assemble_momentum")
 
 
-def _states(fields, name):
+def _states(fields, name, numel):
     if len(fields) != 3:
         raise NaluError("%s: three states (Nm1, N, Np1) expected" % name)
-    return [_ptr(f) for f in fields]
+    return [_dev(f, "%s state %d" % (name, i), "f64", numel=numel) for i, f in enumerate(fields)]
 
 
 def _gammas(gamma):
@@ -695,8 +785,10 @@ def _gammas(gamma):
 
 def assemble_momentum_mass(ls, mesh, rho, u, gjp, dt, gamma=(1.0, -1.0, 0.0), lumped=False, balanced_force=False):
     """MomentumMassElemKernel: rho = (rhoNm1, rhoN, rhoNp1), u = (uNm1, uN, uNp1), nodal device arrays."""
-    r, v, g = _states(rho, "momentum_mass rho"), _states(u, "momentum_mass u"), _gammas(gamma)
-    _check(lib().nalu_assemble_momentum_mass(ls.handle, mesh.handle, r[0], r[1], r[2], v[0], v[1], v[2], _ptr(gjp),
+    N = mesh.num_nodes
+    r, v, g = _states(rho, "momentum_mass rho", N), _states(u, "momentum_mass u", 3 * N), _gammas(gamma)
+    _check(lib().nalu_assemble_momentum_mass(ls.handle, mesh.handle, r[0], r[1], r[2], v[0], v[1], v[2],
+                                             _dev(gjp, "momentum_mass Gjp", "f64", numel=3 * N),
                                              float(dt), g.ctypes.data_as(_dp), int(lumped), int(balanced_force)),
            "assemble_momentum_mass")
 
@@ -704,22 +796,28 @@ def assemble_momentum_mass(ls, mesh, rho, u, gjp, dt, gamma=(1.0, -1.0, 0.0), lu
 def assemble_momentum_segregated(ls, mesh, u, mu, mdot, rhs3, include_divu=0.0, skew=False, shifted=False):
     """Momentum as 3 right-hand sides sharing one scalar matrix (numDof 1 system);
     rhs3 [3][N] (device, row order) is added the residual of each component."""
-    _check(lib().nalu_assemble_momentum_segregated(ls.handle, mesh.handle, _ptr(u), _ptr(mu), _ptr(mdot),
-                                                   float(include_divu), int(skew), int(shifted), _ptr(rhs3)),
+    N, E = mesh.num_nodes, mesh.num_elems
+    _check(lib().nalu_assemble_momentum_segregated(ls.handle, mesh.handle, _dev(u, "momentum u", "f64", numel=3 * N),
+                                                   _dev(mu, "momentum mu", "f64", numel=N),
+                                                   _dev(mdot, "momentum mdot", "f64", numel=12 * E),
+                                                   float(include_divu), int(skew), int(shifted),
+                                                   _dev(rhs3, "momentum rhs3", "f64", shape=(3, ls._rows()))),
            "assemble_momentum_segregated")
 
 
 def assemble_momentum_mass_segregated(ls, mesh, rho, u, gjp, dt, rhs3, gamma=(1.0, -1.0, 0.0), lumped=False,
                                       balanced_force=False):
-    r, v, g = _states(rho, "momentum_mass rho"), _states(u, "momentum_mass u"), _gammas(gamma)
+    N = mesh.num_nodes
+    r, v, g = _states(rho, "momentum_mass rho", N), _states(u, "momentum_mass u", 3 * N), _gammas(gamma)
     _check(lib().nalu_assemble_momentum_mass_segregated(ls.handle, mesh.handle, r[0], r[1], r[2], v[0], v[1], v[2],
-                                                        _ptr(gjp), float(dt), g.ctypes.data_as(_dp), int(lumped),
-                                                        int(balanced_force), _ptr(rhs3)),
+                                                        _dev(gjp, "momentum_mass Gjp", "f64", numel=3 * N), float(dt),
+                                                        g.ctypes.data_as(_dp), int(lumped), int(balanced_force),
+                                                        _dev(rhs3, "momentum rhs3", "f64", shape=(3, ls._rows()))),
            "assemble_momentum_mass_segregated")
 
 
 def assemble_continuity_mass(ls, mesh, rho, dt, gamma=(1.0, -1.0, 0.0), lumped=False):
     """ContinuityMassElemKernel: rho = (rhoNm1, rhoN, rhoNp1), nodal device arrays."""
-    r, g = _states(rho, "continuity_mass rho"), _gammas(gamma)
+    r, g = _states(rho, "continuity_mass rho", mesh.num_nodes), _gammas(gamma)
     _check(lib().nalu_assemble_continuity_mass(ls.handle, mesh.handle, r[0], r[1], r[2], float(dt),
                                                g.ctypes.data_as(_dp), int(lumped)), "assemble_continuity_mass")

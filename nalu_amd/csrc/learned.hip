@@ -1628,6 +1628,8 @@ extern "C" int nalu_linsys_set_learned_precond(nalu_linsys* ls, nalu_gnn* g) {
     }
     nalu::check_net(g);
     NALU_REQUIRE(ls->num_dof == 1, NALU_ERR_UNSUPPORTED, "set_learned_precond: scalar systems only (numDof 1)");
+    NALU_REQUIRE(ls->cfg.method == NALU_METHOD_FGMRES && ls->cfg.precond == NALU_PREC_LEARNED, NALU_ERR_ARG,
+                 "set_learned_precond: the system must be configured with method fgmres and preconditioner learned");
     nalu::Learned* st = nalu::ensure_state(ls);
     st->net = g;
     st->fallback = false;

@@ -116,6 +116,10 @@ extern "C" int nalu_linsys_create(int num_dof, const nalu_solver_config* cfg, na
                  "linsys_create: unknown preconditioner");
     NALU_REQUIRE(cfg->precision == NALU_PRECISION_FP64 || cfg->precision == NALU_PRECISION_MIXED, NALU_ERR_ARG,
                  "linsys_create: unknown precision");
+    // the network is a nonlinear operator: only flexible GMRES admits a
+    // preconditioner that changes from one application to the next
+    NALU_REQUIRE(cfg->precond != NALU_PREC_LEARNED || cfg->method == NALU_METHOD_FGMRES, NALU_ERR_ARG,
+                 "linsys_create: the learned preconditioner needs method fgmres");
     auto* ls = new nalu_linsys();
     ls->num_dof = num_dof;
     ls->cfg = *cfg;

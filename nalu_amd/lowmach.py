@@ -41,6 +41,9 @@ class LowMachStep:
                  stream=None, pressure_precond="sgs", muelu_params=None, momentum_solve="stacked"):
         import torch
         self.torch = torch
+        if getattr(mesh, "has_ids", False):
+            # rhs3 / du3 are in row order, which equals node order only without explicit ids
+            raise NaluError("LowMachStep: meshes with explicit node ids are not supported (rows would not be nodes)")
         self.mesh, self.dt, self.picard, self.stream = mesh, float(dt), int(picard), stream
         if momentum_solve not in ("stacked", "separate"):
             raise NaluError("LowMachStep: momentum_solve is 'stacked' or 'separate'")

@@ -87,3 +87,34 @@ def test_product_does_not_import_oracle():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
                 txt = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "pyoracle" not in txt and "liboracle" not in txt and "or_graph_build" not in txt, f
+
+
+def test_learned_preconditioner_requires_fgmres():
+    # a nonlinear preconditioner breaks CG and non-flexible GMRES recurrences
+    for method in ("gmres", "cg"):
+        with pytest.raises(na.NaluError, match="fgmres"):
+            na.LinearSystem(1, na.SolverConfig(method, "learned"))
+    na.LinearSystem(1, na.SolverConfig("fgmres", "learned"))
+
+
+def test_device_array_validation_without_gpu():
+    torch = pytest.importorskip("torch")
+    ok = torch.zeros(6, dtype=torch.float64)
+    with pytest.raises(na.NaluError, match="dtype"):
+        na._dev(torch.zeros(6, dtype=torch.float32), "x", "f64", numel=6)
+    with pytest.raises(na.NaluError, match="entries"):
+        na._dev(ok, "x", "f64", numel=7)
+    with pytest.raises(na.NaluError, match="shape"):
+        na._dev(ok, "x", "f64", shape=(2, 4))
+    with pytest.raises(na.NaluError, match="contiguous"):
+        na._dev(torch.zeros((4, 6), dtype=torch.float64).t(), "x", "f64", numel=24)
+    with pytest.raises(na.NaluError, match="int32"):
+        na._dev(torch.zeros(3, dtype=torch.int64), "nodes", "i32")
+    with pytest.raises(na.NaluError, match="missing"):
+        na._dev(None, "x", "f64")
+    assert na._dev(None, "x", "f64", optional=True) is None
+    with pytest.raises(na.NaluError, match="raw address"):
+        na._dev(12345, "x", "f64")
+    # a well-formed host tensor passes every check but the device one
+    with pytest.raises(na.NaluError, match="device"):
+        na._dev(ok.reshape(2, 3), "x", "f64", shape=(2, 3))

@@ -1,0 +1,372 @@
+"""The five benchmark configurations at their stated sizes, on the GPU.
+
+1. heat, 32^3: whole-mesh parity with the oracle (graph bit-exact, entries
+   within 1e-12 of their row's largest entry) and GMRES(50)+SGS to 1e-8;
+2. low-Mach 128^3, FP64: mdot and continuity against the whole-mesh oracle,
+   momentum (numDof 3, 513.6 M entries) against the oracle on >= 4096 sampled
+   rows including corner / edge / face rows, graph nnz = 9 (3n+1)^3, the
+   momentum row structure against the numDof expansion of the oracle graph,
+   CG+SGS pressure solve to 1e-8;
+3. the same case in mixed precision: per-field errors of the FP32-arithmetic
+   assembly against the FP64 oracle (mdot, continuity matrix / rhs, the
+   segregated momentum matrix and its three right-hand sides on sampled rows),
+   mixed pressure solve and the 3-rhs momentum solve to 1e-8;
+4. learned preconditioner: 2000 pairs harvested at 64^3 (density
+   lognormal(0, 0.3)), trained, evaluated inside mixed FGMRES on 128^3 to the
+   1e-8 FP64 true residual without falling back;
+5. stretched 256^3 (ratio 1.05 from every face, aspect ratio 100, jitter
+   0.05h, density lognormal(0, 0.3)): FP32-vs-oracle assembly errors on the
+   most stretched elements, mixed FGMRES + AMG and mixed FGMRES + learned solves
+   to the 1e-8 FP64 true residual, fallback recorded.
+
+Every true residual is recomputed on the host with the oracle's SpMV.  The
+sampled-row oracle is tests/sampling.py (pinned against the whole-mesh oracle
+by tests/test_sampling.py).
+"""
+import time
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import nalu_amd as na  # noqa: E402
+from nalu_amd.problems import heat_problem, lowmach_problem  # noqa: E402
+from tests.fixtures import rel_err_rowwise  # noqa: E402
+from tests.sampling import compare_rows, node_incidence, oracle_rows, sample_nodes  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+TOL = 1e-12
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def system(mesh, nd, cfg):
+    ls = na.LinearSystem(nd, cfg)
+    ls.buildElemToNodeGraph(mesh)
+    ls.finalizeLinearSystem()
+    return ls
+
+
+def true_resid(oracle, ls, b, x):
+    rp, ci, v, _ = ls.csr()
+    b = b.cpu().numpy() if hasattr(b, "cpu") else b
+    x = x.cpu().numpy() if hasattr(x, "cpu") else x
+    return float(np.linalg.norm(b - oracle.spmv(rp, ci, v, x)) / np.linalg.norm(b))
+
+
+# ----------------------------------------------------------------- config 1
+@pytest.mark.timeout(300)
+def test_config1_heat_32(oracle):
+    n = 32
+    say("case: config 1, heat %d^3, GMRES(50)+SGS" % n)
+    P = heat_problem(n)
+    mesh = na.Mesh(P["coords"], P["conn"])
+    ls = system(mesh, 1, na.SolverConfig("gmres", "sgs", tolerance=1e-8, kspace=50, max_iterations=2000))
+    T, k = dev(P["T"]), dev(P["k"])
+    ls.zeroSystem()
+    na.assemble_scalar_diff(ls, mesh, T, k)
+    ls.applyDirichletBCs(dev(P["bc_nodes"], torch.int32), T, dev(P["bc_values"]))
+    ls.loadComplete()
+    rp, ci, v, r = ls.csr()
+    g = oracle.graph_build(P["coords"].shape[0], P["conn"])
+    ov, orr = oracle.assemble_scalar_diff(g, P["conn"], P["coords"], P["T"], P["k"])
+    oracle.apply_dirichlet(g, ov, orr, P["bc_nodes"], P["T"], P["bc_values"])
+    np.testing.assert_array_equal(rp, g.row_ptr)
+    np.testing.assert_array_equal(ci, g.col_idx)
+    assert g.nnz == (3 * n + 1) ** 3
+    err = rel_err_rowwise(v, ov, g.row_ptr)
+    say("  values row-relative error %.2e" % err)
+    assert err <= TOL
+    np.testing.assert_allclose(r, orr, rtol=0, atol=TOL * np.abs(orr).max())
+    dT = torch.zeros(len(r), dtype=torch.float64, device=DEV)
+    st = ls.solve(dT)
+    res = true_resid(oracle, ls, orr, dT)
+    xo, ito, _ = oracle.gmres(g.row_ptr, g.col_idx, ov, orr, kspace=50, maxit=3000, tol=1e-10, precond="sgs")
+    dx = float(np.linalg.norm(dT.cpu().numpy() - xo) / np.linalg.norm(xo))
+    say("  status %d, %d iterations (oracle sequential SGS: %d), true resid %.2e, |x - x_oracle| %.1e"
+        % (st, ls.linearSolveIterations(), ito, res, dx))
+    assert st == 0 and res <= 1e-8 and dx <= 1e-6
+
+
+# ----------------------------------------------------------------- configs 2 and 3: shared 128^3 inputs
+@pytest.fixture(scope="module")
+def lm128(oracle):
+    n = 128
+    t0 = time.time()
+    P = lowmach_problem(n, seed=4321)
+    nn = P["coords"].shape[0]
+    say("setup: low-Mach %d^3 inputs and oracle (mdot, graph, continuity) ..." % n)
+    om = oracle.compute_mdot(P["conn"], P["coords"], P["rho"], P["p"], P["u"], P["gpdx"], P["tau"])
+    g1 = oracle.graph_build(nn, P["conn"], 1)
+    ov, orr = oracle.assemble_continuity(g1, P["conn"], P["coords"], P["rho"], P["p"], P["u"], P["gpdx"], P["tau"])
+    inc = node_incidence(P["conn"], nn)
+    nodes = sample_nodes(n, 1500, seed=11)
+    say("setup: oracle done in %.1f s, %d sampled nodes" % (time.time() - t0, nodes.size))
+    return dict(n=n, P=P, om=om, g1=g1, ov=ov, orr=orr, inc=inc, nodes=nodes, mesh=na.Mesh(P["coords"], P["conn"]))
+
+
+def fields(P):
+    return dev(P["rho"]), dev(P["p"]), dev(P["u"]), dev(P["gpdx"]), dev(P["mu"])
+
+
+def momentum_expect(oracle, L, mdot):
+    P, conn, X = L["P"], L["P"]["conn"], L["P"]["coords"]
+    return oracle_rows(L["nodes"], L["inc"], conn, 3, lambda e: oracle.momentum_advdiff_elem(
+        X[conn[e]], P["u"][conn[e]], P["mu"][conn[e]], mdot[e]))
+
+
+@pytest.mark.timeout(900)
+def test_config2_lowmach_128_fp64(oracle, lm128):
+    L = lm128
+    n, P, mesh = L["n"], L["P"], L["mesh"]
+    say("case: config 2, low-Mach %d^3 FP64: mdot, continuity, momentum, CG+SGS" % n)
+    E = P["conn"].shape[0]
+    rho, p, u, gp, mu = fields(P)
+    mdot = torch.zeros((E, 12), dtype=torch.float64, device=DEV)
+    na.compute_mdot(mesh, rho, p, u, gp, mdot, tau=P["tau"])
+    md = mdot.cpu().numpy()
+    err_m = float(np.abs(md - L["om"]).max() / np.abs(L["om"]).max())
+    say("  mdot error %.2e" % err_m)
+    assert err_m <= TOL
+    cont = system(mesh, 1, na.SolverConfig("cg", "sgs", tolerance=1e-8, max_iterations=5000))
+    cont.zeroSystem()
+    na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode="gather")
+    cont.loadComplete()
+    rp, ci, v, r = cont.csr()
+    g1 = L["g1"]
+    np.testing.assert_array_equal(rp, g1.row_ptr)
+    np.testing.assert_array_equal(ci, g1.col_idx)
+    assert g1.nnz == (3 * n + 1) ** 3
+    err_c = rel_err_rowwise(v, L["ov"], g1.row_ptr)
+    err_cr = float(np.abs(r - L["orr"]).max() / np.abs(L["orr"]).max())
+    say("  continuity: graph bit-exact, values %.2e, rhs %.2e" % (err_c, err_cr))
+    assert err_c <= TOL and err_cr <= TOL
+    del v
+    # momentum, numDof 3: structure against the numDof expansion of the oracle graph, values on sampled rows
+    mom = system(mesh, 3, na.SolverConfig("gmres", "sgs"))
+    mom.zeroSystem()
+    na.assemble_momentum(mom, mesh, u, mu, mdot, mode="tile")
+    mom.loadComplete()
+    rp3, ci3, v3, r3 = mom.csr()
+    assert int(rp3[-1]) == 9 * (3 * n + 1) ** 3
+    L1 = np.diff(g1.row_ptr)
+    exp_rp = np.zeros(3 * len(L1) + 1, dtype=np.int64)
+    exp_rp[1:] = np.cumsum(np.repeat(3 * L1, 3))
+    np.testing.assert_array_equal(rp3, exp_rp)
+    exp = momentum_expect(oracle, L, md)
+    err3, err3r, k = compare_rows(exp, rp3, ci3, v3, r3)
+    scale = np.abs(r3).max()
+    say("  momentum: %d sampled rows, values %.2e, rhs %.2e (of max |rhs|)" % (k, err3, err3r / scale))
+    assert k >= 4096 and err3 <= TOL and err3r <= TOL * scale
+    del rp3, ci3, v3, r3, mom
+    # pressure-Poisson: CG + SGS on the zero-mean rhs
+    cont.removeRhsMean()
+    _, _, _, rb = cont.csr()
+    x = torch.zeros(cont.sizes()[0], dtype=torch.float64, device=DEV)
+    st = cont.solve(x)
+    res = true_resid(oracle, cont, rb, x)
+    say("  CG+SGS: status %d, %d iterations, true resid %.2e" % (st, cont.linearSolveIterations(), res))
+    assert st == 0 and res <= 1e-8
+
+
+@pytest.mark.timeout(900)
+def test_config3_lowmach_128_mixed(oracle, lm128):
+    L = lm128
+    n, P, mesh = L["n"], L["P"], L["mesh"]
+    say("case: config 3, low-Mach %d^3 mixed precision" % n)
+    E, N = P["conn"].shape[0], P["coords"].shape[0]
+    rho, p, u, gp, mu = fields(P)
+    mdot = torch.zeros((E, 12), dtype=torch.float64, device=DEV)
+    na.compute_mdot(mesh, rho, p, u, gp, mdot, tau=P["tau"], precision="fp32")
+    md = mdot.cpu().numpy()
+    err_m = float(np.abs(md - L["om"]).max() / np.abs(L["om"]).max())
+    cont = system(mesh, 1, na.SolverConfig("fgmres", "sgs", tolerance=1e-8, max_iterations=5000, precision="mixed"))
+    cont.zeroSystem()
+    na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode="gather")
+    cont.loadComplete()
+    rp, ci, v, r = cont.csr()
+    err_c = rel_err_rowwise(v, L["ov"], L["g1"].row_ptr)
+    err_cr = float(np.abs(r - L["orr"]).max() / np.abs(L["orr"]).max())
+    say("  FP32 assembly vs FP64 oracle: mdot %.2e, continuity matrix %.2e (row-relative), rhs %.2e" %
+        (err_m, err_c, err_cr))
+    assert err_m <= 5e-5 and err_c <= 1e-6 and err_cr <= 1e-3
+    del v
+    # segregated momentum (one scalar matrix, three right-hand sides), one backward-Euler step
+    dt = 1.0 / n
+    seg = system(mesh, 1, na.SolverConfig("fgmres", "sgs", tolerance=1e-8, max_iterations=2000, kspace=50,
+                                          precision="mixed"))
+    rhs3 = torch.zeros((3, N), dtype=torch.float64, device=DEV)
+    seg.zeroSystem()
+    na.assemble_momentum_segregated(seg, mesh, u, mu, mdot, rhs3)
+    na.assemble_momentum_mass_segregated(seg, mesh, (rho, rho, rho), (u, u, u), gp, dt, rhs3)
+    seg.loadComplete()
+    srp, sci, sv, _ = seg.csr()
+    b3 = rhs3.cpu().numpy()
+    conn, X, U = P["conn"], P["coords"], P["u"]
+    gam = np.array([1.0, -1.0, 0.0])
+
+    def seg_elem(e):
+        # shared scalar matrix (trace of the coupled 3x3 blocks minus the diffusion
+        # operator) / 3, and the coupled residual as three right-hand sides
+        nd = conn[e]
+        la, ra = oracle.momentum_advdiff_elem(X[nd], U[nd], P["mu"][nd], L["om"][e])
+        lm, rm = oracle.momentum_mass_elem(X[nd], np.stack([P["rho"][nd]] * 3), np.stack([U[nd]] * 3),
+                                           P["gpdx"][nd], dt, gam)
+        lt = la + lm
+        dl, _ = oracle.scalar_diff_elem(X[nd], np.zeros(8), P["mu"][nd])
+        tr = sum(lt[i::3, i::3] for i in range(3))
+        return (tr - dl) / 3.0, (ra + rm).reshape(8, 3)
+
+    cache = {}
+    ptr, elems, local = L["inc"]
+    worst, worst_r = 0.0, np.zeros(3)
+    for a in L["nodes"]:
+        cols, rr = {}, np.zeros(3)
+        for q in range(ptr[a], ptr[a + 1]):
+            e, la = int(elems[q]), int(local[q])
+            if e not in cache:
+                cache[e] = seg_elem(e)
+            lhs, rhs = cache[e]
+            rr += rhs[la]
+            for c in range(8):
+                cols[int(conn[e, c])] = cols.get(int(conn[e, c]), 0.0) + lhs[la, c]
+        s, t = int(srp[a]), int(srp[a + 1])
+        ck = np.array(sorted(cols))
+        assert np.array_equal(sci[s:t], ck)
+        ev = np.array([cols[c] for c in ck])
+        worst = max(worst, float(np.abs(sv[s:t] - ev).max() / np.abs(ev).max()))
+        worst_r = np.maximum(worst_r, np.abs(b3[:, a] - rr))
+    err_r3 = worst_r / np.abs(b3).max(axis=1)
+    say("  segregated momentum vs oracle (%d sampled rows): matrix %.2e, rhs u/v/w %s" %
+        (L["nodes"].size, worst, np.array2string(err_r3, precision=2)))
+    assert worst <= 5e-5 and np.all(err_r3 <= 5e-4)
+    del sv
+    # solves to the FP64 1e-8 true residual: the pressure system and the three momentum right-hand sides
+    cont.removeRhsMean()
+    _, _, _, rb = cont.csr()
+    x = torch.zeros(N, dtype=torch.float64, device=DEV)
+    st = cont.solve(x)
+    res = true_resid(oracle, cont, rb, x)
+    say("  mixed FGMRES+SGS pressure: status %d, %d iterations, true resid %.2e" %
+        (st, cont.linearSolveIterations(), res))
+    assert st == 0 and res <= 1e-8
+    X3 = torch.zeros_like(rhs3)
+    st3, its3, rel3 = seg.solve_stacked(rhs3, X3)
+    res3 = [true_resid(oracle, seg, rhs3[i], X3[i]) for i in range(3)]
+    say("  mixed stacked 3-rhs momentum: status %d, %d iterations, true resid %s" %
+        (st3, its3, np.array2string(np.array(res3), precision=2)))
+    assert st3 == 0 and max(res3) <= 1e-8
+
+
+# ----------------------------------------------------------------- config 5
+@pytest.mark.timeout(1200)
+def test_config5_stretched_256(oracle):
+    from nalu_amd.training import train_pressure_preconditioner
+    n, n_t = 256, 64
+    say("case: config 5, stretched %d^3 (ratio 1.05, aspect 100, jitter 0.05h)" % n)
+    P = lowmach_problem(n, jitter=0.05, rho_sigma=0.3, seed=777, stretch=1.05, max_aspect=100.0)
+    X, conn = P["coords"], P["conn"]
+    hx = np.diff(X[:n + 1, 0])
+    aspect = float(hx.max() / hx.min())
+    assert abs(aspect - 100.0) < 1e-6
+    mesh = na.Mesh(X, conn)
+    rho, p, u, gp, _ = fields(P)
+    # FP32-arithmetic assembly against the FP64 oracle on the most stretched elements:
+    # nodes next to the middle of each wall (one short and two long edges) and near the corners
+    n1 = n + 1
+    m = n // 2
+    pts = []
+    for s in (1, 2, n - 2, n - 1):
+        for a in (m - 1, m, m + 1):
+            for b in (m - 1, m, m + 1):
+                pts += [(s, a, b), (a, s, b), (a, b, s)]
+    pts += [(i, j, k) for i in (1, n - 1) for j in (1, n - 1) for k in (1, n - 1)]
+    nodes = np.unique([i + n1 * (j + n1 * k) for i, j, k in pts]).astype(np.int64)
+    inc = node_incidence(conn, X.shape[0])
+    exp = oracle_rows(nodes, inc, conn, 1, lambda e: oracle.continuity_adv_elem(
+        X[conn[e]], P["rho"][conn[e]], P["p"][conn[e]], P["u"][conn[e]], P["gpdx"][conn[e]]))
+    cfg = lambda prec: na.SolverConfig("fgmres", prec, tolerance=1e-8, max_iterations=20000, kspace=50,
+                                       precision="mixed")
+    ls = system(mesh, 1, cfg("muelu"))
+    ls.set_muelu_params(na.MueLuParams(cheb_degree=2))
+    ls.zeroSystem()
+    na.assemble_continuity(ls, mesh, rho, p, u, gp, tau=1.0, mode="gather")
+    ls.loadComplete()
+    rp, ci, v, r = ls.csr()
+    err, err_r, k = compare_rows(exp, rp, ci, v, r)
+    rel_r = err_r / np.abs(r).max()
+    say("  FP32 assembly vs FP64 oracle on %d rows of aspect-%.0f elements: matrix %.2e, rhs %.2e" %
+        (k, aspect, err, rel_r))
+    assert err <= 1e-5 and rel_r <= 1e-3
+    ls.removeRhsMean()
+    _, _, _, rb = ls.csr()
+    del v
+    x = torch.zeros(ls.sizes()[0], dtype=torch.float64, device=DEV)
+    t0 = time.time()
+    st = ls.solve(x)
+    res = true_resid(oracle, ls, rb, x)
+    say("  mixed FGMRES + AMG: status %d, %d iterations, true resid %.2e, %.2f s, levels %s" %
+        (st, ls.linearSolveIterations(), res, time.time() - t0, ls.amg_info()["rows"]))
+    assert st == 0 and res <= 1e-8
+    del ls
+    torch.cuda.empty_cache()
+    ratio_t = 1.05 ** (n / n_t)
+    net, rec = train_pressure_preconditioner(
+        lowmach_problem(n_t, jitter=0.05, rho_sigma=0.3, seed=778, stretch=ratio_t, max_aspect=100.0),
+        num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=10, log=say)
+    ls = system(mesh, 1, cfg("learned"))
+    ls.zeroSystem()
+    na.assemble_continuity(ls, mesh, rho, p, u, gp, tau=1.0, mode="gather")
+    ls.loadComplete()
+    ls.removeRhsMean()
+    ls.set_learned_preconditioner(net)
+    x.zero_()
+    t0 = time.time()
+    st = ls.solve(x)
+    res = true_resid(oracle, ls, rb, x)
+    say("  mixed FGMRES + learned: status %d, %d iterations, true resid %.2e, %.2f s, fell back to SGS: %s" %
+        (st, ls.linearSolveIterations(), res, time.time() - t0, ls.learned_fallback()))
+    assert st == 0 and res <= 1e-8
+
+
+# ----------------------------------------------------------------- config 4
+@pytest.mark.timeout(900)
+def test_config4_learned_train64_eval128(oracle):
+    from nalu_amd.training import train_pressure_preconditioner
+    say("case: config 4, learned preconditioner trained on 2000 pairs at 64^3, evaluated at 128^3")
+    net, rec = train_pressure_preconditioner(lowmach_problem(64, rho_sigma=0.3, seed=4321), num_rhs=400,
+                                             pairs_per_rhs=5, harvest_tol=1e-6, epochs=10, log=say)
+    assert rec["pairs"] == 2000
+    P = lowmach_problem(128, rho_sigma=0.3, seed=4322)
+    mesh = na.Mesh(P["coords"], P["conn"])
+    rho, p, u, gp, _ = fields(P)
+    out = {}
+    for prec in ("learned", "sgs"):
+        ls = system(mesh, 1, na.SolverConfig("fgmres", prec, tolerance=1e-8, max_iterations=3000, kspace=50,
+                                             precision="mixed"))
+        ls.zeroSystem()
+        na.assemble_continuity(ls, mesh, rho, p, u, gp, tau=1.0, mode="gather")
+        ls.loadComplete()
+        ls.removeRhsMean()
+        if prec == "learned":
+            ls.set_learned_preconditioner(net)
+        _, _, _, rb = ls.csr()
+        x = torch.zeros(ls.sizes()[0], dtype=torch.float64, device=DEV)
+        st = ls.solve(x)
+        res = true_resid(oracle, ls, rb, x)
+        fb = ls.learned_fallback() if prec == "learned" else False
+        out[prec] = (st, ls.linearSolveIterations(), res, fb)
+        say("  128^3 mixed FGMRES(50) + %s: status %d, %d iterations, true resid %.2e, fell back %s" %
+            (prec, st, out[prec][1], res, fb))
+    st, it, res, fb = out["learned"]
+    assert st == 0 and res <= 1e-8 and not fb
+    assert it < out["sgs"][1]
