@@ -126,13 +126,33 @@ int nalu_linsys_build_elem_to_node_graph(nalu_linsys *ls, const nalu_mesh *m);
 int nalu_linsys_finalize(nalu_linsys *ls);
 /* zeroSystem (src/TpetraLinearSystem.C:1355-1372): values, rhs and solution. */
 int nalu_linsys_zero(nalu_linsys *ls);
-/* sumInto, Kokkos overload (src/TpetraLinearSystem.C:1457-1522), batched over
- * num_elems elements: lhs_dev [E][R][R] row-major, rhs_dev [E][R], R = 8*numDof,
- * conn_dev [E][8].  Atomic scatter. */
+/* sumInto, Kokkos overload (TpetraLinearSystem::sumInto, src/TpetraLinearSystem.C:1457-1522,
+ * called per element by SolverAlgorithm::apply_coeff, src/SolverAlgorithm.C:58-68),
+ * batched over num_elems elements on the device.  Per element e:
+ *   entities_dev [E][n] node indices (n = num_entities, n*numDof <= 64),
+ *   rhs_dev [E][R], lhs_dev [E][R][R] row-major (R = n*numDof),
+ *   local_ids_dev / sort_permutation_dev [E][R]: caller scratch, overwritten
+ *   with the element's sorted column LIDs and their permutation, as the
+ *   reference overwrites its team scratch views.
+ * Each row walks its sorted CSR row once (sum_into_row, :1376-1453); entries
+ * outside the graph are dropped; additions are atomic (the reference's
+ * non-Serial path). */
+int nalu_linsys_sum_into_elements(nalu_linsys *ls, int64_t num_elems, int num_entities,
+                                  const int32_t *entities_dev, const double *rhs_dev,
+                                  const double *lhs_dev, int32_t *local_ids_dev,
+                                  int32_t *sort_permutation_dev);
+/* The same for the graph mesh's own elements, looked up through the element
+ * scatter table built with the graph (no sort, no row walk): conn_dev must be
+ * the mesh's connectivity (nalu_mesh_device_ptrs), lhs_dev [E][R][R], rhs_dev
+ * [E][R], R = 8*numDof, elements 0 .. num_elems-1.  Atomic scatter. */
 int nalu_linsys_sum_into_batch(nalu_linsys *ls, int64_t num_elems, const int32_t *conn_dev,
                                const double *lhs_dev, const double *rhs_dev);
 /* sumInto, std::vector overload (src/TpetraLinearSystem.C:1524-1577), one
- * element from host arrays: entities_host [n] node indices, lhs_host [R][R]. */
+ * element from host arrays: entities_host [n] node indices, rhs_host [R],
+ * lhs_host [R][R].  The arrays are copied into a host queue and the call
+ * returns; queued elements are applied on the device, in one batch, before
+ * any other operation on the system reads or writes it (zeroSystem discards
+ * them). */
 int nalu_linsys_sum_into(nalu_linsys *ls, int num_entities, const int32_t *entities_host,
                          const double *rhs_host, const double *lhs_host);
 /* applyDirichletBCs (src/TpetraLinearSystem.C:1579-1653): rows of the listed
@@ -287,6 +307,43 @@ int nalu_assemble_continuity_mass(nalu_linsys *ls, const nalu_mesh *m, const dou
                                   const double *rho_n, const double *rho_np1, double dt,
                                   const double *gamma, int lumped);
 
+/* ----------------------------------------- consolidated element algorithm */
+/* AssembleElemSolverAlgorithm with its kernel list
+ * (include/AssembleElemSolverAlgorithm.h:52-105, src/AssembleElemSolverAlgorithm.C:75-100;
+ * Kernel::setup / execute, include/kernel/Kernel.h:95-125).  Kernel kinds and
+ * the device fields each one reads (field[i], nodal arrays; mdot [E][12]): */
+#define NALU_KERNEL_SCALAR_DIFF 0      /* ScalarDiffElemKernel: q, diffusion coefficient */
+#define NALU_KERNEL_CONTINUITY_ADV 1   /* ContinuityAdvElemKernel: rho, p, u, Gpdx */
+#define NALU_KERNEL_MOMENTUM_ADVDIFF 2 /* MomentumAdvDiffElemKernel: u, mu, mdot */
+#define NALU_KERNEL_CONTINUITY_MASS 3  /* ContinuityMassElemKernel: rho Nm1, N, Np1 */
+#define NALU_KERNEL_MOMENTUM_MASS 4    /* MomentumMassElemKernel: rho Nm1, N, Np1, u Nm1, N, Np1, Gjp */
+typedef struct nalu_kernel_desc {
+  int kind;             /* NALU_KERNEL_* */
+  int shifted_grad_op;  /* SolutionOptions shiftedGradOp (diffusion / Poisson / viscous gradients) */
+  int shift_mdot;       /* continuity: shifted mdot shape functions */
+  int reduced_sens;     /* continuity: reduced-sensitivity (shifted) lhs */
+  int skew_symmetric;   /* momentum: skew-symmetric advection shape functions */
+  int lumped;           /* mass kernels: lumped mass */
+  int balanced_force;   /* momentum mass: balanced-force pressure-gradient density */
+  double include_divu;  /* momentum: divU scaling (SolutionOptions includeDivU, default 0) */
+  const double *field[8];
+} nalu_kernel_desc;
+typedef struct nalu_elem_algorithm nalu_elem_algorithm;
+/* An algorithm assembling into ls over the elements of m (ls finalized on m). */
+int nalu_elem_algorithm_create(nalu_linsys *ls, const nalu_mesh *m, nalu_elem_algorithm **out);
+/* The algorithm owns its kernels (src/Algorithm.C:46-55). */
+int nalu_elem_algorithm_destroy(nalu_elem_algorithm *alg);
+/* activeKernels_.push_back: kernels run in the order they were added. */
+int nalu_elem_algorithm_add_kernel(nalu_elem_algorithm *alg, const nalu_kernel_desc *kernel);
+/* The TimeIntegrator every kernel's setup() reads: dt and the BDF gammas
+ * (default dt = 1, gamma = 1, -1, 0). */
+int nalu_elem_algorithm_set_time_integrator(nalu_elem_algorithm *alg, double dt, const double *gamma3);
+/* execute(): setup of every kernel (ContinuityAdvElemKernel: projTimeScale =
+ * dt / gamma1), then the kernels' element contributions assembled into the
+ * system, each kernel as one fused gather + geometry + physics + scatter pass
+ * (element-visit tiles for the advection / diffusion kernels). */
+int nalu_elem_algorithm_execute(nalu_elem_algorithm *alg);
+
 /* LinearSolverConfig "write_matrix_files" (src/LinearSolverConfig.C:102): when
  * base_name is not NULL, every solve writes <base>-O-<k>.mm.1 (MatrixMarket
  * coordinate, 1-based Tpetra GIDs), <base>-O-<k>.rhs.1 and, after the solve,
@@ -294,6 +351,11 @@ int nalu_assemble_continuity_mass(nalu_linsys *ls, const nalu_mesh *m, const dou
  * (TpetraLinearSystem::writeToFile / writeSolutionToFile,
  * src/TpetraLinearSystem.C:1794-1817,1969-2090).  NULL turns it off. */
 int nalu_linsys_set_write_matrix_files(nalu_linsys *ls, const char *base_name);
+/* TpetraLinearSystem::writeToFile / writeSolutionToFile (src/TpetraLinearSystem.C:1969-1994,
+ * 2070-2090), written now under the same names as the solve-time output:
+ * <base_name>-O-<counter>.mm.1 / .rhs.1 and <base_name>-O-<counter>.sln.1. */
+int nalu_linsys_write_to_file(nalu_linsys *ls, const char *base_name);
+int nalu_linsys_write_solution_to_file(nalu_linsys *ls, const char *base_name);
 
 /* ------------------------------------------------- learned preconditioner
  * No counterpart in the reference: a message-passing network over the matrix

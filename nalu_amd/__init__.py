@@ -53,6 +53,12 @@ class _MueLuParams(C.Structure):
                 ("drop_tol", C.c_double), ("sa_damping", C.c_double)]
 
 
+class _KernelDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("shifted_grad_op", C.c_int), ("shift_mdot", C.c_int), ("reduced_sens", C.c_int),
+                ("skew_symmetric", C.c_int), ("lumped", C.c_int), ("balanced_force", C.c_int),
+                ("include_divu", C.c_double), ("field", C.c_void_p * 8)]
+
+
 class _Config(C.Structure):
     _fields_ = [("method", C.c_int), ("precond", C.c_int), ("tolerance", C.c_double),
                 ("final_tolerance", C.c_double), ("max_iterations", C.c_int), ("kspace", C.c_int),
@@ -89,6 +95,7 @@ def lib():
         L.nalu_linsys_zero.argtypes = [_vp]
         L.nalu_linsys_sum_into_batch.argtypes = [_vp, _i64, _vp, _vp, _vp]
         L.nalu_linsys_sum_into.argtypes = [_vp, C.c_int, _i32p, _dp, _dp]
+        L.nalu_linsys_sum_into_elements.argtypes = [_vp, _i64, C.c_int, _vp, _vp, _vp, _vp, _vp]
         L.nalu_linsys_apply_dirichlet.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int, C.c_int]
         L.nalu_linsys_reset_rows.argtypes = [_vp, _i64, _vp, C.c_int, C.c_int, C.c_double, C.c_double]
         L.nalu_linsys_load_complete.argtypes = [_vp]
@@ -153,6 +160,11 @@ def lib():
         L.nalu_linsys_set_learned_precond.argtypes = [_vp, _vp]
         L.nalu_linsys_set_write_matrix_files.argtypes = [_vp, C.c_char_p]
         L.nalu_linsys_learned_fallback.argtypes = [_vp]
+        L.nalu_elem_algorithm_create.argtypes = [_vp, _vp, C.POINTER(_vp)]
+        L.nalu_elem_algorithm_destroy.argtypes = [_vp]
+        L.nalu_elem_algorithm_add_kernel.argtypes = [_vp, C.POINTER(_KernelDesc)]
+        L.nalu_elem_algorithm_set_time_integrator.argtypes = [_vp, C.c_double, _dp]
+        L.nalu_elem_algorithm_execute.argtypes = [_vp]
         _L = L
     return _L
 
@@ -356,12 +368,33 @@ class LinearSystem:
         _check(lib().nalu_linsys_zero(self._h), "zeroSystem")
 
     def sumInto(self, entities, rhs, lhs):
-        """std::vector overload: one element, host arrays (entities = node indices)."""
-        ents = _np(entities, np.int32)
-        rhs = _np(rhs, np.float64)
-        lhs = _np(lhs, np.float64)
+        """std::vector overload: one element, host arrays (entities = node indices,
+        rhs [R], lhs [R][R] row-major, R = len(entities) * numDof).  Queued; the
+        queue is applied on the device before the system is next used."""
+        ents = _np(entities, np.int32).ravel()
+        R = ents.shape[0] * self.num_dof
+        rhs = _np(rhs, np.float64).ravel()
+        lhs = _np(lhs, np.float64).ravel()
+        if ents.shape[0] < 1 or rhs.shape[0] != R or lhs.shape[0] != R * R:
+            raise NaluError("sumInto: %d entities need rhs [%d] and lhs [%d][%d]" % (ents.shape[0], R, R, R))
         _check(lib().nalu_linsys_sum_into(self._h, ents.shape[0], ents.ctypes.data_as(_i32p),
                                           rhs.ctypes.data_as(_dp), lhs.ctypes.data_as(_dp)), "sumInto")
+
+    def sumIntoElements(self, entities, rhs, lhs, local_ids, sort_permutation):
+        """Kokkos overload on the device, batched: entities [E, n] int32 node
+        indices, rhs [E, R], lhs [E, R, R] (R = n numDof), local_ids and
+        sort_permutation [E, R] int32 caller scratch (overwritten with each
+        element's sorted column LIDs and their permutation)."""
+        if entities.dim() != 2:
+            raise NaluError("sumIntoElements: entities must be [E, n]")
+        E, n = int(entities.shape[0]), int(entities.shape[1])
+        R = n * self.num_dof
+        _check(lib().nalu_linsys_sum_into_elements(
+            self._h, E, n, _dev(entities, "sumIntoElements entities", "i32", shape=(E, n)),
+            _dev(rhs, "sumIntoElements rhs", "f64", shape=(E, R)),
+            _dev(lhs, "sumIntoElements lhs", "f64", shape=(E, R, R)),
+            _dev(local_ids, "sumIntoElements localIds", "i32", shape=(E, R)),
+            _dev(sort_permutation, "sumIntoElements sortPermutation", "i32", shape=(E, R))), "sumIntoElements")
 
     def sumIntoBatch(self, lhs, rhs, num_elems=None):
         """Kokkos overload batched over the graph mesh's elements: lhs [E,R,R], rhs [E,R] device tensors."""
@@ -714,8 +747,9 @@ class LearnedPreconditioner:
 
 # ------------------------------------------------------------------ assembly algorithms
 # scatter strategies: "gather" = one thread per row walks its elements (deterministic),
-# "atomic" = element-centric FP64 atomics, "tile" = node-tile LDS accumulation (fastest)
-MODE = {"gather": 0, "atomic": 1, "tile": 2}
+# "atomic" = element-centric FP64 atomics, "tile" = node-tile LDS accumulation, "etile" =
+# element-visit tiles (each incident element evaluated once per tile of rows)
+MODE = {"gather": 0, "atomic": 1, "tile": 2, "etile": 3}
 
 
 def assemble_scalar_diff(ls, mesh, q, k, shifted=False, mode="tile"):
@@ -821,3 +855,80 @@ def assemble_continuity_mass(ls, mesh, rho, dt, gamma=(1.0, -1.0, 0.0), lumped=F
     r, g = _states(rho, "continuity_mass rho", mesh.num_nodes), _gammas(gamma)
     _check(lib().nalu_assemble_continuity_mass(ls.handle, mesh.handle, r[0], r[1], r[2], float(dt),
                                                g.ctypes.data_as(_dp), int(lumped)), "assemble_continuity_mass")
+
+
+# ------------------------------------------------------------------ consolidated element algorithm
+class Kernel:
+    """Kernel of the consolidated element algorithm (include/kernel/Kernel.h:95-125):
+    a kind, its options and the device fields it reads.  Fields stay referenced
+    by the kernel object while the algorithm holds it."""
+    KIND = None
+    FIELDS = ()
+
+    def __init__(self, *fields, shifted_grad_op=False, shift_mdot=False, reduced_sens=False, skew_symmetric=False,
+                 lumped=False, balanced_force=False, include_divu=0.0):
+        if len(fields) != len(self.FIELDS):
+            raise NaluError("%s: fields %s expected" % (type(self).__name__, ", ".join(self.FIELDS)))
+        self.fields = fields
+        self.opts = dict(shifted_grad_op=int(shifted_grad_op), shift_mdot=int(shift_mdot),
+                         reduced_sens=int(reduced_sens), skew_symmetric=int(skew_symmetric), lumped=int(lumped),
+                         balanced_force=int(balanced_force), include_divu=float(include_divu))
+
+    def desc(self, mesh):
+        N, E = mesh.num_nodes, mesh.num_elems
+        d = _KernelDesc(kind=self.KIND, **self.opts)
+        for i, (f, name) in enumerate(zip(self.fields, self.FIELDS)):
+            numel = 12 * E if name == "mdot" else (3 * N if name in ("u", "Gpdx", "Gjp", "uNm1", "uN", "uNp1") else N)
+            d.field[i] = _dev(f, "%s %s" % (type(self).__name__, name), "f64", numel=numel)
+        return d
+
+
+class ScalarDiffElemKernel(Kernel):
+    KIND, FIELDS = 0, ("q", "diffusion coefficient")
+
+
+class ContinuityAdvElemKernel(Kernel):
+    KIND, FIELDS = 1, ("rho", "p", "u", "Gpdx")
+
+
+class MomentumAdvDiffElemKernel(Kernel):
+    KIND, FIELDS = 2, ("u", "mu", "mdot")
+
+
+class ContinuityMassElemKernel(Kernel):
+    KIND, FIELDS = 3, ("rhoNm1", "rhoN", "rhoNp1")
+
+
+class MomentumMassElemKernel(Kernel):
+    KIND, FIELDS = 4, ("rhoNm1", "rhoN", "rhoNp1", "uNm1", "uN", "uNp1", "Gjp")
+
+
+class AssembleElemSolverAlgorithm:
+    """AssembleElemSolverAlgorithm (include/AssembleElemSolverAlgorithm.h:52-105):
+    activeKernels_ run in order by execute(), after each kernel's setup with the
+    time integrator (dt, gamma1..3)."""
+
+    def __init__(self, linsys, mesh):
+        self.linsys, self.mesh = linsys, mesh
+        self.activeKernels_ = []
+        h = _vp()
+        _check(lib().nalu_elem_algorithm_create(linsys.handle, mesh.handle, C.byref(h)), "AssembleElemSolverAlgorithm")
+        self._h = h
+
+    def add_kernel(self, kernel):
+        _check(lib().nalu_elem_algorithm_add_kernel(self._h, C.byref(kernel.desc(self.mesh))), "add_kernel")
+        self.activeKernels_.append(kernel)
+
+    def set_time_integrator(self, dt, gamma=(1.0, -1.0, 0.0)):
+        g = _gammas(gamma)
+        _check(lib().nalu_elem_algorithm_set_time_integrator(self._h, float(dt), g.ctypes.data_as(_dp)),
+               "set_time_integrator")
+
+    def execute(self):
+        _check(lib().nalu_elem_algorithm_execute(self._h), "AssembleElemSolverAlgorithm::execute")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _L is not None:
+            _L.nalu_elem_algorithm_destroy(h)
+            self._h = None

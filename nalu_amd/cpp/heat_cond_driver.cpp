@@ -3,6 +3,8 @@
 // (src/HeatCondEquationSystem.C:1038-1042), then EquationSystem::assemble_and_solve
 // (src/EquationSystem.C:264-307): zeroSystem, element assembly, Dirichlet,
 // loadComplete, solve; then T += dT (src/HeatCondEquationSystem.C:1105-1110).
+// Assembly goes through AssembleElemSolverAlgorithm with a ScalarDiffElemKernel,
+// Dirichlet rows through applyDirichletBCs, as the reference's equation system does.
 //
 // Usage: heat_cond_driver [n=32] [jitter=0.1] [seed=1234]
 #include <hip/hip_runtime.h>
@@ -11,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
 #include <vector>
 
 #include "../../include/nalu_hip.hpp"
@@ -58,10 +61,15 @@ int main(int argc, char** argv) {
     HIPOK(hipMemcpy(dbcv, bcv.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(dbc, bc.data(), sizeof(int32_t) * bc.size(), hipMemcpyHostToDevice));
 
+    // the equation system's consolidated element algorithm with its one kernel
+    nalu_hip::AssembleElemSolverAlgorithm assemble(linsys, mesh);
+    assemble.addKernel(std::make_unique<nalu_hip::ScalarDiffElemKernel>(dT, dk));
+    const nalu_hip::DirichletBC bcT{(int64_t)bc.size(), dbc, dT, dbcv};
+
     const auto t0 = std::chrono::steady_clock::now();
     linsys.zeroSystem();
-    nalu_hip::assembleScalarDiff(linsys, mesh, dT, dk);
-    linsys.applyDirichletBCs((int64_t)bc.size(), dbc, dT, dbcv, 0, 1);
+    assemble.execute();
+    linsys.applyDirichletBCs(bcT, 0, 1);
     linsys.loadComplete();
     const int status = linsys.solve(dd);
     HIPOK(hipDeviceSynchronize());

@@ -512,18 +512,6 @@ __global__ void k_dirichlet(const int32_t* __restrict__ nodes, int64_t nbc, int6
   rhs[row] = reset_mode ? rhs_residual : bcv[(int64_t)nd * node + d] - sol[(int64_t)nd * node + d];
 }
 
-// Opt a kernel into more than the default 64 KiB of dynamic LDS when needed.
-void set_lds_limit(const void* kern, size_t bytes) {
-  int dev = 0;
-  NALU_HIP_CHECK(hipGetDevice(&dev));
-  int maxopt = 0;
-  NALU_HIP_CHECK(hipDeviceGetAttribute(&maxopt, hipDeviceAttributeSharedMemPerBlockOptin, dev));
-  if (maxopt <= 0) NALU_HIP_CHECK(hipDeviceGetAttribute(&maxopt, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  NALU_REQUIRE(bytes <= (size_t)maxopt, NALU_ERR_UNSUPPORTED,
-               "assembly tile needs " + std::to_string(bytes) + " B of LDS, device allows " + std::to_string(maxopt));
-  if (bytes > 64 * 1024) NALU_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-}
-
 template <class Op>
 void launch_scalar(nalu_linsys* ls, const nalu_mesh* m, const Op& op, int mode) {
   hipStream_t s = ls->stream;
@@ -829,15 +817,31 @@ void check_assembly_args(nalu_linsys* ls, const nalu_mesh* m, int nd) {
   NALU_REQUIRE(ls->finalized, NALU_ERR_STATE, "assembly: finalizeLinearSystem not called");
   NALU_REQUIRE(ls->mesh == m, NALU_ERR_ARG, "assembly: mesh differs from the one the graph was built on");
   NALU_REQUIRE(ls->num_dof == nd, NALU_ERR_ARG, "assembly: numDof mismatch for this kernel");
+  // queued host sumInto calls precede this assembly (and keep it from folding the zero)
+  flush_sum_into_queue(ls);
 }
 
 }  // namespace
+
+// Opt a kernel into more than the default 64 KiB of dynamic LDS when needed.
+void set_lds_limit(const void* kern, size_t bytes) {
+  int dev = 0;
+  NALU_HIP_CHECK(hipGetDevice(&dev));
+  int maxopt = 0;
+  NALU_HIP_CHECK(hipDeviceGetAttribute(&maxopt, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+  if (maxopt <= 0) NALU_HIP_CHECK(hipDeviceGetAttribute(&maxopt, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  NALU_REQUIRE(bytes <= (size_t)maxopt, NALU_ERR_UNSUPPORTED,
+               "assembly tile needs " + std::to_string(bytes) + " B of LDS, device allows " + std::to_string(maxopt));
+  if (bytes > 64 * 1024) NALU_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
 
 void materialize_pending_zero(nalu_linsys* ls) {
   if (ls->values_pending_zero)
     NALU_HIP_CHECK(hipMemsetAsync(ls->values.p, 0, ls->values.bytes(), ls->stream));
   if (ls->rhs_pending_zero) NALU_HIP_CHECK(hipMemsetAsync(ls->rhs.p, 0, ls->rhs.bytes(), ls->stream));
   ls->values_pending_zero = ls->rhs_pending_zero = false;
+  // host sumInto calls queued since the zero come next
+  flush_sum_into_queue(ls);
 }
 
 }  // namespace nalu
@@ -849,9 +853,11 @@ extern "C" int nalu_assemble_scalar_diff(nalu_linsys* ls, const nalu_mesh* m, co
   return guarded([&]() -> int {
     check_assembly_args(ls, m, 1);
     NALU_REQUIRE(q && k, NALU_ERR_ARG, "scalar_diff: null field");
-    NALU_REQUIRE(mode >= 0 && mode <= 2, NALU_ERR_ARG, "scalar_diff: mode must be 0, 1 or 2");
+    NALU_REQUIRE(mode >= 0 && mode <= 3, NALU_ERR_ARG, "scalar_diff: mode must be 0, 1, 2 or 3");
     PhaseTimer t(ls, PH_ASSEMBLE);
-    if (fp32_assembly(ls)) {
+    if (mode == 3) {
+      etile_scalar_diff(ls, m, q, k, shifted != 0, fp32_assembly(ls));
+    } else if (fp32_assembly(ls)) {
       if (shifted) launch_scalar(ls, m, ScalarDiffOp<true, float>{q, k}, mode);
       else launch_scalar(ls, m, ScalarDiffOp<false, float>{q, k}, mode);
     } else {
@@ -869,9 +875,13 @@ extern "C" int nalu_assemble_continuity(nalu_linsys* ls, const nalu_mesh* m, con
     check_assembly_args(ls, m, 1);
     NALU_REQUIRE(rho && p && u && gp, NALU_ERR_ARG, "continuity: null field");
     NALU_REQUIRE(tau > 0.0, NALU_ERR_ARG, "continuity: projTimeScale must be positive");
-    NALU_REQUIRE(mode >= 0 && mode <= 2, NALU_ERR_ARG, "continuity: mode must be 0, 1 or 2");
+    NALU_REQUIRE(mode >= 0 && mode <= 3, NALU_ERR_ARG, "continuity: mode must be 0, 1, 2 or 3");
     PhaseTimer t(ls, PH_ASSEMBLE);
     const int key = (shift_mdot ? 4 : 0) | (shift_poisson ? 2 : 0) | (reduced_sens ? 1 : 0);
+    if (mode == 3) {
+      etile_continuity(ls, m, rho, p, u, gp, tau, key, fp32_assembly(ls));
+      return NALU_OK;
+    }
     switch (key) {
 #define NALU_CT(K, A, B, C)                                                          \
   case K:                                                                            \
@@ -919,11 +929,15 @@ extern "C" int nalu_assemble_momentum(nalu_linsys* ls, const nalu_mesh* m, const
   return guarded([&]() -> int {
     check_assembly_args(ls, m, 3);
     NALU_REQUIRE(u && mu && mdot, NALU_ERR_ARG, "momentum: null field");
-    NALU_REQUIRE(mode == 1 || mode == 2, NALU_ERR_UNSUPPORTED,
-                 "momentum: mode 1 (atomic) or 2 (node tile); the per-thread row gather is scalar-only");
+    NALU_REQUIRE(mode >= 1 && mode <= 3, NALU_ERR_UNSUPPORTED,
+                 "momentum: mode 1 (atomic), 2 (node tile) or 3 (element tile); the per-thread row gather is "
+                 "scalar-only");
     PhaseTimer t(ls, PH_ASSEMBLE);
     const int64_t E = m->num_elems;
-    if (mode == 2) {
+    if (mode == 3) {
+      build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
+      etile_momentum(ls, m, u, mu, mdot, include_divu, skew != 0, shifted != 0, fp32_assembly(ls));
+    } else if (mode == 2) {
       build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
       NALU_REQUIRE(ls->values_pending_zero == ls->rhs_pending_zero, NALU_ERR_STATE,
                    "tile assembly: values/rhs zero state diverged");

@@ -174,6 +174,10 @@ struct nalu_linsys {
 
   // learned preconditioner (optional)
   void* learned = nullptr;
+  // element-visit tile plans of the assembly (etile.hip)
+  void* etile_plans = nullptr;
+  // queued calls of the host (std::vector) sumInto overload (suminto.hip)
+  void* sum_into_queue = nullptr;
   // AMG preconditioner: parameters and hierarchy (amg.hip)
   nalu_muelu_params muelu{1000, 10, 4, 20.0, 1.1, 30, 0.0, 4.0 / 3.0};
   void* amg = nullptr;
@@ -231,6 +235,18 @@ void build_graph(nalu_linsys* ls, hipStream_t s);
 void build_coloring(nalu_linsys* ls, hipStream_t s);
 // assemble.hip
 void materialize_pending_zero(nalu_linsys* ls);
+void set_lds_limit(const void* kern, size_t bytes);
+// etile.hip: element-visit tile assembly
+void etile_scalar_diff(nalu_linsys* ls, const nalu_mesh* m, const double* q, const double* k, bool shifted, bool fp32);
+void etile_continuity(nalu_linsys* ls, const nalu_mesh* m, const double* rho, const double* p, const double* u,
+                      const double* gp, double tau, int key, bool fp32);
+void etile_momentum(nalu_linsys* ls, const nalu_mesh* m, const double* u, const double* mu, const double* mdot,
+                    double include_divu, bool skew, bool shifted, bool fp32);
+void etile_release(nalu_linsys* ls);
+// suminto.hip: queue of host sumInto calls, applied before anything else touches the system
+void flush_sum_into_queue(nalu_linsys* ls);
+void discard_sum_into_queue(nalu_linsys* ls);
+void release_sum_into_queue(nalu_linsys* ls);
 // krylov.hip
 void precond_setup(nalu_linsys* ls);
 void precond_apply(nalu_linsys* ls, const double* r, double* z);

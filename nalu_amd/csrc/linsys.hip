@@ -135,6 +135,8 @@ extern "C" int nalu_linsys_destroy(nalu_linsys* ls) {
   flush_timers(ls);
   learned_release(ls);
   amg_release(ls);
+  etile_release(ls);
+  release_sum_into_queue(ls);
   delete ls;
   return NALU_OK;
 }
@@ -177,52 +179,11 @@ extern "C" int nalu_linsys_zero(nalu_linsys* ls) {
     PhaseTimer t(ls, PH_ZERO);
     // values/rhs are zeroed lazily: the first owner-computes assembly writes
     // its rows instead of accumulating; anything else materializes the zero.
+    discard_sum_into_queue(ls);
     ls->values_pending_zero = true;
     ls->rhs_pending_zero = true;
     ls->precond_ready = false;
     NALU_HIP_CHECK(hipMemsetAsync(ls->sln.p, 0, ls->sln.bytes(), ls->stream));
-    return NALU_OK;
-  });
-}
-
-extern "C" int nalu_linsys_sum_into(nalu_linsys* ls, int ne, const int32_t* ents, const double* rhs_h,
-                                    const double* lhs_h) {
-  return guarded([&]() -> int {
-    NALU_REQUIRE(ls && ls->finalized, NALU_ERR_STATE, "sumInto: system not finalized");
-    NALU_REQUIRE(ne > 0 && ne <= 64 && ents && rhs_h && lhs_h, NALU_ERR_ARG, "sumInto: bad arguments");
-    PhaseTimer t(ls, PH_ASSEMBLE);
-    materialize_pending_zero(ls);
-    const int nd = ls->num_dof, R = ne * nd;
-    // host-side walk of the rows touched (the std::vector overload is not
-    // thread-safe in the reference either, src/TpetraLinearSystem.C:1541)
-    std::vector<int32_t> lid(ne);
-    for (int i = 0; i < ne; ++i) {
-      NALU_REQUIRE(ents[i] >= 0 && ents[i] < ls->num_nodes, NALU_ERR_ARG, "sumInto: entity out of range");
-      NALU_HIP_CHECK(hipMemcpyAsync(&lid[i], ls->node_lid.p + ents[i], sizeof(int32_t), hipMemcpyDeviceToHost, ls->stream));
-    }
-    NALU_HIP_CHECK(hipStreamSynchronize(ls->stream));
-    for (int a = 0; a < ne; ++a)
-      for (int d = 0; d < nd; ++d) {
-        const int64_t row = (int64_t)nd * lid[a] + d;
-        int32_t rp[2];
-        NALU_HIP_CHECK(hipMemcpy(rp, ls->row_ptr.p + row, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
-        std::vector<int32_t> cols(rp[1] - rp[0]);
-        std::vector<double> vals(rp[1] - rp[0]);
-        NALU_HIP_CHECK(hipMemcpy(cols.data(), ls->col_idx.p + rp[0], sizeof(int32_t) * cols.size(), hipMemcpyDeviceToHost));
-        NALU_HIP_CHECK(hipMemcpy(vals.data(), ls->values.p + rp[0], sizeof(double) * vals.size(), hipMemcpyDeviceToHost));
-        for (int b = 0; b < ne; ++b)
-          for (int e = 0; e < nd; ++e) {
-            const int32_t col = nd * lid[b] + e;
-            const auto it = std::lower_bound(cols.begin(), cols.end(), col);
-            // entries outside the graph are dropped, as sum_into_row does (:1436-1451)
-            if (it != cols.end() && *it == col) vals[it - cols.begin()] += lhs_h[(size_t)(a * nd + d) * R + b * nd + e];
-          }
-        NALU_HIP_CHECK(hipMemcpy(ls->values.p + rp[0], vals.data(), sizeof(double) * vals.size(), hipMemcpyHostToDevice));
-        double r = 0.0;
-        NALU_HIP_CHECK(hipMemcpy(&r, ls->rhs.p + row, sizeof(double), hipMemcpyDeviceToHost));
-        r += rhs_h[a * nd + d];
-        NALU_HIP_CHECK(hipMemcpy(ls->rhs.p + row, &r, sizeof(double), hipMemcpyHostToDevice));
-      }
     return NALU_OK;
   });
 }
@@ -308,6 +269,40 @@ void write_solution_file(nalu_linsys* ls) {
 
 }  // namespace
 }  // namespace nalu
+
+// writeToFile / writeSolutionToFile on demand (include/LinearSystem.h:126-127):
+// the same files as the solve-time output, named from base_name.
+extern "C" int nalu_linsys_write_to_file(nalu_linsys* ls, const char* base_name) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized && base_name && *base_name, NALU_ERR_ARG, "writeToFile: bad arguments");
+    const std::string keep = ls->write_base;
+    ls->write_base = base_name;
+    try {
+      write_matrix_files(ls);
+    } catch (...) {
+      ls->write_base = keep;
+      throw;
+    }
+    ls->write_base = keep;
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_linsys_write_solution_to_file(nalu_linsys* ls, const char* base_name) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized && base_name && *base_name, NALU_ERR_ARG, "writeSolutionToFile: bad arguments");
+    const std::string keep = ls->write_base;
+    ls->write_base = base_name;
+    try {
+      write_solution_file(ls);
+    } catch (...) {
+      ls->write_base = keep;
+      throw;
+    }
+    ls->write_base = keep;
+    return NALU_OK;
+  });
+}
 
 extern "C" int nalu_linsys_set_write_matrix_files(nalu_linsys* ls, const char* base_name) {
   return guarded([&]() -> int {

@@ -110,7 +110,12 @@ class LowMachStep:
             assemble_momentum_segregated(self.mom, self.mesh, self.u, self.mu, self.mdot, self.rhs3)
             assemble_momentum_mass_segregated(self.mom, self.mesh, (self.rho, self.rho, self.rho), (uN, uN, self.u),
                                               self.Gjp, self.dt, self.rhs3, gamma=self.g)
-            self.mom.applyDirichletBCs(self.wall_i32, self.u[:, 0].contiguous(), self.wall_u[:, 0].contiguous())
+            # Dirichlet rows (the field arrays are nodal: component 0 of the velocity
+            # and its boundary value; rhs3 then takes all three components)
+            u0 = self.u[:, 0].contiguous()
+            bc0 = u0.clone()
+            bc0[self.wall] = self.wall_u[:, 0]
+            self.mom.applyDirichletBCs(self.wall_i32, u0, bc0)
             self.rhs3[:, self.wall] = (self.wall_u - self.u[self.wall]).T
             self.mom.loadComplete()
             self.du3.zero_()

@@ -37,7 +37,7 @@ def mixed_system(coords, conn, nd=1):
 
 
 @pytest.mark.parametrize("stretch", [1.0, 1.25])
-@pytest.mark.parametrize("mode", ["gather", "tile"])
+@pytest.mark.parametrize("mode", ["gather", "tile", "etile"])
 def test_fp32_continuity_and_mdot(oracle, mode, stretch):
     n = 8
     print("case: fp32 continuity + mdot n=%d mode=%s stretch=%g" % (n, mode, stretch), flush=True)
@@ -62,7 +62,7 @@ def test_fp32_continuity_and_mdot(oracle, mode, stretch):
     assert e_mat > 0.0
 
 
-@pytest.mark.parametrize("mode", ["atomic", "tile"])
+@pytest.mark.parametrize("mode", ["atomic", "tile", "etile"])
 def test_fp32_momentum(oracle, mode):
     n = 6
     print("case: fp32 momentum n=%d %s" % (n, mode), flush=True)

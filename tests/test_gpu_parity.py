@@ -43,7 +43,7 @@ def make_system(coords, conn, nd=1, cfg=None, ids=None):
 
 # ------------------------------------------------------------- reference golds on one element
 
-@pytest.mark.parametrize("mode", ["gather", "atomic", "tile"])
+@pytest.mark.parametrize("mode", ["gather", "atomic", "tile", "etile"])
 def test_scalar_diff_gold_gpu(golds, mode):
     print("case: scalar_diff gold, mode", mode, flush=True)
     mesh, ls = make_system(UNIT_HEX, np.arange(8, dtype=np.int32)[None, :])
@@ -55,7 +55,7 @@ def test_scalar_diff_gold_gpu(golds, mode):
     np.testing.assert_allclose(r, 0.0, atol=1e-14)
 
 
-@pytest.mark.parametrize("mode", ["gather", "atomic", "tile"])
+@pytest.mark.parametrize("mode", ["gather", "atomic", "tile", "etile"])
 @pytest.mark.parametrize("name,shift,reduced", [("continuity_default", False, False),
                                                 ("continuity_reduced_sens", False, True),
                                                 ("continuity_reduced_sens", True, True)])
@@ -72,7 +72,7 @@ def test_continuity_gold_gpu(golds, mode, name, shift, reduced):
     np.testing.assert_allclose(r, golds[name]["rhs"], rtol=0, atol=1e-14)
 
 
-@pytest.mark.parametrize("mode", ["atomic", "tile"])
+@pytest.mark.parametrize("mode", ["atomic", "tile", "etile"])
 def test_momentum_gold_gpu(golds, mode):
     print("case: momentum gold", mode, flush=True)
     X = UNIT_HEX
@@ -113,7 +113,7 @@ def test_tpetra_gold_sum_into_batch(golds):
 
 # ------------------------------------------------------------- whole-mesh parity
 
-@pytest.mark.parametrize("mode", ["gather", "atomic", "tile"])
+@pytest.mark.parametrize("mode", ["gather", "atomic", "tile", "etile"])
 def test_heat_assembly_parity(oracle, mode):
     n = 8
     print("case: heat assembly n=%d mode=%s" % (n, mode), flush=True)
@@ -163,7 +163,7 @@ def test_gather_accumulates_after_another_algorithm(oracle):
     np.testing.assert_allclose(r2, 2 * r1, rtol=0, atol=TOL * np.abs(r1).max())
 
 
-@pytest.mark.parametrize("mode", ["gather", "atomic", "tile"])
+@pytest.mark.parametrize("mode", ["gather", "atomic", "tile", "etile"])
 def test_continuity_and_mdot_parity(oracle, mode):
     n = 8
     print("case: continuity + mdot n=%d mode=%s" % (n, mode), flush=True)
@@ -184,7 +184,7 @@ def test_continuity_and_mdot_parity(oracle, mode):
     np.testing.assert_allclose(mdot.cpu().numpy(), om, rtol=0, atol=TOL * np.abs(om).max())
 
 
-@pytest.mark.parametrize("mode", ["atomic", "tile"])
+@pytest.mark.parametrize("mode", ["atomic", "tile", "etile"])
 def test_momentum_parity(oracle, mode):
     n = 6
     print("case: momentum n=%d %s" % (n, mode), flush=True)
