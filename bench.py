@@ -14,7 +14,8 @@ viscosity 1e-3, dt = gamma1 = 1.  One step is the hot path once:
 `value` is the assembly throughput (elements of steps 1-3 per second, Melem/s);
 the pressure solve time, iterations and residual sit beside it.  Inputs are
 resident in HBM before the timed region.  The dominant kernel's roofline and a
-CPU baseline (the FP64 oracle, one core, on a bounded sample) are reported too.
+CPU baseline (the oracle's arithmetic built -O3 -march=native -fopenmp, on every host core and on
+one core, at the same size) are reported too.
 """
 import argparse
 import json
@@ -37,7 +38,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128, help="elements per direction")
-    ap.add_argument("--cpu-n", type=int, default=48, help="mesh size of the CPU baseline sample")
+    ap.add_argument("--cpu-n", type=int, default=128, help="mesh size of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
@@ -48,6 +49,8 @@ def parse():
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--muelu-degree", type=int, default=2, help="Chebyshev degree of the AMG smoother")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-assembly / mixed-precision block")
+    ap.add_argument("--assembly-mode", default="etile", choices=["gather", "atomic", "tile", "etile"],
+                    help="scatter strategy of the continuity and momentum assembly (default: element-visit tiles)")
     ap.add_argument("--learned-apply-only", type=int, default=0, metavar="K",
                     help="only time K applications of an untrained learned preconditioner on the n^3 "
                          "continuity system (profiling aid), print one JSON line and exit")
@@ -76,27 +79,34 @@ def algorithmic_bytes(N, E, P, rows, nnz1, nnz3):
 
 
 def cpu_baseline(n_cpu):
-    """The FP64 oracle, single core, on an n_cpu^3 mesh of the same workload."""
+    """The oracle's arithmetic compiled -O3 -march=native -fopenmp (oracle/cpu_baseline.c),
+    on the n_cpu^3 low-Mach case: assembly (zeroSystem + mdot + continuity +
+    momentum, elements threaded, atomic sumInto) on every host core the process
+    may use and on one core, and the CG + SGS pressure solve to 1e-8 on every
+    core (SGS per thread block of rows, as Ifpack2 runs it per MPI rank)."""
     from nalu_amd.problems import lowmach_problem
-    from oracle import pyoracle as O
+    from oracle import cpu_baseline as CB
     P = lowmach_problem(n_cpu, seed=4321)
-    nn, E = P["coords"].shape[0], P["conn"].shape[0]
-    g1 = O.graph_build(nn, P["conn"], 1)
-    g3 = O.graph_build(nn, P["conn"], 3)
-    t0 = O.wtime()
-    md = O.compute_mdot(P["conn"], P["coords"], P["rho"], P["p"], P["u"], P["gpdx"], P["tau"])
-    v, r = O.assemble_continuity(g1, P["conn"], P["coords"], P["rho"], P["p"], P["u"], P["gpdx"], P["tau"])
-    O.assemble_momentum(g3, P["conn"], P["coords"], P["u"], P["mu"], md)
-    t_asm = O.wtime() - t0
-    b = r - r.mean()
-    t0 = O.wtime()
-    _, it, res = O.cg(g1.row_ptr, g1.col_idx, v, b, maxit=20000, tol=1e-8, precond="sgs")
-    t_solve = O.wtime() - t0
-    return {"value": E / t_asm / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
-            "sample": "%d^3 low-Mach mesh (%d elements), same fields: mdot + continuity + momentum assembly "
-                      "%.2f s; CG + sequential SGS pressure solve to 1e-8: %d iters, %.2f s, resid %.1e"
-                      % (n_cpu, E, t_asm, it, t_solve, res),
-            "pressure_solve_s": t_solve, "pressure_iters": it}
+    E = P["conn"].shape[0]
+    case = CB.LowMachCase(P)
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = CB.max_threads()
+    CB.set_threads(threads)
+    case.assemble()
+    t_all = float(np.median([case.assemble() for _ in range(3)]))
+    it, res, t_solve = case.pressure_solve(tol=1e-8)
+    CB.set_threads(1)
+    t_one = case.assemble()
+    CB.set_threads(threads)
+    return {"value": E / t_all / 1e6, "unit": "Melem/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "one_core_melem_s": E / t_one / 1e6,
+            "sample": "%d^3 low-Mach mesh (%d elements), same fields and seed as the GPU case: zeroSystem + mdot + "
+                      "continuity + momentum assembly %.3f s on %d threads (median of 3), %.2f s on 1 thread; CG + "
+                      "per-thread-block SGS pressure solve to 1e-8: %d iterations, %.2f s, resid %.1e "
+                      "(oracle arithmetic, gcc -O3 -march=native -fopenmp)"
+                      % (n_cpu, E, t_all, threads, t_one, it, t_solve, res),
+            "assembly_s": t_all, "assembly_s_one_core": t_one, "pressure_solve_s": t_solve, "pressure_iters": it,
+            "pressure_true_rel_resid": res}
 
 
 def train_network(args, na, torch, devc, Pt, tag=""):
@@ -497,6 +507,11 @@ def main():
     t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=devc).contiguous()
     rho, p, u, gp, mu = t(P["rho"]), t(P["p"]), t(P["u"]), t(P["gpdx"]), t(P["mu"])
     mdot = torch.zeros((E, 12), dtype=torch.float64, device=devc)
+    # Gpdx: projected nodal gradient of p (interior Green-Gauss + exposed-face
+    # closure) on the dual nodal volume, computed on the GPU every step
+    dual_vol = torch.zeros(N, dtype=torch.float64, device=devc)
+    na.compute_dual_volume(mesh, dual_vol)
+    gp = torch.zeros((N, 3), dtype=torch.float64, device=devc)
     rows, nnz1 = cont.sizes()
     _, nnz3 = mom.sizes()
     Pn = nnz1  # scalar node-pair nonzeros
@@ -515,18 +530,22 @@ def main():
         del src, dst
 
     ev = lambda: torch.cuda.Event(enable_timing=True)
+    cmode = args.assembly_mode
+    mmode = "tile" if args.assembly_mode == "gather" else args.assembly_mode  # no row gather for numDof 3
 
     def step(timers=None):
-        e = [ev() for _ in range(5)] if timers is not None else None
+        e = [ev() for _ in range(6)] if timers is not None else None
+        if e: e[5].record()
+        na.compute_nodal_gradient(mesh, p, dual_vol, gp, stream=stream)
         if e: e[0].record()
         na.compute_mdot(mesh, rho, p, u, gp, mdot, tau=P["tau"], stream=stream)
         if e: e[1].record()
         cont.zeroSystem()
-        na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode="gather")
+        na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode=cmode)
         cont.loadComplete()
         if e: e[2].record()
         mom.zeroSystem()
-        na.assemble_momentum(mom, mesh, u, mu, mdot, mode="tile")
+        na.assemble_momentum(mom, mesh, u, mu, mdot, mode=mmode)
         mom.loadComplete()
         if e: e[3].record()
         # zero-mean projection of the continuity rhs (singular Neumann pressure system)
@@ -543,6 +562,8 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    gp_ref = t(P["gpdx"])
+    gp_rel = float(torch.linalg.norm(gp - gp_ref) / torch.linalg.norm(gp_ref))
     timers = []
     print("bench: timing %d steps" % args.steps, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -551,8 +572,9 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
-    ph = {"mdot": [], "continuity_assembly": [], "momentum_assembly": [], "pressure_solve": []}
+    ph = {"nodal_gradient": [], "mdot": [], "continuity_assembly": [], "momentum_assembly": [], "pressure_solve": []}
     for (e, it, rel, st) in timers:
+        ph["nodal_gradient"].append(e[5].elapsed_time(e[0]) * 1e-3)
         ph["mdot"].append(e[0].elapsed_time(e[1]) * 1e-3)
         ph["continuity_assembly"].append(e[1].elapsed_time(e[2]) * 1e-3)
         ph["momentum_assembly"].append(e[2].elapsed_time(e[3]) * 1e-3)
@@ -589,8 +611,12 @@ def main():
         "sgs_apply": (t_sgs * (iters + 1), byt["sgs_apply"], 2 * ncol - 1),
         "spmv": (t_spmv * (iters + 1), byt["spmv"], 1),
     }
-    kern_name = {"mdot": "k_mdot_elem", "continuity_assembly": "k_scalar_gather<ContinuityOp>",
-                 "momentum_assembly": "k_momentum_tile", "sgs_apply": "k_sell_fwd + k_sell_bwd",
+    kname = {"gather": ("k_scalar_gather<ContinuityOp>", "k_momentum_tile"),
+             "atomic": ("k_scalar_atomic<ContinuityOp>", "k_momentum_atomic"),
+             "tile": ("k_scalar_tile<ContinuityOp>", "k_momentum_tile"),
+             "etile": ("k_scalar_items<ContinuityEt> + k_scalar_etile", "k_momentum_items + k_momentum_etile")}
+    kern_name = {"mdot": "k_mdot_elem", "continuity_assembly": kname[cmode][0],
+                 "momentum_assembly": kname[mmode][1], "sgs_apply": "k_sell_fwd + k_sell_bwd",
                  "spmv": "k_sell_spmv"}
     dom = max(per_step, key=lambda k: per_step[k][0])
     unit_time = {"mdot": med["mdot"], "continuity_assembly": med["continuity_assembly"],
@@ -621,7 +647,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        print("bench: CPU baseline (oracle, %d^3 sample)" % args.cpu_n, file=sys.stderr, flush=True)
+        print("bench: CPU baseline (threaded oracle arithmetic, %d^3)" % args.cpu_n, file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_n)
 
     out = {
@@ -638,7 +664,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded LCG mesh jitter and velocity noise, analytic Taylor-Green fields)",
         "config": {"workload": "low-Mach momentum + continuity assembly and CG+SGS pressure-Poisson solve, "
-                               "%d^3 Hex8, jitter 0.1h, plain FP64" % n,
+                               "%d^3 Hex8, jitter 0.1h, plain FP64" % n, "assembly_mode": cmode,
                    "elements": E, "nodes": N, "continuity_nnz": nnz1, "momentum_nnz": nnz3,
                    "parallelism": "single GPU"},
         "assembly_s": t_asm,
@@ -647,6 +673,8 @@ def main():
         "pressure_iters": iters,
         "pressure_true_rel_resid": resid,
         "sgs_colors": ncol,
+        "gpdx": {"source": "projected nodal gradient of p on the GPU each step (interior + exposed-face closure)",
+                 "rel_l2_vs_analytic_gradient": gp_rel},
         "pressure_solve_compare": compare,
         "mixed_precision": mixed,
         "stretched_pressure_compare": stretched,

@@ -69,7 +69,7 @@ typedef struct nalu_solver_config {
  * reg_tests/xml/matches_ml_default.xml and milestone.xml; defaults are that
  * first file's values plus MueLu's / Ifpack2's own defaults for the rest). */
 typedef struct nalu_muelu_params {
-  int max_coarse_size; /* "coarse: max size" (1000); the coarsest level is solved densely */
+  int max_coarse_size; /* "coarse: max size" (1000, at most 2048); the coarsest level is solved densely */
   int max_levels;      /* "max levels" (10) */
   int cheb_degree;     /* "chebyshev: degree" (4) */
   double cheb_ratio;   /* "chebyshev: ratio eigenvalue" (20): lambda_min = lambda_max / ratio */
@@ -256,13 +256,24 @@ int nalu_compute_mdot_f32(const nalu_mesh *m, const double *rho_dev, const doubl
  * (ComputeGeometryInteriorAlgorithm, src/ComputeGeometryInteriorAlgorithm.C:77-120);
  * dual_vol_dev [num_nodes] is overwritten.  stream: hipStream_t or NULL. */
 int nalu_compute_dual_volume(const nalu_mesh *m, double *dual_vol_dev, void *stream);
-/* Projected nodal gradient, interior Green-Gauss part (AssembleNodalGradElemAlgorithm,
- * src/AssembleNodalGradElemAlgorithm.C:85-140): dqdx_dev [num_nodes][3] is
- * overwritten with sum_scs (+-) q_ip A / V_node; shifted uses the shifted scs
- * shape functions.  The exposed-face closure (AssembleNodalGradBoundaryAlgorithm)
- * is not part of this call. */
+/* Projected nodal gradient of a nodal scalar q: dqdx_dev [num_nodes][3] is
+ * overwritten with the interior Green-Gauss part (AssembleNodalGradElemAlgorithm,
+ * src/AssembleNodalGradElemAlgorithm.C:85-140: sum_scs (+-) q_ip A / V_node)
+ * plus the exposed-face closure (AssembleNodalGradBoundaryAlgorithm,
+ * src/AssembleNodalGradBoundaryAlgorithm.C:54-152) over every exposed side of
+ * the block; shifted uses the shifted shape functions in both. */
 int nalu_compute_nodal_gradient(const nalu_mesh *m, const double *q_dev, const double *dual_vol_dev,
                                 int shifted, double *dqdx_dev, void *stream);
+/* The interior part alone (overwrites dqdx_dev). */
+int nalu_compute_nodal_gradient_interior(const nalu_mesh *m, const double *q_dev, const double *dual_vol_dev,
+                                         int shifted, double *dqdx_dev, void *stream);
+/* The exposed-face closure alone (adds into dqdx_dev). */
+int nalu_nodal_gradient_boundary(const nalu_mesh *m, const double *q_dev, const double *dual_vol_dev,
+                                 int shifted, double *dqdx_dev, void *stream);
+/* Exposed sides of the block (sides no other element shares), nodes in HexSCS
+ * side order (src/master_element/Hex8CVFEM.C:415-422, outward right-hand
+ * normal): *num_faces, and faces_host [num_faces][4] when not NULL. */
+int nalu_mesh_exposed_faces(const nalu_mesh *m, int64_t *num_faces, int32_t *faces_host);
 /* ContinuityAdvElemKernel (src/kernel/ContinuityAdvElemKernel.C:88-148).
  * Assembly kernels evaluate with FP32 arithmetic (FP64 Jacobians, FP64
  * accumulation into the matrix) when the system was created with precision

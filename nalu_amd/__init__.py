@@ -139,6 +139,9 @@ def lib():
                                             C.c_uint64, _dp]
         L.nalu_compute_dual_volume.argtypes = [_vp, _vp, _vp]
         L.nalu_compute_nodal_gradient.argtypes = [_vp, _vp, _vp, C.c_int, _vp, _vp]
+        L.nalu_compute_nodal_gradient_interior.argtypes = [_vp, _vp, _vp, C.c_int, _vp, _vp]
+        L.nalu_nodal_gradient_boundary.argtypes = [_vp, _vp, _vp, C.c_int, _vp, _vp]
+        L.nalu_mesh_exposed_faces.argtypes = [_vp, _i64p, _i32p]
         # learned preconditioner
         _fp = C.POINTER(C.c_float)
         L.nalu_gnn_create.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp)]
@@ -267,6 +270,16 @@ class Mesh:
         cp, kp = _vp(), _vp()
         _check(lib().nalu_mesh_device_ptrs(self._h, C.byref(cp), C.byref(kp)), "mesh_device_ptrs")
         return cp.value, kp.value
+
+    def exposed_faces(self):
+        """[nfaces, 4] node indices of the block's exposed sides (HexSCS side order)."""
+        n = C.c_int64(0)
+        _check(lib().nalu_mesh_exposed_faces(self._h, C.byref(n), None), "mesh_exposed_faces")
+        faces = np.zeros((n.value, 4), dtype=np.int32)
+        if n.value:
+            _check(lib().nalu_mesh_exposed_faces(self._h, C.byref(n), faces.ctypes.data_as(_i32p)),
+                   "mesh_exposed_faces")
+        return faces
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -776,13 +789,14 @@ def compute_dual_volume(mesh, dual_vol, stream=None):
                                           stream), "compute_dual_volume")
 
 
-def compute_nodal_gradient(mesh, q, dual_vol, dqdx, shifted=False, stream=None):
-    """Interior Green-Gauss nodal gradient [N][3] of the nodal field q."""
+def compute_nodal_gradient(mesh, q, dual_vol, dqdx, shifted=False, stream=None, boundary=True):
+    """Projected nodal gradient [N][3] of the nodal field q: the interior Green-Gauss
+    part plus (boundary=True) the exposed-face closure."""
     N = mesh.num_nodes
-    _check(lib().nalu_compute_nodal_gradient(mesh.handle, _dev(q, "nodal gradient q", "f64", numel=N),
-                                             _dev(dual_vol, "nodal gradient dual volume", "f64", numel=N), int(shifted),
-                                             _dev(dqdx, "nodal gradient dqdx", "f64", numel=3 * N), stream),
-           "compute_nodal_gradient")
+    fn = lib().nalu_compute_nodal_gradient if boundary else lib().nalu_compute_nodal_gradient_interior
+    _check(fn(mesh.handle, _dev(q, "nodal gradient q", "f64", numel=N),
+              _dev(dual_vol, "nodal gradient dual volume", "f64", numel=N), int(shifted),
+              _dev(dqdx, "nodal gradient dqdx", "f64", numel=3 * N), stream), "compute_nodal_gradient")
 
 
 def assemble_continuity(ls, mesh, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False,

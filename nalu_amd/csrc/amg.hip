@@ -49,6 +49,13 @@
 #include "internal.hpp"
 
 namespace nalu {
+
+// Largest coarsest level solved densely.  The Gauss-Jordan step reads and
+// writes the n x 2n augmented matrix once per pivot, 32 n^3 bytes in all:
+// 275 GB (about 45 ms at HBM rate) at 2048 rows; MueLu's default "coarse: max
+// size" is 1000.
+constexpr int64_t kDenseCoarseMax = 2048;
+
 namespace {
 
 constexpr int kT = 256;
@@ -951,8 +958,9 @@ void amg_setup(nalu_linsys* ls) {
   // dense (pseudo-)inverse of the coarsest operator
   Level& Lc = amg->lev.back();
   const int64_t nc = Lc.A.n;
-  NALU_REQUIRE(nc <= 8192, NALU_ERR_UNSUPPORTED,
-               "amg: coarsest level has " + std::to_string(nc) + " rows (dense solve limit 8192)");
+  NALU_REQUIRE(nc <= kDenseCoarseMax, NALU_ERR_UNSUPPORTED,
+               "amg: coarsest level has " + std::to_string(nc) + " rows (dense solve limit " +
+                   std::to_string(kDenseCoarseMax) + ")");
   amg->nc = nc;
   {
     DevBuf<double> M(2 * nc * nc);
@@ -1138,8 +1146,8 @@ using namespace nalu;
 extern "C" int nalu_linsys_set_muelu_params(nalu_linsys* ls, const nalu_muelu_params* p) {
   return guarded([&]() -> int {
     NALU_REQUIRE(ls && p, NALU_ERR_ARG, "set_muelu_params: null argument");
-    NALU_REQUIRE(p->max_coarse_size >= 1 && p->max_coarse_size <= 8192, NALU_ERR_ARG,
-                 "set_muelu_params: coarse: max size must be in [1, 8192]");
+    NALU_REQUIRE(p->max_coarse_size >= 1 && p->max_coarse_size <= kDenseCoarseMax, NALU_ERR_ARG,
+                 "set_muelu_params: coarse: max size must be in [1, " + std::to_string(kDenseCoarseMax) + "]");
     NALU_REQUIRE(p->max_levels >= 1 && p->max_levels <= 32, NALU_ERR_ARG, "set_muelu_params: max levels in [1, 32]");
     NALU_REQUIRE(p->cheb_degree >= 1 && p->cheb_degree <= 16, NALU_ERR_ARG,
                  "set_muelu_params: chebyshev: degree in [1, 16]");

@@ -1153,19 +1153,57 @@ extern "C" int nalu_compute_dual_volume(const nalu_mesh* m, double* dual_vol, vo
   });
 }
 
+namespace {
+void nodal_grad_interior(const nalu_mesh* m, const double* q, const double* dual_vol, int shifted, double* dqdx,
+                         hipStream_t s) {
+  build_node_incidence(const_cast<nalu_mesh*>(m), s);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(blocks_for(m->num_nodes, 256)), dim3(256), 0, s, m->conn.p, m->coords.p, q,
+                       dual_vol, m->inc_ptr.p, m->inc_elem.p, m->num_nodes, dqdx);
+  };
+  if (shifted) go(k_nodal_grad<true>);
+  else go(k_nodal_grad<false>);
+  NALU_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
 extern "C" int nalu_compute_nodal_gradient(const nalu_mesh* m, const double* q, const double* dual_vol, int shifted,
                                            double* dqdx, void* stream) {
   return guarded([&]() -> int {
     NALU_REQUIRE(m && q && dual_vol && dqdx, NALU_ERR_ARG, "nodal_gradient: null argument");
     hipStream_t s = (hipStream_t)stream;
-    build_node_incidence(const_cast<nalu_mesh*>(m), s);
-    auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(blocks_for(m->num_nodes, 256)), dim3(256), 0, s, m->conn.p, m->coords.p, q,
-                         dual_vol, m->inc_ptr.p, m->inc_elem.p, m->num_nodes, dqdx);
-    };
-    if (shifted) go(k_nodal_grad<true>);
-    else go(k_nodal_grad<false>);
-    NALU_HIP_CHECK(hipGetLastError());
+    nodal_grad_interior(m, q, dual_vol, shifted, dqdx, s);
+    nodal_grad_boundary(const_cast<nalu_mesh*>(m), q, dual_vol, shifted != 0, dqdx, s);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_compute_nodal_gradient_interior(const nalu_mesh* m, const double* q, const double* dual_vol,
+                                                    int shifted, double* dqdx, void* stream) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(m && q && dual_vol && dqdx, NALU_ERR_ARG, "nodal_gradient: null argument");
+    nodal_grad_interior(m, q, dual_vol, shifted, dqdx, (hipStream_t)stream);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_nodal_gradient_boundary(const nalu_mesh* m, const double* q, const double* dual_vol, int shifted,
+                                            double* dqdx, void* stream) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(m && q && dual_vol && dqdx, NALU_ERR_ARG, "nodal_gradient_boundary: null argument");
+    nodal_grad_boundary(const_cast<nalu_mesh*>(m), q, dual_vol, shifted != 0, dqdx, (hipStream_t)stream);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_mesh_exposed_faces(const nalu_mesh* m, int64_t* num_faces, int32_t* faces_host) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(m && num_faces, NALU_ERR_ARG, "mesh_exposed_faces: null argument");
+    build_exposed_faces(const_cast<nalu_mesh*>(m), nullptr);
+    *num_faces = m->num_exposed;
+    if (faces_host && m->num_exposed)
+      NALU_HIP_CHECK(hipMemcpy(faces_host, m->exposed_faces.p, sizeof(int32_t) * 4 * m->num_exposed,
+                               hipMemcpyDeviceToHost));
     return NALU_OK;
   });
 }

@@ -308,7 +308,38 @@ __device__ __forceinline__ void scalar_visit(const Op& op, int64_t e, uint32_t p
 // Tile kernel for numDof-1 systems.  Block b handles tile xcd_block(b): the
 // tiles of one XCD are contiguous, so neighbouring tiles share nodal data
 // through that XCD's L2.
-template <class Op>
+// zero n doubles of LDS, four independent stores per thread and pass
+__device__ __forceinline__ void lds_zero(double* seg, int n) {
+  const int B = blockDim.x;
+  for (int i = threadIdx.x; i < n; i += 4 * B) {
+    seg[i] = 0.0;
+    if (i + B < n) seg[i + B] = 0.0;
+    if (i + 2 * B < n) seg[i + 2 * B] = 0.0;
+    if (i + 3 * B < n) seg[i + 3 * B] = 0.0;
+  }
+}
+
+// write (or add) n LDS doubles to dst, four LDS reads in flight per thread
+__device__ __forceinline__ void lds_flush(const double* seg, int n, double* __restrict__ dst, bool accumulate) {
+  const int B = blockDim.x;
+  for (int i = threadIdx.x; i < n; i += 4 * B) {
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i + k * B < n ? seg[i + k * B] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i + k * B < n) {
+        if (accumulate) dst[i + k * B] += v[k];
+        else __builtin_nontemporal_store(v[k], dst + i + k * B);
+      }
+  }
+}
+
+// Tile kernel for numDof-1 systems over a list of tiles.  Block b handles
+// tiles[xcd_block(b)]: the tiles of one XCD are contiguous, so neighbouring
+// tiles share nodal data through that XCD's L2.  FAST: every visit of the
+// listed tiles is a node or edge visit, so only those code paths are built.
+template <class Op, bool FAST>
 __global__ __launch_bounds__(256) void k_scalar_etile(Op op, const int32_t* __restrict__ conn,
                                                       const double* __restrict__ coords,
                                                       const int32_t* __restrict__ lid,
@@ -316,18 +347,20 @@ __global__ __launch_bounds__(256) void k_scalar_etile(Op op, const int32_t* __re
                                                       const uint64_t* __restrict__ elem_pos,
                                                       const int32_t* __restrict__ vptr,
                                                       const int32_t* __restrict__ velem,
-                                                      const uint8_t* __restrict__ vcode, int T, int64_t nrows,
-                                                      int64_t ntiles, double* __restrict__ vals,
+                                                      const uint8_t* __restrict__ vcode,
+                                                      const int32_t* __restrict__ tiles, int64_t ntl, int T,
+                                                      int64_t nrows, double* __restrict__ vals,
                                                       double* __restrict__ rhs, int accumulate) {
   extern __shared__ double seg[];
-  const int64_t tile = xcd_block(blockIdx.x, ntiles);
-  if (tile < 0) return;
+  const int64_t pos = xcd_block(blockIdx.x, ntl);
+  if (pos < 0) return;
+  const int64_t tile = tiles[pos];
   const int32_t a0 = (int32_t)(tile * T);
   const int32_t a1 = (int32_t)min((int64_t)a0 + T, nrows);
   const int32_t s0 = nrp[a0], s1 = nrp[a1];
   const int nseg = s1 - s0, nr = a1 - a0;
   double* rseg = seg + nseg;
-  for (int i = threadIdx.x; i < nseg + nr; i += blockDim.x) seg[i] = 0.0;
+  lds_zero(seg, nseg + nr);
   __syncthreads();
   const int32_t v0 = vptr[tile], v1 = vptr[tile + 1];
   for (int32_t vb = v0; vb < v1; vb += blockDim.x) {
@@ -338,22 +371,18 @@ __global__ __launch_bounds__(256) void k_scalar_etile(Op op, const int32_t* __re
     if (act) {
       const int64_t e = velem[vi];
       const uint32_t perm = kSym.perm[code & 31];
-      switch (cls) {
-        case kClassA: scalar_visit<kClassA>(op, e, perm, conn, coords, lid, nrp, elem_pos, a0, a1, s0, seg, rseg); break;
-        case kClassB: scalar_visit<kClassB>(op, e, perm, conn, coords, lid, nrp, elem_pos, a0, a1, s0, seg, rseg); break;
-        case kClassC: scalar_visit<kClassC>(op, e, perm, conn, coords, lid, nrp, elem_pos, a0, a1, s0, seg, rseg); break;
-        default: scalar_visit<kClassD>(op, e, perm, conn, coords, lid, nrp, elem_pos, a0, a1, s0, seg, rseg); break;
+      if constexpr (FAST) {
+        if (cls == kClassA) scalar_visit<kClassA>(op, e, perm, conn, coords, lid, nrp, elem_pos, a0, a1, s0, seg, rseg);
+        else scalar_visit<kClassB>(op, e, perm, conn, coords, lid, nrp, elem_pos, a0, a1, s0, seg, rseg);
+      } else {
+        if (cls == kClassC) scalar_visit<kClassC>(op, e, perm, conn, coords, lid, nrp, elem_pos, a0, a1, s0, seg, rseg);
+        else scalar_visit<kClassD>(op, e, perm, conn, coords, lid, nrp, elem_pos, a0, a1, s0, seg, rseg);
       }
     }
   }
   __syncthreads();
-  if (accumulate) {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[a0 + i] += rseg[i];
-  } else {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store(seg[i], vals + s0 + i);
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[a0 + i] = rseg[i];
-  }
+  lds_flush(seg, nseg, vals + s0, accumulate != 0);
+  lds_flush(rseg, nr, rhs + a0, accumulate != 0);
 }
 
 // ---------------------------------------------------------------- momentum, one scs
@@ -482,8 +511,8 @@ __device__ __forceinline__ void momentum_visit(int64_t e, int code, const int32_
 
 // Momentum (numDof 3): the segment holds the 3 rows of each owned node
 // (rows 3a..3a+2 are contiguous in the CSR, 3L entries each).
-template <bool SKEW, bool SHIFT, typename R>
-__global__ __launch_bounds__(256) void k_momentum_etile(const int32_t* __restrict__ conn,
+template <bool SKEW, bool SHIFT, typename R, bool FAST>
+__global__ __launch_bounds__(128) void k_momentum_etile(const int32_t* __restrict__ conn,
                                                         const double* __restrict__ coords,
                                                         const double* __restrict__ u, const double* __restrict__ mu,
                                                         const double* __restrict__ mdot, double incl_divu,
@@ -492,18 +521,20 @@ __global__ __launch_bounds__(256) void k_momentum_etile(const int32_t* __restric
                                                         const uint64_t* __restrict__ elem_pos,
                                                         const int32_t* __restrict__ vptr,
                                                         const int32_t* __restrict__ velem,
-                                                        const uint8_t* __restrict__ vcode, int T, int64_t nnodes,
-                                                        int64_t ntiles, double* __restrict__ vals,
+                                                        const uint8_t* __restrict__ vcode,
+                                                        const int32_t* __restrict__ tiles, int64_t ntl, int T,
+                                                        int64_t nnodes, double* __restrict__ vals,
                                                         double* __restrict__ rhs, int accumulate) {
   extern __shared__ double seg[];
-  const int64_t tile = xcd_block(blockIdx.x, ntiles);
-  if (tile < 0) return;
+  const int64_t pos = xcd_block(blockIdx.x, ntl);
+  if (pos < 0) return;
+  const int64_t tile = tiles[pos];
   const int32_t a0 = (int32_t)(tile * T);
   const int32_t a1 = (int32_t)min((int64_t)a0 + T, nnodes);
   const int32_t s0 = nrp[a0];
   const int nseg = 9 * (nrp[a1] - s0), nr = 3 * (a1 - a0);
   double* rseg = seg + nseg;
-  for (int i = threadIdx.x; i < nseg + nr; i += blockDim.x) seg[i] = 0.0;
+  lds_zero(seg, nseg + nr);
   __syncthreads();
   const int32_t v0 = vptr[tile], v1 = vptr[tile + 1];
   for (int32_t vb = v0; vb < v1; vb += blockDim.x) {
@@ -513,27 +544,201 @@ __global__ __launch_bounds__(256) void k_momentum_etile(const int32_t* __restric
     const int cls = wave_max(act ? code >> 5 : 0);
     if (act) {
       const int64_t e = velem[vi];
-      switch (cls) {
 #define NALU_MV(K)                                                                                             \
-  case K:                                                                                                      \
-    momentum_visit<K, SKEW, SHIFT, R>(e, code, conn, coords, u, mu, mdot, incl_divu, lid, nrp, elem_pos, a0, a1, \
-                                      s0, seg, rseg);                                                          \
-    break;
-        NALU_MV(kClassA) NALU_MV(kClassB) NALU_MV(kClassC)
-        default: NALU_MV(kClassD)
+  momentum_visit<K, SKEW, SHIFT, R>(e, code, conn, coords, u, mu, mdot, incl_divu, lid, nrp, elem_pos, a0, a1, s0, \
+                                    seg, rseg)
+      if constexpr (FAST) {
+        if (cls == kClassA) NALU_MV(kClassA);
+        else NALU_MV(kClassB);
+      } else {
+        if (cls == kClassC) NALU_MV(kClassC);
+        else NALU_MV(kClassD);
+      }
 #undef NALU_MV
+    }
+  }
+  __syncthreads();
+  lds_flush(seg, nseg, vals + 9 * (int64_t)s0, accumulate != 0);
+  lds_flush(rseg, nr, rhs + 3 * (int64_t)a0, accumulate != 0);
+}
+
+// ---------------------------------------------------------------- item kernels
+// One thread per (visit, scs) for tiles of node and edge visits: wavefront w of
+// a block takes scs w of the edge class's five ({0, 1, 3, 8, 9} in the
+// canonical frame) for 64 visits at a time, so every wavefront runs one
+// compile-time scs and a block keeps five times as many wavefronts in flight
+// per tile as one thread per visit.  Node visits (class A) ride along: scs 1
+// and 9 touch no node they own.
+constexpr int kItemSlots = 5;
+__host__ __device__ constexpr int item_ip(int slot) {
+  return slot == 0 ? 0 : slot == 1 ? 1 : slot == 2 ? 3 : slot == 3 ? 8 : 9;
+}
+
+template <int IP, class Op>
+__device__ __forceinline__ void scalar_item(const Op& op, int64_t e, uint32_t perm, const int32_t* __restrict__ conn,
+                                            const double* __restrict__ coords, const int32_t* __restrict__ lid,
+                                            const int32_t* __restrict__ nrp, const uint64_t* __restrict__ elem_pos,
+                                            int32_t a0, int32_t a1, int32_t s0, double* seg, double* rseg) {
+  constexpr int il = scs_left(IP), ir = scs_right(IP);
+  constexpr bool Lc = (kClassNodes[kClassB] >> il) & 1u, Rc = (kClassNodes[kClassB] >> ir) & 1u;
+  Visit v;
+  load_visit<kClassB>(v, e, perm, conn, lid, elem_pos, a0, a1);
+  const bool lo = Lc && v.own[il], ro = Rc && v.own[ir];
+  if (!(lo || ro)) return;
+  typename Op::Elem el;
+  op.load(v, coords, el);
+  const int32_t bl = lo ? nrp[v.row[il]] - s0 : 0, br = ro ? nrp[v.row[ir]] - s0 : 0;
+  const auto r = op.template scs<IP>(el, [&](int c, auto val) {
+    const double d = (double)val;
+    if (lo) unsafeAtomicAdd(seg + bl + col_pos(v, il, c), d);
+    if (ro) unsafeAtomicAdd(seg + br + col_pos(v, ir, c), -d);
+  });
+  if (lo) unsafeAtomicAdd(rseg + (v.row[il] - a0), (double)r);
+  if (ro) unsafeAtomicAdd(rseg + (v.row[ir] - a0), -(double)r);
+}
+
+template <class Op>
+__global__ __launch_bounds__(64 * kItemSlots) void k_scalar_items(
+    Op op, const int32_t* __restrict__ conn, const double* __restrict__ coords, const int32_t* __restrict__ lid,
+    const int32_t* __restrict__ nrp, const uint64_t* __restrict__ elem_pos, const int32_t* __restrict__ vptr,
+    const int32_t* __restrict__ velem, const uint8_t* __restrict__ vcode, const int32_t* __restrict__ tiles,
+    int64_t ntl, int T, int64_t nrows, double* __restrict__ vals, double* __restrict__ rhs, int accumulate) {
+  extern __shared__ double seg[];
+  const int64_t pos = xcd_block(blockIdx.x, ntl);
+  if (pos < 0) return;
+  const int64_t tile = tiles[pos];
+  const int32_t a0 = (int32_t)(tile * T);
+  const int32_t a1 = (int32_t)min((int64_t)a0 + T, nrows);
+  const int32_t s0 = nrp[a0], s1 = nrp[a1];
+  const int nseg = s1 - s0, nr = a1 - a0;
+  double* rseg = seg + nseg;
+  lds_zero(seg, nseg + nr);
+  __syncthreads();
+  const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int32_t v0 = vptr[tile], v1 = vptr[tile + 1];
+  for (int32_t g = v0; g < v1; g += 64) {
+    const int32_t vi = g + lane;
+    if (vi < v1) {
+      const int64_t e = velem[vi];
+      const uint32_t perm = kSym.perm[vcode[vi] & 31];
+      switch (slot) {
+#define NALU_SI(S)                                                                                            \
+  case S:                                                                                                     \
+    scalar_item<item_ip(S)>(op, e, perm, conn, coords, lid, nrp, elem_pos, a0, a1, s0, seg, rseg);            \
+    break;
+        NALU_SI(0) NALU_SI(1) NALU_SI(2) NALU_SI(3) NALU_SI(4)
+#undef NALU_SI
       }
     }
   }
   __syncthreads();
-  const int64_t o = 9 * (int64_t)s0;
-  if (accumulate) {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[o + i] += seg[i];
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * (int64_t)a0 + i] += rseg[i];
-  } else {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store(seg[i], vals + o + i);
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * (int64_t)a0 + i] = rseg[i];
+  lds_flush(seg, nseg, vals + s0, accumulate != 0);
+  lds_flush(rseg, nr, rhs + a0, accumulate != 0);
+}
+
+template <int IP, bool SKEW, bool SHIFT, typename R>
+__device__ __forceinline__ void momentum_item(int64_t e, int code, const int32_t* __restrict__ conn,
+                                              const double* __restrict__ coords, const double* __restrict__ u,
+                                              const double* __restrict__ mu, const double* __restrict__ mdot,
+                                              double incl_divu, const int32_t* __restrict__ lid,
+                                              const int32_t* __restrict__ nrp,
+                                              const uint64_t* __restrict__ elem_pos, int32_t a0, int32_t a1,
+                                              int32_t s0, double* seg, double* rseg) {
+  constexpr int il = scs_left(IP), ir = scs_right(IP);
+  constexpr bool Lc = (kClassNodes[kClassB] >> il) & 1u, Rc = (kClassNodes[kClassB] >> ir) & 1u;
+  Visit v;
+  const uint32_t perm = kSym.perm[code & 31];
+  load_visit<kClassB>(v, e, perm, conn, lid, elem_pos, a0, a1);
+  const bool lo = Lc && v.own[il], ro = Rc && v.own[ir];
+  if (!(lo || ro)) return;
+  R x[8][3], uu[8][3], mm[8];
+  gather_coords(coords, v.nd, x);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int64_t a = v.nd[c];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) uu[c][j] = R(u[3 * a + j]);
+    mm[c] = R(mu[a]);
   }
+  int32_t bl = 0, br = 0, Ll = 0, Lr = 0;
+  if (lo) {
+    const int32_t b = nrp[v.row[il]];
+    bl = 9 * (b - s0);
+    Ll = nrp[v.row[il] + 1] - b;
+  }
+  if (ro) {
+    const int32_t b = nrp[v.row[ir]];
+    br = 9 * (b - s0);
+    Lr = nrp[v.row[ir] + 1] - b;
+  }
+  const int q = (int)((kSym.ipmap[code & 31] >> (5 * IP)) & 31u);
+  const double m = mdot[12 * e + (q & 15)];
+  const R mdc = R((q & 16) ? -m : m);
+  R rr[3];
+  momentum_scs<IP, SKEW, SHIFT, false>(x, uu, mm, mdc, R(incl_divu), rr, [&](int c, const R (&blk)[3][3]) {
+    if (lo) {
+      double* dst = seg + bl + 3 * col_pos(v, il, c);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) unsafeAtomicAdd(dst + 3 * i * Ll + j, (double)blk[i][j]);
+    }
+    if (ro) {
+      double* dst = seg + br + 3 * col_pos(v, ir, c);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) unsafeAtomicAdd(dst + 3 * i * Lr + j, -(double)blk[i][j]);
+    }
+  });
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (lo) unsafeAtomicAdd(rseg + 3 * (v.row[il] - a0) + i, (double)rr[i]);
+    if (ro) unsafeAtomicAdd(rseg + 3 * (v.row[ir] - a0) + i, -(double)rr[i]);
+  }
+}
+
+template <bool SKEW, bool SHIFT, typename R>
+__global__ __launch_bounds__(64 * kItemSlots) void k_momentum_items(
+    const int32_t* __restrict__ conn, const double* __restrict__ coords, const double* __restrict__ u,
+    const double* __restrict__ mu, const double* __restrict__ mdot, double incl_divu, const int32_t* __restrict__ lid,
+    const int32_t* __restrict__ nrp, const uint64_t* __restrict__ elem_pos, const int32_t* __restrict__ vptr,
+    const int32_t* __restrict__ velem, const uint8_t* __restrict__ vcode, const int32_t* __restrict__ tiles,
+    int64_t ntl, int T, int64_t nnodes, double* __restrict__ vals, double* __restrict__ rhs, int accumulate) {
+  extern __shared__ double seg[];
+  const int64_t pos = xcd_block(blockIdx.x, ntl);
+  if (pos < 0) return;
+  const int64_t tile = tiles[pos];
+  const int32_t a0 = (int32_t)(tile * T);
+  const int32_t a1 = (int32_t)min((int64_t)a0 + T, nnodes);
+  const int32_t s0 = nrp[a0];
+  const int nseg = 9 * (nrp[a1] - s0), nr = 3 * (a1 - a0);
+  double* rseg = seg + nseg;
+  lds_zero(seg, nseg + nr);
+  __syncthreads();
+  const int slot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int32_t v0 = vptr[tile], v1 = vptr[tile + 1];
+  for (int32_t g = v0; g < v1; g += 64) {
+    const int32_t vi = g + lane;
+    if (vi < v1) {
+      const int64_t e = velem[vi];
+      const int code = vcode[vi];
+      switch (slot) {
+#define NALU_MI(S)                                                                                              \
+  case S:                                                                                                       \
+    momentum_item<item_ip(S), SKEW, SHIFT, R>(e, code, conn, coords, u, mu, mdot, incl_divu, lid, nrp, elem_pos, \
+                                              a0, a1, s0, seg, rseg);                                           \
+    break;
+        NALU_MI(0) NALU_MI(1) NALU_MI(2) NALU_MI(3) NALU_MI(4)
+#undef NALU_MI
+      }
+    }
+  }
+  __syncthreads();
+  lds_flush(seg, nseg, vals + 9 * (int64_t)s0, accumulate != 0);
+  lds_flush(rseg, nr, rhs + 3 * (int64_t)a0, accumulate != 0);
 }
 
 // ---------------------------------------------------------------- tile plan build
@@ -585,6 +790,13 @@ __global__ void k_visit_max(const int32_t* __restrict__ vptr, int64_t ntiles, in
   if (t < ntiles) atomicMax(mx, vptr[t + 1] - vptr[t]);
 }
 
+// largest visit class of each tile (visits are sorted by class inside a tile)
+__global__ void k_tile_class(const int32_t* __restrict__ vptr, const uint8_t* __restrict__ vcode, int64_t ntiles,
+                             uint8_t* __restrict__ cls) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntiles) cls[t] = vptr[t + 1] > vptr[t] ? (uint8_t)(vcode[vptr[t + 1] - 1] >> 5) : (uint8_t)0;
+}
+
 int bitsof(uint64_t v) {
   int b = 1;
   while (b < 64 && (v >> b) != 0) ++b;
@@ -597,6 +809,9 @@ struct TilePlan {
   int max_visits = 0;
   DevBuf<int32_t> vptr, velem;
   DevBuf<uint8_t> vcode;
+  // tiles whose visits are all node or edge visits (the compact kernel) and the rest
+  DevBuf<int32_t> fast_tiles, other_tiles;
+  int64_t nfast = 0, nother = 0;
 };
 
 struct Plans {
@@ -653,7 +868,21 @@ void build_plan(nalu_linsys* ls, TilePlan& P, int T) {
   NALU_HIP_CHECK(hipMemsetAsync(mx.p, 0, sizeof(int), s));
   k_visit_max<<<grid_of(P.ntiles), kB, 0, s>>>(P.vptr.p, P.ntiles, mx.p);
   NALU_HIP_CHECK(hipMemcpyAsync(&P.max_visits, mx.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  DevBuf<uint8_t> tcls(P.ntiles);
+  k_tile_class<<<grid_of(P.ntiles), kB, 0, s>>>(P.vptr.p, P.vcode.p, P.ntiles, tcls.p);
+  std::vector<uint8_t> hc(P.ntiles);
+  NALU_HIP_CHECK(hipMemcpyAsync(hc.data(), tcls.p, P.ntiles, hipMemcpyDeviceToHost, s));
   NALU_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<int32_t> fast, other;
+  for (int64_t t = 0; t < P.ntiles; ++t) (hc[t] <= kClassB ? fast : other).push_back((int32_t)t);
+  P.nfast = (int64_t)fast.size();
+  P.nother = (int64_t)other.size();
+  P.fast_tiles.alloc(std::max<int64_t>(1, P.nfast));
+  P.other_tiles.alloc(std::max<int64_t>(1, P.nother));
+  if (P.nfast)
+    NALU_HIP_CHECK(hipMemcpy(P.fast_tiles.p, fast.data(), sizeof(int32_t) * P.nfast, hipMemcpyHostToDevice));
+  if (P.nother)
+    NALU_HIP_CHECK(hipMemcpy(P.other_tiles.p, other.data(), sizeof(int32_t) * P.nother, hipMemcpyHostToDevice));
   NALU_HIP_CHECK(hipGetLastError());
 }
 
@@ -685,12 +914,12 @@ void etile_release(nalu_linsys* ls) {
   ls->etile_plans = nullptr;
 }
 
-// Tile sizes: scalar systems keep 62 rows (27-entry rows: 14 KB of LDS), the
-// momentum system 62 nodes (3 rows of 81 entries each: 121 KB of LDS).  With
-// rows following lattice lines a tile of T rows has at most 4T + 8 visits, so
-// 62 keeps them within one pass of 256 threads.
+// Tile sizes.  With rows following lattice lines a tile of T rows has at most
+// 4T + 8 visits: scalar systems take 62 rows (27-entry rows, 14 KB of LDS) for
+// one pass of 256 threads; the momentum system 29 nodes (3 rows of 81 entries
+// each, 57 KB of LDS, two blocks per CU) for one pass of 128 threads.
 constexpr int kScalarT = 62;
-constexpr int kMomentumT = 62;
+constexpr int kMomentumT = 29;
 
 template <class Op>
 void etile_scalar(nalu_linsys* ls, const nalu_mesh* m, const Op& op) {
@@ -699,10 +928,20 @@ void etile_scalar(nalu_linsys* ls, const nalu_mesh* m, const Op& op) {
                "tile assembly: values/rhs zero state diverged");
   const int accumulate = ls->values_pending_zero ? 0 : 1;
   const size_t lds = ((size_t)P.T * ls->max_node_row_len + P.T) * sizeof(double);
-  set_lds_limit((const void*)k_scalar_etile<Op>, lds);
-  hipLaunchKernelGGL((k_scalar_etile<Op>), dim3((unsigned)xcd_grid(P.ntiles)), dim3(256), lds, ls->stream, op,
-                     m->conn.p, m->coords.p, ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, P.vptr.p, P.velem.p,
-                     P.vcode.p, P.T, ls->num_nodes, P.ntiles, ls->values.p, ls->rhs.p, accumulate);
+  auto go = [&](auto kern, const int32_t* tiles, int64_t ntl) {
+    if (ntl == 0) return;
+    set_lds_limit((const void*)kern, lds);
+    hipLaunchKernelGGL(kern, dim3((unsigned)xcd_grid(ntl)), dim3(256), lds, ls->stream, op, m->conn.p, m->coords.p,
+                       ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, P.vptr.p, P.velem.p, P.vcode.p, tiles, ntl, P.T,
+                       ls->num_nodes, ls->values.p, ls->rhs.p, accumulate);
+  };
+  if (P.nfast) {
+    set_lds_limit((const void*)k_scalar_items<Op>, lds);
+    hipLaunchKernelGGL(k_scalar_items<Op>, dim3((unsigned)xcd_grid(P.nfast)), dim3(64 * kItemSlots), lds, ls->stream,
+                       op, m->conn.p, m->coords.p, ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, P.vptr.p, P.velem.p,
+                       P.vcode.p, P.fast_tiles.p, P.nfast, P.T, ls->num_nodes, ls->values.p, ls->rhs.p, accumulate);
+  }
+  go(k_scalar_etile<Op, false>, P.other_tiles.p, P.nother);
   NALU_HIP_CHECK(hipGetLastError());
   ls->values_pending_zero = ls->rhs_pending_zero = false;
 }
@@ -745,16 +984,29 @@ void etile_momentum(nalu_linsys* ls, const nalu_mesh* m, const double* u, const 
                "tile assembly: values/rhs zero state diverged");
   const int accumulate = ls->values_pending_zero ? 0 : 1;
   const size_t lds = ((size_t)P.T * ls->max_node_row_len * 9 + 3 * P.T) * sizeof(double);
-  auto go = [&](auto kern) {
+  auto go1 = [&](auto kern, const int32_t* tiles, int64_t ntl) {
+    if (ntl == 0) return;
     set_lds_limit((const void*)kern, lds);
-    hipLaunchKernelGGL(kern, dim3((unsigned)xcd_grid(P.ntiles)), dim3(256), lds, ls->stream, m->conn.p, m->coords.p,
-                       u, mu, mdot, include_divu, ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, P.vptr.p, P.velem.p,
-                       P.vcode.p, P.T, ls->num_nodes, P.ntiles, ls->values.p, ls->rhs.p, accumulate);
+    hipLaunchKernelGGL(kern, dim3((unsigned)xcd_grid(ntl)), dim3(128), lds, ls->stream, m->conn.p, m->coords.p, u,
+                       mu, mdot, include_divu, ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, P.vptr.p, P.velem.p,
+                       P.vcode.p, tiles, ntl, P.T, ls->num_nodes, ls->values.p, ls->rhs.p, accumulate);
   };
-  if (skew && shifted) fp32 ? go(k_momentum_etile<true, true, float>) : go(k_momentum_etile<true, true, double>);
-  else if (skew) fp32 ? go(k_momentum_etile<true, false, float>) : go(k_momentum_etile<true, false, double>);
-  else if (shifted) fp32 ? go(k_momentum_etile<false, true, float>) : go(k_momentum_etile<false, true, double>);
-  else fp32 ? go(k_momentum_etile<false, false, float>) : go(k_momentum_etile<false, false, double>);
+  auto go = [&](auto fast, auto other) {
+    if (P.nfast) {
+      set_lds_limit((const void*)fast, lds);
+      hipLaunchKernelGGL(fast, dim3((unsigned)xcd_grid(P.nfast)), dim3(64 * kItemSlots), lds, ls->stream, m->conn.p,
+                         m->coords.p, u, mu, mdot, include_divu, ls->node_lid.p, ls->nrp.p, ls->elem_pos.p, P.vptr.p,
+                         P.velem.p, P.vcode.p, P.fast_tiles.p, P.nfast, P.T, ls->num_nodes, ls->values.p, ls->rhs.p,
+                         accumulate);
+    }
+    go1(other, P.other_tiles.p, P.nother);
+  };
+#define NALU_MG(SK, SH, R) go(k_momentum_items<SK, SH, R>, k_momentum_etile<SK, SH, R, false>)
+  if (skew && shifted) fp32 ? NALU_MG(true, true, float) : NALU_MG(true, true, double);
+  else if (skew) fp32 ? NALU_MG(true, false, float) : NALU_MG(true, false, double);
+  else if (shifted) fp32 ? NALU_MG(false, true, float) : NALU_MG(false, true, double);
+  else fp32 ? NALU_MG(false, false, float) : NALU_MG(false, false, double);
+#undef NALU_MG
   NALU_HIP_CHECK(hipGetLastError());
   ls->values_pending_zero = ls->rhs_pending_zero = false;
 }

@@ -76,6 +76,26 @@ struct DevBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Page-locked host buffer (released with its owner).
+struct HostPinned {
+  void* p = nullptr;
+  size_t n = 0;
+  HostPinned() = default;
+  HostPinned(const HostPinned&) = delete;
+  HostPinned& operator=(const HostPinned&) = delete;
+  ~HostPinned() {
+    if (p) (void)hipHostFree(p);
+  }
+  void alloc(size_t bytes) {
+    if (bytes <= n) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    NALU_HIP_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    n = bytes;
+  }
+};
+
 // ---------------------------------------------------------------- timers
 enum Phase { PH_INIT = 0, PH_ZERO, PH_ASSEMBLE, PH_DIRICHLET, PH_LOADCOMPLETE, PH_PRECOND, PH_SOLVE, PH_COUNT };
 
@@ -96,6 +116,10 @@ struct nalu_mesh {
   nalu::DevBuf<int32_t> inc_ptr;   // [N+1]
   nalu::DevBuf<int32_t> inc_elem;  // [8E] element*8 + local node index, sorted per node
   int max_incidence = 0;
+  // exposed sides (built on demand): [num_exposed][4] nodes in HexSCS side order
+  bool exposed_ready = false;
+  int64_t num_exposed = 0;
+  nalu::DevBuf<int32_t> exposed_faces;
 };
 
 // ---------------------------------------------------------- linear system
@@ -165,9 +189,17 @@ struct nalu_linsys {
   // together in time; the 8 XCDs take contiguous eighths of the order
   nalu::DevBuf<int32_t> sell_sched;
   nalu::DevBuf<int32_t> sl_col, su_col, sl_src, su_src;
+  // compressed columns: per 64-entry slot a base column, per entry the 8-bit
+  // offset from it (255 = escape to the int32 column above)
+  nalu::DevBuf<uint8_t> sl_off, su_off;
+  nalu::DevBuf<int32_t> sl_base, su_base;
   nalu::DevBuf<double> sl_val, su_val;
   nalu::DevBuf<float> sl_val32, su_val32;
   nalu::DevBuf<double> pwork;  // permuted in/out vectors of precond_apply
+  // workspace of the fused CG (vectors, partial sums, device state), kept
+  // across solves, and its page-locked host mirror of the state
+  nalu::DevBuf<unsigned char> cg_work;
+  nalu::HostPinned cg_host;
   // layout of the vectors of a stacked multi-rhs solve while it runs: false =
   // [nrhs][rows] slices, true = interleaved [rows][nrhs] (batched kernels)
   bool stack_interleaved = false;
@@ -243,6 +275,9 @@ void etile_continuity(nalu_linsys* ls, const nalu_mesh* m, const double* rho, co
 void etile_momentum(nalu_linsys* ls, const nalu_mesh* m, const double* u, const double* mu, const double* mdot,
                     double include_divu, bool skew, bool shifted, bool fp32);
 void etile_release(nalu_linsys* ls);
+// nodalgrad.hip: exposed sides and the boundary closure of the nodal gradient
+void build_exposed_faces(nalu_mesh* m, hipStream_t s);
+void nodal_grad_boundary(nalu_mesh* m, const double* q, const double* vol, bool shifted, double* dqdx, hipStream_t s);
 // suminto.hip: queue of host sumInto calls, applied before anything else touches the system
 void flush_sum_into_queue(nalu_linsys* ls);
 void discard_sum_into_queue(nalu_linsys* ls);

@@ -331,6 +331,80 @@ __global__ void k_cg_p(double* __restrict__ p, const double* __restrict__ z, con
   if (i < n) p[i] = z[i] + (rz_new[0] / rz_old[0]) * p[i];
 }
 
+// ------------------------------------------------------------ fused CG building blocks
+// Device-side CG for one right-hand side on the SELL copy with the SGS
+// preconditioner: the dot products ride on the kernels that produce their
+// operands (SpMV: p.q, the update: r.r, the sweeps that finalize z: r.z) as
+// per-block partial sums, which one single-block launch reduces in a fixed
+// order (deterministic).  No grid-wide election inside the producing kernels:
+// on a multi-XCD part every device-scope release writes back the XCD's L2, and
+// thousands of blocks doing so per launch cost more than the launch saved.
+// The convergence test is a device flag the host reads every few iterations,
+// so an iteration costs no host round trip; kernels of an iteration after
+// convergence return at once.
+struct CgState {
+  double r0;       // |r_0|
+  double tol2r02;  // (tol |r_0|)^2
+  double rz[2];    // r.z, current and next (alternating slots)
+  double pq;       // p.q
+  double rr;       // r.r
+  int done;        // 1 converged, 2 non-finite residual
+  int iters;       // iteration at which done was set
+};
+
+// One block: sum of part[0, nb) in a fixed order into the CgState field
+// `which` (0 p.q, 1 r.r with the convergence test at iteration it, 2 r.z of
+// slot `slot`).
+__global__ __launch_bounds__(kThreads) void k_cg_reduce(const double* __restrict__ part, int nb, int which,
+                                                        int slot, int it, CgState* __restrict__ st) {
+  if (st->done) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+  s = block_sum(s);
+  if (threadIdx.x != 0) return;
+  if (which == 0) {
+    st->pq = s;
+  } else if (which == 2) {
+    st->rz[slot] = s;
+  } else {
+    st->rr = s;
+    if (!(s == s) || s > 1e300) {
+      st->done = 2;
+      st->iters = it;
+    } else if (s <= st->tol2r02) {
+      st->done = 1;
+      st->iters = it;
+    }
+  }
+}
+
+// x += alpha p, r -= alpha q (alpha = r.z / p.q); per-block partials of r.r
+__global__ __launch_bounds__(kThreads) void k_cg_update_f(double* __restrict__ x, double* __restrict__ r,
+                                                          const double* __restrict__ p,
+                                                          const double* __restrict__ q, int64_t n, int cur,
+                                                          double* __restrict__ part, const CgState* __restrict__ st) {
+  if (st->done) return;
+  const double alpha = st->rz[cur] / st->pq;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// p = z + (r.z_new / r.z_old) p
+__global__ void k_cg_p_f(double* __restrict__ p, const double* __restrict__ z, int64_t n, int nxt, int cur,
+                         const CgState* __restrict__ st) {
+  if (st->done) return;
+  const double beta = st->rz[nxt] / st->rz[cur];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + beta * p[i];
+}
+
 // ------------------------------------------------------------ host helpers
 struct Red {
   nalu_linsys* ls;
@@ -600,6 +674,20 @@ __global__ __launch_bounds__(kThreads) void k_bwd_color_vec(const int32_t* __res
 }
 
 // ------------------------------------------------------------ SELL-64 kernels
+// Column table of one part (L or U) of the SELL copy: 8-bit offsets from a
+// per-slot base, with the int32 columns as the escape table.
+struct SellCols {
+  const uint8_t* off;
+  const int32_t* base;
+  const int32_t* col;
+  __device__ __forceinline__ int32_t at(int64_t slot, int64_t idx) const {
+    const uint32_t o = __builtin_nontemporal_load(off + idx);
+    int32_t j = base[slot] + (int32_t)o;
+    if (o == 255u) j = col[idx];
+    return j;
+  }
+};
+
 // One wavefront per chunk, one lane per row: every load of column indices and
 // values is a fully coalesced 64-lane access, no reductions or LDS.
 // chunk widths = longest L / U part of the chunk's rows
@@ -645,6 +733,46 @@ __global__ __launch_bounds__(kThreads) void k_sell_fill(const SellChunk* __restr
   }
 }
 
+// Column compression of one SELL slot (64 entries, one wavefront): base = the
+// real column that starts the window [base, base + 254] holding the most real
+// entries of the slot (smallest on ties); an entry stores col - base in 8 bits,
+// 255 = escape to the int32 column table.  Padding entries point at base (their
+// value is 0, and base is a column the sweeps read as final).  A slot without
+// real entries keeps the padding column of its first lane.
+__global__ __launch_bounds__(kThreads) void k_sell_compress(const int32_t* __restrict__ col,
+                                                            const int32_t* __restrict__ src, int64_t nslots,
+                                                            uint8_t* __restrict__ off, int32_t* __restrict__ base) {
+  const int64_t s = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= nslots) return;  // the whole wavefront leaves together
+  const int64_t at = 64 * s + lane;
+  const int32_t c = col[at];
+  const int real = src[at] >= 0 ? 1 : 0;
+  int cnt = 0;
+  for (int k = 0; k < 64; ++k) {
+    const int32_t ck = __shfl(c, k, 64);
+    const int rk = __shfl(real, k, 64);
+    const int64_t d = (int64_t)ck - (int64_t)c;
+    cnt += (rk && d >= 0 && d <= 254) ? 1 : 0;
+  }
+  int best = real ? cnt : -1;
+  int32_t bcol = c;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const int oc = __shfl_xor(best, o, 64);
+    const int32_t ocol = __shfl_xor(bcol, o, 64);
+    if (oc > best || (oc == best && ocol < bcol)) {
+      best = oc;
+      bcol = ocol;
+    }
+  }
+  const int32_t c0 = __shfl(c, 0, 64);
+  const int32_t b = best < 0 ? c0 : bcol;
+  if (lane == 0) base[s] = b;
+  const int64_t d = (int64_t)c - (int64_t)b;
+  off[at] = !real ? (uint8_t)0 : (d >= 0 && d <= 254 ? (uint8_t)d : (uint8_t)255);
+}
+
 template <typename T>
 __global__ void k_take_sell(const int32_t* __restrict__ src, const double* __restrict__ v, int64_t n,
                             T* __restrict__ out) {
@@ -656,8 +784,8 @@ __global__ void k_take_sell(const int32_t* __restrict__ src, const double* __res
 // entries stream through once (non-temporal loads), the gathered x stays cached
 template <typename T, typename TY>
 __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restrict__ ch, int64_t nch,
-                                                        const int32_t* __restrict__ lcol, const T* __restrict__ lval,
-                                                        const int32_t* __restrict__ ucol, const T* __restrict__ uval,
+                                                        const SellCols lcs, const T* __restrict__ lval,
+                                                        const SellCols ucs, const T* __restrict__ uval,
                                                         const T* __restrict__ dv, const T* __restrict__ x,
                                                         TY* __restrict__ y, const TY* __restrict__ b, int mode) {
   const int64_t c = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
@@ -667,16 +795,16 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restr
   const bool act = lane < k.nrows;
   const int row = k.row0 + (act ? lane : 0);
   T s = dv[row] * x[row];
-  const int32_t* lc = lcol + k.loff + lane;
-  const T* lv = lval + k.loff + lane;
 #pragma unroll 4
-  for (int q = 0; q < k.lw; ++q)
-    s += __builtin_nontemporal_load(&lv[q * 64]) * x[__builtin_nontemporal_load(&lc[q * 64])];
-  const int32_t* uc = ucol + k.uoff + lane;
-  const T* uv = uval + k.uoff + lane;
+  for (int q = 0; q < k.lw; ++q) {
+    const int64_t at = k.loff + (int64_t)q * 64 + lane;
+    s += __builtin_nontemporal_load(&lval[at]) * x[lcs.at((k.loff >> 6) + q, at)];
+  }
 #pragma unroll 4
-  for (int q = 0; q < k.uw; ++q)
-    s += __builtin_nontemporal_load(&uv[q * 64]) * x[__builtin_nontemporal_load(&uc[q * 64])];
+  for (int q = 0; q < k.uw; ++q) {
+    const int64_t at = k.uoff + (int64_t)q * 64 + lane;
+    s += __builtin_nontemporal_load(&uval[at]) * x[ucs.at((k.uoff >> 6) + q, at)];
+  }
   if (act) y[row] = mode ? (TY)((T)b[row] - s) : (TY)s;
 }
 
@@ -688,7 +816,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restr
 // color's chunks, so neighbouring rows gather z through the same L2.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
-                                                       const int32_t* __restrict__ lcol, const T* __restrict__ lval,
+                                                       const SellCols lcs, const T* __restrict__ lval,
                                                        const T* __restrict__ invd, const T* __restrict__ r,
                                                        T* __restrict__ z) {
   __shared__ T part[kThreads / 2];
@@ -704,11 +832,11 @@ __global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restri
   const bool act = ok && lane < k.nrows;
   T s = T(0);
   if (act) {
-    const int32_t* lc = lcol + k.loff + lane;
-    const T* lv = lval + k.loff + lane;
 #pragma unroll 4
-    for (int q = half; q < k.lw; q += 2)
-      s += __builtin_nontemporal_load(&lv[q * 64]) * z[__builtin_nontemporal_load(&lc[q * 64])];
+    for (int q = half; q < k.lw; q += 2) {
+      const int64_t at = k.loff + (int64_t)q * 64 + lane;
+      s += __builtin_nontemporal_load(&lval[at]) * z[lcs.at((k.loff >> 6) + q, at)];
+    }
   }
   if (half == 1) part[slot] = s;
   __syncthreads();
@@ -721,7 +849,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restri
 // backward sweep over the chunks of one color, in place: z = y - (U z) / d
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
-                                                       const int32_t* __restrict__ ucol, const T* __restrict__ uval,
+                                                       const SellCols ucs, const T* __restrict__ uval,
                                                        const T* __restrict__ invd, T* __restrict__ z) {
   __shared__ T part[kThreads / 2];
   const int64_t nb = (c1 - c0 + 1) / 2;  // blocks of two chunks, XCD-contiguous ranges
@@ -736,11 +864,11 @@ __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restri
   const bool act = ok && lane < k.nrows;
   T s = T(0);
   if (act) {
-    const int32_t* uc = ucol + k.uoff + lane;
-    const T* uv = uval + k.uoff + lane;
 #pragma unroll 4
-    for (int q = half; q < k.uw; q += 2)
-      s += __builtin_nontemporal_load(&uv[q * 64]) * z[__builtin_nontemporal_load(&uc[q * 64])];
+    for (int q = half; q < k.uw; q += 2) {
+      const int64_t at = k.uoff + (int64_t)q * 64 + lane;
+      s += __builtin_nontemporal_load(&uval[at]) * z[ucs.at((k.uoff >> 6) + q, at)];
+    }
   }
   if (half == 1) part[slot] = s;
   __syncthreads();
@@ -750,13 +878,95 @@ __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restri
   z[row] = z[row] - s * invd[row];
 }
 
+// q = A p (SELL copy); per-block partials of p.q
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_sell_spmv_dot(const SellChunk* __restrict__ ch, int64_t nch,
+                                                            const SellCols lcs, const T* __restrict__ lval,
+                                                            const SellCols ucs, const T* __restrict__ uval,
+                                                            const T* __restrict__ dv, const T* __restrict__ x,
+                                                            T* __restrict__ y, double* __restrict__ part,
+                                                            const CgState* __restrict__ st) {
+  if (st->done) return;
+  const int64_t c = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double d = 0.0;
+  if (c < nch) {
+    const SellChunk k = ch[c];
+    const bool act = lane < k.nrows;
+    const int row = k.row0 + (act ? lane : 0);
+    T s = dv[row] * x[row];
+#pragma unroll 4
+    for (int q = 0; q < k.lw; ++q) {
+      const int64_t at = k.loff + (int64_t)q * 64 + lane;
+      s += __builtin_nontemporal_load(&lval[at]) * x[lcs.at((k.loff >> 6) + q, at)];
+    }
+#pragma unroll 4
+    for (int q = 0; q < k.uw; ++q) {
+      const int64_t at = k.uoff + (int64_t)q * 64 + lane;
+      s += __builtin_nontemporal_load(&uval[at]) * x[ucs.at((k.uoff >> 6) + q, at)];
+    }
+    if (act) {
+      y[row] = s;
+      d = (double)s * (double)x[row];
+    }
+  }
+  d = block_sum(d);
+  if (threadIdx.x == 0) part[blockIdx.x] = d;
+}
+
+// Sweeps of the fused CG: as k_sell_fwd / k_sell_bwd, plus the partial sums
+// r.z of the rows they finalize (the forward sweep of the last color, every
+// backward sweep) in part[off + block].  All return at once after convergence.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(kThreads) void k_sell_sweep_dot(const SellChunk* __restrict__ ch, int64_t c0,
+                                                             int64_t c1, const SellCols cs, const T* __restrict__ val,
+                                                             const T* __restrict__ invd, const T* __restrict__ r,
+                                                             T* __restrict__ z, int dot, double* __restrict__ part,
+                                                             int off, const CgState* __restrict__ st) {
+  __shared__ T hpart[kThreads / 2];
+  if (st->done) return;
+  const int64_t nb = (c1 - c0 + 1) / 2;
+  const int64_t pos = xcd_block(blockIdx.x, nb);
+  const int64_t c = pos < 0 ? c1 : c0 + pos * (kThreads / 128) + (threadIdx.x >> 7);
+  const int lane = threadIdx.x & 63;
+  const int half = (threadIdx.x >> 6) & 1;
+  const int hslot = (threadIdx.x >> 7) * 64 + lane;
+  const bool ok = c < c1;
+  SellChunk k{};
+  if (ok) k = ch[c];
+  const bool act = ok && lane < k.nrows;
+  T s = T(0);
+  if (act) {
+    const int w = UPPER ? k.uw : k.lw;
+    const int64_t o = UPPER ? k.uoff : k.loff;
+#pragma unroll 4
+    for (int q = half; q < w; q += 2) {
+      const int64_t at = o + (int64_t)q * 64 + lane;
+      s += __builtin_nontemporal_load(&val[at]) * z[cs.at((o >> 6) + q, at)];
+    }
+  }
+  if (half == 1) hpart[hslot] = s;
+  __syncthreads();
+  double d = 0.0;
+  if (act && half == 0) {
+    const int row = k.row0 + lane;
+    s += hpart[hslot];
+    const T zr = UPPER ? z[row] - s * invd[row] : (r[row] - s) * invd[row];
+    z[row] = zr;
+    if (dot) d = (double)r[row] * (double)zr;
+  }
+  if (!dot) return;  // grid-uniform
+  d = block_sum(d);
+  if (threadIdx.x == 0) part[off + blockIdx.x] = d;
+}
+
 // Batched forms for NR right-hand sides interleaved [n][NR] (one matrix read
 // and one contiguous gather of NR values per entry): SpMV and the two sweeps,
 // with the matrix layouts of the kernels above.
 template <typename T, typename TY, int NR>
 __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __restrict__ ch, int64_t nch,
-                                                          const int32_t* __restrict__ lcol, const T* __restrict__ lval,
-                                                          const int32_t* __restrict__ ucol, const T* __restrict__ uval,
+                                                          const SellCols lcs, const T* __restrict__ lval,
+                                                          const SellCols ucs, const T* __restrict__ uval,
                                                           const T* __restrict__ dv, const T* __restrict__ x,
                                                           TY* __restrict__ y, const TY* __restrict__ b, int mode,
                                                           int64_t n) {
@@ -770,21 +980,19 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __res
   const T d = dv[row];
 #pragma unroll
   for (int r = 0; r < NR; ++r) s[r] = d * x[row * NR + r];
-  const int32_t* lc = lcol + k.loff + lane;
-  const T* lv = lval + k.loff + lane;
 #pragma unroll 2
   for (int q = 0; q < k.lw; ++q) {
-    const T a = __builtin_nontemporal_load(&lv[q * 64]);
-    const int64_t j = __builtin_nontemporal_load(&lc[q * 64]);
+    const int64_t at = k.loff + (int64_t)q * 64 + lane;
+    const T a = __builtin_nontemporal_load(&lval[at]);
+    const int64_t j = lcs.at((k.loff >> 6) + q, at);
 #pragma unroll
     for (int r = 0; r < NR; ++r) s[r] += a * x[j * NR + r];
   }
-  const int32_t* uc = ucol + k.uoff + lane;
-  const T* uv = uval + k.uoff + lane;
 #pragma unroll 2
   for (int q = 0; q < k.uw; ++q) {
-    const T a = __builtin_nontemporal_load(&uv[q * 64]);
-    const int64_t j = __builtin_nontemporal_load(&uc[q * 64]);
+    const int64_t at = k.uoff + (int64_t)q * 64 + lane;
+    const T a = __builtin_nontemporal_load(&uval[at]);
+    const int64_t j = ucs.at((k.uoff >> 6) + q, at);
 #pragma unroll
     for (int r = 0; r < NR; ++r) s[r] += a * x[j * NR + r];
   }
@@ -796,7 +1004,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __res
 // UPPER = false: forward sweep z = (r - L z) / d; true: backward z -= (U z) / d
 template <typename T, int NR, bool UPPER>
 __global__ __launch_bounds__(kThreads) void k_sell_sweep_m(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
-                                                           const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                           const SellCols cs, const T* __restrict__ val,
                                                            const T* __restrict__ invd, const T* __restrict__ rr,
                                                            T* __restrict__ z, int64_t n) {
   __shared__ T part[NR][kThreads / 2];
@@ -815,12 +1023,12 @@ __global__ __launch_bounds__(kThreads) void k_sell_sweep_m(const SellChunk* __re
   for (int r = 0; r < NR; ++r) s[r] = T(0);
   if (act) {
     const int w = UPPER ? k.uw : k.lw;
-    const int32_t* cc = col + (UPPER ? k.uoff : k.loff) + lane;
-    const T* vv = val + (UPPER ? k.uoff : k.loff) + lane;
+    const int64_t off = UPPER ? k.uoff : k.loff;
 #pragma unroll 2
     for (int q = half; q < w; q += 2) {
-      const T a = __builtin_nontemporal_load(&vv[q * 64]);
-      const int64_t j = __builtin_nontemporal_load(&cc[q * 64]);
+      const int64_t at = off + (int64_t)q * 64 + lane;
+      const T a = __builtin_nontemporal_load(&val[at]);
+      const int64_t j = cs.at((off >> 6) + q, at);
 #pragma unroll
       for (int r = 0; r < NR; ++r) s[r] += a * z[j * NR + r];
     }
@@ -972,6 +1180,17 @@ void build_sell(nalu_linsys* ls) {
                      ls->sl_col.p, ls->sl_src.p);
   hipLaunchKernelGGL(k_sell_fill, dim3(nbw), dim3(kThreads), 0, s, dch, nch, ls->urp.p, ls->uci.p, ls->usrc.p, 1,
                      ls->su_col.p, ls->su_src.p);
+  // compressed columns (every chunk offset is a multiple of 64: one slot = 64 entries)
+  ls->sl_off.alloc(std::max<int64_t>(1, lo));
+  ls->su_off.alloc(std::max<int64_t>(1, uo));
+  ls->sl_base.alloc(std::max<int64_t>(1, lo / 64));
+  ls->su_base.alloc(std::max<int64_t>(1, uo / 64));
+  if (lo > 0)
+    hipLaunchKernelGGL(k_sell_compress, dim3((unsigned)((lo / 64 + 3) / 4)), dim3(kThreads), 0, s, ls->sl_col.p,
+                       ls->sl_src.p, lo / 64, ls->sl_off.p, ls->sl_base.p);
+  if (uo > 0)
+    hipLaunchKernelGGL(k_sell_compress, dim3((unsigned)((uo / 64 + 3) / 4)), dim3(kThreads), 0, s, ls->su_col.p,
+                       ls->su_src.p, uo / 64, ls->su_off.p, ls->su_base.p);
   NALU_HIP_CHECK(hipGetLastError());
   ls->sell_ready = true;
 }
@@ -990,6 +1209,10 @@ Split<T> split_of(nalu_linsys* ls) {
     return {ls->lrp.p, ls->lci.p, ls->urp.p, ls->uci.p, ls->lval32.p, ls->uval32.p, ls->dval32.p, ls->pinvd32.p};
 }
 
+SellCols sell_cols(const nalu_linsys* ls, bool upper) {
+  return upper ? SellCols{ls->su_off.p, ls->su_base.p, ls->su_col.p} : SellCols{ls->sl_off.p, ls->sl_base.p, ls->sl_col.p};
+}
+
 template <typename T>
 const T* sell_vals(const nalu_linsys* ls, bool upper) {
   if constexpr (std::is_same<T, double>::value) return upper ? ls->su_val.p : ls->sl_val.p;
@@ -1003,8 +1226,8 @@ void pspmv(nalu_linsys* ls, const T* x, TY* y, const TY* b, int mode) {
   if (kc == 64) {
     const int64_t nch = ls->sell_nchunks;
     hipLaunchKernelGGL((k_sell_spmv<T, TY>), dim3((unsigned)std::max<int64_t>(1, (nch + 3) / 4)), dim3(kThreads), 0,
-                       ls->stream, reinterpret_cast<const SellChunk*>(ls->sell_chunks.p), nch, ls->sl_col.p,
-                       sell_vals<T>(ls, false), ls->su_col.p, sell_vals<T>(ls, true), A.dv, x, y, b, mode);
+                       ls->stream, reinterpret_cast<const SellChunk*>(ls->sell_chunks.p), nch, sell_cols(ls, false),
+                       sell_vals<T>(ls, false), sell_cols(ls, true), sell_vals<T>(ls, true), A.dv, x, y, b, mode);
     return;
   }
   if (kc == 8 || kc == 4) {
@@ -1040,13 +1263,13 @@ void psgs(nalu_linsys* ls, const T* r, T* z) {
       const int64_t c0 = ccp[c], c1 = ccp[c + 1];
       if (c1 > c0)
         hipLaunchKernelGGL(k_sell_fwd<T>, dim3((unsigned)xcd_grid((c1 - c0 + 1) / 2)), dim3(kThreads), 0, s, ch,
-                           c0, c1, ls->sl_col.p, sell_vals<T>(ls, false), A.invd, r, z);
+                           c0, c1, sell_cols(ls, false), sell_vals<T>(ls, false), A.invd, r, z);
     }
     for (int c = ls->num_colors - 2; c >= 0; --c) {  // the last color has no U entries
       const int64_t c0 = ccp[c], c1 = ccp[c + 1];
       if (c1 > c0)
         hipLaunchKernelGGL(k_sell_bwd<T>, dim3((unsigned)xcd_grid((c1 - c0 + 1) / 2)), dim3(kThreads), 0, s, ch,
-                           c0, c1, ls->su_col.p, sell_vals<T>(ls, true), A.invd, z);
+                           c0, c1, sell_cols(ls, true), sell_vals<T>(ls, true), A.invd, z);
     }
     return;
   }
@@ -1094,8 +1317,8 @@ void spmv_nr(nalu_linsys* ls, const T* x, TY* y, const TY* b, int mode) {
   const Split<T> A = split_of<T>(ls);
   const int64_t nch = ls->sell_nchunks;
   hipLaunchKernelGGL((k_sell_spmv_m<T, TY, NR>), dim3((unsigned)std::max<int64_t>(1, (nch + 3) / 4)), dim3(kThreads),
-                     0, ls->stream, reinterpret_cast<const SellChunk*>(ls->sell_chunks.p), nch, ls->sl_col.p,
-                     sell_vals<T>(ls, false), ls->su_col.p, sell_vals<T>(ls, true), A.dv, x, y, b, mode,
+                     0, ls->stream, reinterpret_cast<const SellChunk*>(ls->sell_chunks.p), nch, sell_cols(ls, false),
+                     sell_vals<T>(ls, false), sell_cols(ls, true), sell_vals<T>(ls, true), A.dv, x, y, b, mode,
                      (int64_t)ls->nrows);
 }
 
@@ -1128,13 +1351,13 @@ void sgs_nr(nalu_linsys* ls, const T* r, T* z) {
     const int64_t c0 = ccp[c], c1 = ccp[c + 1];
     if (c1 > c0)
       hipLaunchKernelGGL((k_sell_sweep_m<T, NR, false>), dim3((unsigned)xcd_grid((c1 - c0 + 1) / 2)), dim3(kThreads),
-                         0, s, ch, c0, c1, ls->sl_col.p, sell_vals<T>(ls, false), A.invd, r, z, n);
+                         0, s, ch, c0, c1, sell_cols(ls, false), sell_vals<T>(ls, false), A.invd, r, z, n);
   }
   for (int c = ls->num_colors - 2; c >= 0; --c) {
     const int64_t c0 = ccp[c], c1 = ccp[c + 1];
     if (c1 > c0)
       hipLaunchKernelGGL((k_sell_sweep_m<T, NR, true>), dim3((unsigned)xcd_grid((c1 - c0 + 1) / 2)), dim3(kThreads),
-                         0, s, ch, c0, c1, ls->su_col.p, sell_vals<T>(ls, true), A.invd, (const T*)nullptr, z, n);
+                         0, s, ch, c0, c1, sell_cols(ls, true), sell_vals<T>(ls, true), A.invd, (const T*)nullptr, z, n);
   }
 }
 
@@ -1373,7 +1596,102 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
 
 // Preconditioned CG in the permuted space; scalars stay on the device, one
 // host read per iteration for the convergence test.
+// CG with the SGS preconditioner on the SELL copy for one right-hand side (b
+// and x in the color-permuted row order, as every solver here takes them),
+// every dot product fused into a producing kernel and the convergence test
+// read from the device every kCheck iterations (see CgState).
+constexpr int kCgCheck = 8;
+
+int cg_fused(nalu_linsys* ls, const double* b, double* x, int maxit, double tol) {
+  hipStream_t s = ls->stream;
+  const int64_t n = ls->nrows;
+  const int64_t nch = ls->sell_nchunks;
+  const int nc = ls->num_colors;
+  const auto& ccp = ls->sell_color_ptr;
+  const SellChunk* ch = reinterpret_cast<const SellChunk*>(ls->sell_chunks.p);
+  const unsigned g_spmv = (unsigned)std::max<int64_t>(1, (nch + 3) / 4);
+  const unsigned g_upd = red_blocks(n);
+  // partial-sum slots of the sweeps that finalize z: forward of the last color, every backward
+  std::vector<int> off(nc + 1, 0);
+  int total = 0;
+  auto sweep_grid = [&](int c) { return (unsigned)xcd_grid(std::max<int64_t>(1, (ccp[c + 1] - ccp[c] + 1) / 2)); };
+  const int cl = nc - 1;
+  off[cl] = total;
+  total += (int)sweep_grid(cl);
+  for (int c = nc - 2; c >= 0; --c) {
+    off[c] = total;
+    total += (int)sweep_grid(c);
+  }
+  const size_t nparts = (size_t)std::max<int64_t>({(int64_t)g_spmv, (int64_t)g_upd, (int64_t)total});
+  const size_t need = 256 + (4 * (size_t)n + nparts) * sizeof(double);
+  if (ls->cg_work.n < need) ls->cg_work.alloc(need);
+  unsigned char* wb = ls->cg_work.p;
+  CgState* st = reinterpret_cast<CgState*>(wb);
+  double* r = reinterpret_cast<double*>(wb + 256);
+  double* z = r + n;
+  double* p = z + n;
+  double* q = p + n;
+  double* part = q + n;
+  ls->cg_host.alloc(sizeof(CgState));
+  CgState* hs = static_cast<CgState*>(ls->cg_host.p);
+  Red red(ls);
+  const SellCols lcs = sell_cols(ls, false), ucs = sell_cols(ls, true);
+  const double* lval = sell_vals<double>(ls, false);
+  const double* uval = sell_vals<double>(ls, true);
+  const double* invd = split_of<double>(ls).invd;
+
+  pspmv<double, double>(ls, x, r, b, 1);
+  const double r0 = std::sqrt(red.dot(r, r, n));
+  if (r0 == 0.0) return 0;
+  NALU_REQUIRE(std::isfinite(r0), NALU_ERR_NUMERIC, "cg: non-finite residual");
+  *hs = CgState{};
+  hs->r0 = r0;
+  hs->tol2r02 = (tol * r0) * (tol * r0);
+  NALU_HIP_CHECK(hipMemcpyAsync(st, hs, sizeof(CgState), hipMemcpyHostToDevice, s));
+  psgs<double>(ls, r, z);
+  red.dot_dev(r, z, n, &st->rz[0]);
+  hipLaunchKernelGGL((k_copy<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, z, p, n);
+  int cur = 0, it = 0;
+  bool stop = false;
+  while (!stop && it < maxit) {
+    ++it;
+    hipLaunchKernelGGL(k_sell_spmv_dot<double>, dim3(g_spmv), dim3(kThreads), 0, s, ch, nch, lcs, lval, ucs, uval,
+                       split_of<double>(ls).dv, (const double*)p, q, part, (const CgState*)st);
+    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kThreads), 0, s, (const double*)part, (int)g_spmv, 0, 0, it, st);
+    hipLaunchKernelGGL(k_cg_update_f, dim3(g_upd), dim3(kThreads), 0, s, x, r, (const double*)p,
+                       (const double*)q, n, cur, part, (const CgState*)st);
+    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kThreads), 0, s, (const double*)part, (int)g_upd, 1, 0, it, st);
+    const int nxt = 1 - cur;
+    for (int c = 0; c < nc; ++c)
+      hipLaunchKernelGGL((k_sell_sweep_dot<double, false>), dim3(sweep_grid(c)), dim3(kThreads), 0, s, ch, ccp[c],
+                         ccp[c + 1], lcs, lval, invd, (const double*)r, z, c == cl ? 1 : 0, part, off[c],
+                         (const CgState*)st);
+    for (int c = nc - 2; c >= 0; --c)
+      hipLaunchKernelGGL((k_sell_sweep_dot<double, true>), dim3(sweep_grid(c)), dim3(kThreads), 0, s, ch, ccp[c],
+                         ccp[c + 1], ucs, uval, invd, (const double*)r, z, 1, part, off[c], (const CgState*)st);
+    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kThreads), 0, s, (const double*)part, total, 2, nxt, it, st);
+    hipLaunchKernelGGL(k_cg_p_f, dim3(nblocks(n)), dim3(kThreads), 0, s, p, (const double*)z, n, nxt, cur,
+                       (const CgState*)st);
+    cur = nxt;
+    if (it % kCgCheck == 0 || it == maxit) {
+      NALU_HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+      NALU_HIP_CHECK(hipStreamSynchronize(s));
+      stop = hs->done != 0;
+    }
+  }
+  NALU_HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+  NALU_HIP_CHECK(hipStreamSynchronize(s));
+  NALU_REQUIRE(hs->done != 2, NALU_ERR_NUMERIC, "cg: non-finite residual");
+  return hs->done ? hs->iters : it;
+}
+
+bool cg_fusable(const nalu_linsys* ls, int nrhs) {
+  return nrhs == 1 && sparse_kernel_choice() == 64 && ls->cfg.precond == NALU_PREC_SGS && ls->num_colors >= 1 &&
+         ls->sell_ready && ls->perm_ready && std::getenv("NALU_CG_UNFUSED") == nullptr;
+}
+
 int cg(nalu_linsys* ls, const double* b, double* x, int maxit, double tol, int nrhs = 1) {
+  if (cg_fusable(ls, nrhs)) return cg_fused(ls, b, x, maxit, tol);
   hipStream_t s = ls->stream;
   const int64_t n = ls->nrows * nrhs;  // nrhs > 1: the block-diagonal stacked system
   Red red(ls);

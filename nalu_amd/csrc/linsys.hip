@@ -116,6 +116,11 @@ extern "C" int nalu_linsys_create(int num_dof, const nalu_solver_config* cfg, na
                  "linsys_create: unknown preconditioner");
     NALU_REQUIRE(cfg->precision == NALU_PRECISION_FP64 || cfg->precision == NALU_PRECISION_MIXED, NALU_ERR_ARG,
                  "linsys_create: unknown precision");
+    NALU_REQUIRE(cfg->kspace >= 1 && cfg->kspace <= 1000, NALU_ERR_ARG, "linsys_create: kspace must be in [1, 1000]");
+    NALU_REQUIRE(cfg->max_iterations >= 1, NALU_ERR_ARG, "linsys_create: max_iterations must be at least 1");
+    NALU_REQUIRE(cfg->tolerance >= 0.0 && cfg->final_tolerance >= 0.0 && std::isfinite(cfg->tolerance) &&
+                     std::isfinite(cfg->final_tolerance),
+                 NALU_ERR_ARG, "linsys_create: tolerances must be finite and non-negative");
     // the network is a nonlinear operator: only flexible GMRES admits a
     // preconditioner that changes from one application to the next
     NALU_REQUIRE(cfg->precond != NALU_PREC_LEARNED || cfg->method == NALU_METHOD_FGMRES, NALU_ERR_ARG,

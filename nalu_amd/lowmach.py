@@ -15,9 +15,10 @@ three-right-hand-side form:
                 u -= (dt / g1) / rho (Gjp_new - Gjp_old) on the non-wall nodes
                 (LowMachEquationSystem::project_nodal_velocity, :884-975)
 
-The nodal gradient has no exposed-face closure here, which the projection
-tolerates on the interior nodes (wall nodes keep their Dirichlet values).
-Meshes are used without explicit node ids (row = node).
+The projected nodal gradient includes the exposed-face closure
+(AssembleNodalGradBoundaryAlgorithm), so Gjp is consistent at wall nodes too;
+wall nodes keep their Dirichlet velocities.  Meshes are used without explicit
+node ids (row = node).
 """
 import numpy as np
 

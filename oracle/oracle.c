@@ -734,6 +734,106 @@ void or_nodal_grad_elem(int64_t nelem, const int32_t *conn, const double *coords
   }
 }
 
+/* HexSCS sideNodeOrdinals_ (src/master_element/Hex8CVFEM.C:415-422): the
+ * nodes of each side, counter-clockwise seen from outside. */
+static const int kHexSide[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6},
+                                   {0, 4, 7, 3}, {0, 3, 2, 1}, {4, 5, 6, 7}};
+
+/* Exposed sides of a Hex8 block: element sides that no other element shares,
+ * nodes in side order.  Returns their number; faces [count][4] may be NULL to
+ * count only.  (The reference receives them as the exposed side parts of the
+ * mesh; this builds the same set from the connectivity.) */
+int64_t or_exposed_faces(int64_t nelem, const int32_t *conn, int64_t nnodes, int32_t *faces) {
+  int64_t *ip = (int64_t *)calloc((size_t)nnodes + 1, sizeof(int64_t));
+  int32_t *ie = (int32_t *)malloc(sizeof(int32_t) * (size_t)(8 * nelem));
+  for (int64_t q = 0; q < 8 * nelem; ++q) ip[conn[q] + 1]++;
+  for (int64_t a = 0; a < nnodes; ++a) ip[a + 1] += ip[a];
+  int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)nnodes);
+  memcpy(fill, ip, sizeof(int64_t) * (size_t)nnodes);
+  for (int64_t q = 0; q < 8 * nelem; ++q) ie[fill[conn[q]]++] = (int32_t)(q / 8);
+  free(fill);
+  int64_t count = 0;
+  for (int64_t e = 0; e < nelem; ++e)
+    for (int f = 0; f < 6; ++f) {
+      int32_t fn[4];
+      for (int k = 0; k < 4; ++k) fn[k] = conn[8 * e + kHexSide[f][k]];
+      int shared = 0;
+      for (int64_t q = ip[fn[0]]; q < ip[fn[0] + 1] && !shared; ++q) {
+        const int64_t o = ie[q];
+        if (o == e) continue;
+        int hits = 0;
+        for (int k = 1; k < 4; ++k)
+          for (int n = 0; n < 8; ++n)
+            if (conn[8 * o + n] == fn[k]) {
+              ++hits;
+              break;
+            }
+        shared = hits == 3;
+      }
+      if (!shared) {
+        if (faces)
+          for (int k = 0; k < 4; ++k) faces[4 * count + k] = fn[k];
+        ++count;
+      }
+    }
+  free(ip);
+  free(ie);
+  return count;
+}
+
+/* Quad3DSCS::determinant (quad3d_scs_det, src/master_element/MasterElementWork.F:1467-1584):
+ * sub-face i of the quad p[4] spans node i, the midpoints of its two edges and
+ * the centroid: A_i = 1/2 (c - p_i) x (e_{i-1} - e_i), e_i = (p_i + p_{i+1}) / 2. */
+void or_quad3d_scs_areav(const double *p, double *areav) {
+  double e[4][3], c[3];
+  for (int d = 0; d < 3; ++d) {
+    for (int i = 0; i < 4; ++i) e[i][d] = (p[3 * i + d] + p[3 * ((i + 1) % 4) + d]) * 0.5;
+    c[d] = (p[d] + p[3 + d] + p[6 + d] + p[9 + d]) * 0.25;
+  }
+  for (int i = 0; i < 4; ++i) {
+    const int im = (i + 3) % 4;
+    const double dx13 = c[0] - p[3 * i], dy13 = c[1] - p[3 * i + 1], dz13 = c[2] - p[3 * i + 2];
+    const double dx24 = e[im][0] - e[i][0], dy24 = e[im][1] - e[i][1], dz24 = e[im][2] - e[i][2];
+    areav[3 * i + 0] = 0.5 * (dz24 * dy13 - dz13 * dy24);
+    areav[3 * i + 1] = 0.5 * (dx24 * dz13 - dx13 * dz24);
+    areav[3 * i + 2] = 0.5 * (dy24 * dx13 - dy13 * dx24);
+  }
+}
+
+/* AssembleNodalGradBoundaryAlgorithm::execute (src/AssembleNodalGradBoundaryAlgorithm.C:54-152):
+ * for every exposed face and its ip i (nearest node i, Quad3DSCS ipNodeMap_),
+ * dqdx[node i] += q_ip A_i / dual_vol[node i], q_ip from the quad shape
+ * functions (quad3d_shape_fcn, MasterElementWork.F:3651-3695) at (+-1/4, +-1/4),
+ * or at the nodes themselves when shifted (+-1/2).  Adds into dqdx. */
+void or_nodal_grad_boundary(int64_t nfaces, const int32_t *faces, const double *coords, const double *q,
+                            const double *dual_vol, int shifted, double *dqdx) {
+  const double h = shifted ? 0.5 : 0.25;
+  const double loc[4][2] = {{-h, -h}, {h, -h}, {h, h}, {-h, h}};
+  double shape[4][4];
+  for (int j = 0; j < 4; ++j) {
+    const double s1 = loc[j][0], s2 = loc[j][1];
+    shape[j][0] = 0.25 + 0.5 * (-s1 - s2) + s1 * s2;
+    shape[j][1] = 0.25 + 0.5 * (s1 - s2) - s1 * s2;
+    shape[j][2] = 0.25 + 0.5 * (s1 + s2) + s1 * s2;
+    shape[j][3] = 0.25 + 0.5 * (-s1 + s2) - s1 * s2;
+  }
+  for (int64_t f = 0; f < nfaces; ++f) {
+    const int32_t *fn = faces + 4 * f;
+    double x[12], areav[12], qf[4];
+    for (int k = 0; k < 4; ++k) {
+      for (int d = 0; d < 3; ++d) x[3 * k + d] = coords[3 * (size_t)fn[k] + d];
+      qf[k] = q[fn[k]];
+    }
+    or_quad3d_scs_areav(x, areav);
+    for (int ip = 0; ip < 4; ++ip) {
+      double qIp = 0.0;
+      for (int c = 0; c < 4; ++c) qIp += shape[ip][c] * qf[c];
+      const double inv = 1.0 / dual_vol[fn[ip]];
+      for (int j = 0; j < 3; ++j) dqdx[3 * (size_t)fn[ip] + j] += qIp * areav[3 * ip + j] * inv;
+    }
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Time-derivative (mass) kernels                                           */
 /* ------------------------------------------------------------------------ */

@@ -101,6 +101,15 @@ void or_dual_nodal_volume(int64_t nelem, const int32_t *conn, const double *coor
                           double *dual_vol);
 void or_nodal_grad_elem(int64_t nelem, const int32_t *conn, const double *coords, const double *q,
                         const double *dual_vol, int shifted, int64_t nnodes, double *dqdx);
+/* exposed sides of the block (nodes in HexSCS side order; faces may be NULL to
+ * count), the Quad3DSCS sub-face area vectors, and the boundary closure of the
+ * nodal gradient (AssembleNodalGradBoundaryAlgorithm.C:54-152, adds into dqdx);
+ * pinned together with the interior part by exact gradients of linear fields at
+ * every node */
+int64_t or_exposed_faces(int64_t nelem, const int32_t *conn, int64_t nnodes, int32_t *faces);
+void or_quad3d_scs_areav(const double *coords, double *areav);
+void or_nodal_grad_boundary(int64_t nfaces, const int32_t *faces, const double *coords, const double *q,
+                            const double *dual_vol, int shifted, double *dqdx);
 /* mass kernels (MomentumMassElemKernel.C:95-170, ContinuityMassElemKernel.C:84-118);
  * pinned by the reference's unit-test golds (consistent and lumped) */
 void or_momentum_mass_elem(const double *coords, const double *rho3, const double *u3,

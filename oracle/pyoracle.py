@@ -25,6 +25,7 @@ def lib():
             raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
         L = C.CDLL(path)
         L.or_graph_build.restype = C.c_int64
+        L.or_exposed_faces.restype = C.c_int64
         L.or_gmres.restype = C.c_int
         L.or_cg.restype = C.c_int
         L.or_hex_volume_grandy.restype = C.c_double
@@ -52,6 +53,9 @@ def lib():
             "or_assemble_momentum": [I64, _i32, _d, _d, _d, _d, D, I, I, _i64, _i32, _i32, _d, _d],
             "or_dual_nodal_volume": [I64, _i32, _d, I64, _d],
             "or_nodal_grad_elem": [I64, _i32, _d, _d, _d, I, I64, _d],
+            "or_exposed_faces": [I64, _i32, I64, _i32],
+            "or_quad3d_scs_areav": [_d, _d],
+            "or_nodal_grad_boundary": [I64, _i32, _d, _d, _d, I, _d],
             "or_momentum_mass_elem": [_d, _d, _d, _d, D, _d, I, I, _d, _d],
             "or_continuity_mass_elem": [_d, _d, D, _d, I, _d],
             "or_assemble_momentum_mass": [I64, _i32, _d, _d, _d, _d, _d, _d, _d, _d, D, _d, I, I, _i64, _i32,
@@ -305,6 +309,33 @@ def nodal_grad_elem(conn, coords, q, dual_vol, shifted=False):
     lib().or_nodal_grad_elem(conn.shape[0], _p(conn, np.int32), _p(coords), _p(f64(q)), _p(f64(dual_vol)),
                              int(shifted), coords.shape[0], _p(out))
     return out
+
+
+def exposed_faces(conn, nnodes):
+    """[nfaces][4] node indices of the block's exposed sides (HexSCS side order)."""
+    conn = i32(conn)
+    n = lib().or_exposed_faces(conn.shape[0], _p(conn, np.int32), int(nnodes), None)
+    faces = np.zeros((n, 4), dtype=np.int32)
+    lib().or_exposed_faces(conn.shape[0], _p(conn, np.int32), int(nnodes), _p(faces, np.int32))
+    return faces
+
+
+def quad3d_scs_areav(coords):
+    out = np.zeros((4, 3))
+    lib().or_quad3d_scs_areav(_p(f64(coords)), _p(out))
+    return out
+
+
+def nodal_grad(conn, coords, q, dual_vol, shifted=False, boundary=True):
+    """Projected nodal gradient: interior (AssembleNodalGradElemAlgorithm) plus, by
+    default, the exposed-face closure (AssembleNodalGradBoundaryAlgorithm)."""
+    nn = coords.shape[0]
+    g = nodal_grad_elem(conn, coords, q, dual_vol, shifted)
+    if boundary:
+        faces = exposed_faces(conn, nn)
+        lib().or_nodal_grad_boundary(faces.shape[0], _p(faces, np.int32), _p(f64(coords)), _p(f64(q)),
+                                     _p(f64(dual_vol)), int(shifted), _p(g))
+    return g
 
 
 def momentum_segregated(g1, g3, conn, coords, u, mu, mdot, include_divu=0.0, skew=False, shifted=False):

@@ -249,9 +249,10 @@ def test_stacked_solve(oracle, precond, precision, k):
 
 @pytest.mark.parametrize("shifted", [False, True])
 def test_dual_volume_and_nodal_gradient_parity(oracle, shifted):
-    """Dual nodal volume and interior Green-Gauss nodal gradient against the
-    oracle on a jittered lattice (1e-12 of the largest entry); the gradient of the
-    pressure then feeds continuity assembly as Gpdx."""
+    """Dual nodal volume and the projected nodal gradient (interior Green-Gauss
+    part, then with the exposed-face closure) against the oracle on a jittered
+    lattice (1e-12 of the largest entry), the exposed sides identical to the
+    oracle's; the gradient of the pressure then feeds continuity assembly as Gpdx."""
     n = 8
     print("case: dual volume + nodal gradient n=%d shifted=%s" % (n, shifted), flush=True)
     P = lowmach_problem(n, seed=21, rho_sigma=0.3)
@@ -263,12 +264,17 @@ def test_dual_volume_and_nodal_gradient_parity(oracle, shifted):
     oV = oracle.dual_nodal_volume(conn, X)
     np.testing.assert_allclose(V.cpu().numpy(), oV, rtol=0, atol=1e-12 * oV.max())
     G = torch.zeros((N, 3), dtype=torch.float64, device=DEV)
+    # interior part alone, then the full projected gradient (with the exposed-face closure)
+    na.compute_nodal_gradient(mesh, dev(P["p"]), V, G, shifted=shifted, boundary=False)
+    oGi = oracle.nodal_grad_elem(conn, X, P["p"], oV, shifted=shifted)
+    err_i = np.abs(G.cpu().numpy() - oGi).max() / np.abs(oGi).max()
     na.compute_nodal_gradient(mesh, dev(P["p"]), V, G, shifted=shifted)
-    oG = oracle.nodal_grad_elem(conn, X, P["p"], oV, shifted=shifted)
+    oG = oracle.nodal_grad(conn, X, P["p"], oV, shifted=shifted)
     err = np.abs(G.cpu().numpy() - oG).max() / np.abs(oG).max()
-    print("  rel err: dual volume %.2e gradient %.2e" % (np.abs(V.cpu().numpy() - oV).max() / oV.max(), err),
-          flush=True)
-    assert err <= TOL
+    np.testing.assert_array_equal(mesh.exposed_faces(), oracle.exposed_faces(conn, N))
+    print("  rel err: dual volume %.2e gradient interior %.2e with closure %.2e"
+          % (np.abs(V.cpu().numpy() - oV).max() / oV.max(), err_i, err), flush=True)
+    assert err_i <= TOL and err <= TOL
     # the produced gradient drives the continuity assembly like the analytic one
     ls = na.LinearSystem(1, na.SolverConfig())
     ls.buildElemToNodeGraph(mesh)
@@ -280,3 +286,20 @@ def test_dual_volume_and_nodal_gradient_parity(oracle, shifted):
     ov, orr = oracle.assemble_continuity(g, conn, X, P["rho"], P["p"], P["u"], oG, P["tau"])
     assert rel_err_rowwise(v, ov, g.row_ptr) <= TOL
     assert np.abs(r - orr).max() <= TOL * np.abs(orr).max()
+
+
+def test_nodal_gradient_exact_on_affine_mesh_all_nodes():
+    """The projected gradient with the closure reproduces the gradient of a linear
+    field at every node, boundary nodes included, on a sheared (affine) lattice."""
+    print("case: nodal gradient of a linear field, sheared lattice n=6", flush=True)
+    n = 6
+    P = lowmach_problem(n, seed=3, jitter=0.0)
+    X = P["coords"] @ np.array([[1.0, 0.2, 0.0], [0.1, 0.9, 0.3], [0.0, -0.1, 1.2]]).T
+    mesh = na.Mesh(X, P["conn"])
+    N = X.shape[0]
+    a = np.array([0.7, -1.3, 2.1])
+    V = torch.zeros(N, dtype=torch.float64, device=DEV)
+    na.compute_dual_volume(mesh, V)
+    G = torch.zeros((N, 3), dtype=torch.float64, device=DEV)
+    na.compute_nodal_gradient(mesh, dev(X @ a + 0.4), V, G)
+    np.testing.assert_allclose(G.cpu().numpy(), np.broadcast_to(a, (N, 3)), rtol=0, atol=1e-11)

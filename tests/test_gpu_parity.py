@@ -309,6 +309,47 @@ def test_pressure_poisson_cg(oracle, prec):
     assert st == 0 and res <= 1e-8 and abs(res - rel) <= 1e-12
 
 
+def test_fused_cg_matches_unfused(oracle, monkeypatch):
+    """The fused CG (dot products inside the SpMV, update and sweep kernels, the
+    convergence test read every 8 iterations) against the kernel-per-step CG on the
+    same system: the same iteration count within one (sums in another order), the
+    same true residual bound, and an iteration cap that is not a multiple of 8."""
+    n = 14
+    print("case: fused vs unfused CG+SGS, continuity n=%d" % n, flush=True)
+    P = lowmach_problem(n, seed=43)
+    mesh, ls = make_system(P["coords"], P["conn"], cfg=na.SolverConfig("cg", "sgs", tolerance=1e-8,
+                                                                       max_iterations=3000))
+    ls.zeroSystem()
+    na.assemble_continuity(ls, mesh, dev(P["rho"]), dev(P["p"]), dev(P["u"]), dev(P["gpdx"]), tau=1.0)
+    ls.loadComplete()
+    rp, ci, v, _ = ls.csr()
+    b = poisson_rhs(len(rp) - 1, 78)
+    out = {}
+    for fused in (True, False):
+        if fused:
+            monkeypatch.delenv("NALU_CG_UNFUSED", raising=False)
+        else:
+            monkeypatch.setenv("NALU_CG_UNFUSED", "1")
+        x = torch.zeros(len(b), dtype=torch.float64, device=DEV)
+        st, it, rel = ls.solve_rhs(dev(b), x)
+        xs = x.cpu().numpy()
+        res = np.linalg.norm(b - oracle.spmv(rp, ci, v, xs)) / np.linalg.norm(b)
+        print("  fused=%d status %d iters %d true resid %.3e" % (fused, st, it, res), flush=True)
+        assert st == 0 and res <= 1e-8 and abs(res - rel) <= 1e-12
+        out[fused] = (it, xs)
+    assert abs(out[True][0] - out[False][0]) <= 1
+    np.testing.assert_allclose(out[True][1], out[False][1], rtol=0, atol=1e-6 * np.abs(out[False][1]).max())
+    monkeypatch.delenv("NALU_CG_UNFUSED", raising=False)
+    mesh, ls = make_system(P["coords"], P["conn"], cfg=na.SolverConfig("cg", "sgs", tolerance=1e-14,
+                                                                       max_iterations=13))
+    ls.zeroSystem()
+    na.assemble_continuity(ls, mesh, dev(P["rho"]), dev(P["p"]), dev(P["u"]), dev(P["gpdx"]), tau=1.0)
+    ls.loadComplete()
+    x = torch.zeros(len(b), dtype=torch.float64, device=DEV)
+    st, it, _ = ls.solve_rhs(dev(b), x)
+    assert st == 1 and it == 13
+
+
 def test_nan_rhs_is_reported():
     print("case: NaN rhs", flush=True)
     P, mesh, ls = _heat_ls(4, na.SolverConfig("gmres", "sgs", max_iterations=50))

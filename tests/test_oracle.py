@@ -433,8 +433,8 @@ def test_dual_volume_closed_form(oracle):
 def test_nodal_gradient_exact_for_linear_fields(oracle, shifted):
     """On an affine lattice (parallelepiped elements) the interior Green-Gauss
     gradient of q = a.x + c is exactly a at every interior node, for the standard
-    and the shifted scs shape functions; boundary nodes lack the exposed-face
-    closure, so their sum differs."""
+    and the shifted scs shape functions; boundary nodes need the exposed-face
+    closure, without which their sum differs."""
     n = 5
     M = [[1.0, 0.3, 0.0], [0.1, 0.8, 0.2], [0.0, -0.2, 1.1]]
     X, conn = _uniform_lattice(n, M)
@@ -450,3 +450,31 @@ def test_nodal_gradient_exact_for_linear_fields(oracle, shifted):
     # a constant field has zero interior gradient
     g0 = oracle.nodal_grad_elem(conn, X, np.full(len(q), 3.0), V, shifted=shifted)
     assert np.abs(g0[interior]).max() <= 1e-12
+    # with the exposed-face closure the gradient is exact at every node, and a
+    # constant field has zero gradient everywhere (each dual volume is closed)
+    gb = oracle.nodal_grad(conn, X, q, V, shifted=shifted)
+    np.testing.assert_allclose(gb, np.broadcast_to(a, gb.shape), rtol=0, atol=1e-12)
+    gb0 = oracle.nodal_grad(conn, X, np.full(len(q), 3.0), V, shifted=shifted)
+    assert np.abs(gb0).max() <= 1e-12
+
+
+def test_exposed_faces_and_quad_subfaces(oracle):
+    """A lattice of n^3 elements exposes 6 n^2 sides, each listed with its outward
+    normal (right-hand rule); the four sub-face area vectors of a planar unit
+    square are a quarter of its area vector each, and on a skewed quad they sum
+    to the quad's shoelace area vector."""
+    n = 3
+    X, conn = _uniform_lattice(n, np.eye(3).tolist())
+    faces = oracle.exposed_faces(conn, X.shape[0])
+    assert faces.shape == (6 * n * n, 4)
+    c = X.mean(axis=0)
+    for f in faces:
+        p = X[f]
+        A = oracle.quad3d_scs_areav(p).sum(axis=0)
+        assert np.dot(A, p.mean(axis=0) - c) > 0  # outward
+        np.testing.assert_allclose(np.linalg.norm(A), (1.0 / n) ** 2, rtol=1e-13)
+    sq = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], dtype=float)
+    np.testing.assert_allclose(oracle.quad3d_scs_areav(sq), np.tile([0, 0, 0.25], (4, 1)), atol=1e-15)
+    sk = np.array([[0, 0, 0.1], [1.2, 0.1, 0], [1.0, 0.9, 0.3], [-0.1, 1.1, 0.0]])
+    shoelace = 0.5 * np.cross(sk[2] - sk[0], sk[3] - sk[1])
+    np.testing.assert_allclose(oracle.quad3d_scs_areav(sk).sum(axis=0), shoelace, atol=1e-14)
