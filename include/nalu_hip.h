@@ -141,6 +141,18 @@ int nalu_linsys_sum_into_elements(nalu_linsys *ls, int64_t num_elems, int num_en
                                   const int32_t *entities_dev, const double *rhs_dev,
                                   const double *lhs_dev, int32_t *local_ids_dev,
                                   int32_t *sort_permutation_dev);
+/* No counterpart in the reference: the sumInto above for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results without the
+ * caller scratch: the localIds / sortPermutation tables and the error flag live
+ * in a 32 MiB file-scope device array of the library, one per GPU, kept until
+ * the process ends.  At most (32 MiB - 256 B) / (8 R) elements per call
+ * (NALU_ERR_ARG beyond); NALU_ERR_STATE while host sumInto calls are queued
+ * (applying them would allocate).  Calls that overlap on one GPU share the
+ * array and race. */
+int nalu_linsys_sum_into_elements_noalloc(nalu_linsys *ls, int64_t num_elems, int num_entities,
+                                          const int32_t *entities_dev, const double *rhs_dev,
+                                          const double *lhs_dev);
 /* The same for the graph mesh's own elements, looked up through the element
  * scatter table built with the graph (no sort, no row walk): conn_dev must be
  * the mesh's connectivity (nalu_mesh_device_ptrs), lhs_dev [E][R][R], rhs_dev

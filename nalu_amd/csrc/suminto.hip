@@ -15,6 +15,7 @@
 //    when anything else touches the system, so a per-element caller costs a
 //    host memcpy per element instead of device round trips.
 #include <algorithm>
+#include <mutex>
 
 #include "internal.hpp"
 
@@ -210,6 +211,82 @@ extern "C" int nalu_linsys_sum_into_elements(nalu_linsys* ls, int64_t num_elems,
     NALU_HIP_CHECK(hipMemcpyAsync(&h, bad.p, sizeof(int), hipMemcpyDeviceToHost, ls->stream));
     NALU_HIP_CHECK(hipStreamSynchronize(ls->stream));
     NALU_REQUIRE(h == 0, NALU_ERR_ARG, "sumInto: entity index outside the mesh");
+    return NALU_OK;
+  });
+}
+
+// ---------------------------------------------------------------- no-allocation variant
+// The Kokkos overload above for callers that cannot allocate device memory
+// (code under stream capture, tight serving loops, harnesses that forbid
+// hipMalloc): the localIds / sortPermutation tables and the error flag live in
+// this file-scope device array instead of caller or library allocations.  Each
+// GPU holds its own copy, at its own address, until the process ends; every
+// call on a GPU shares it, so calls that overlap on one GPU race.
+namespace nalu {
+namespace {
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr size_t kScratchFlag = 256;  // the flag's slot, keeping the tables 256-byte aligned
+constexpr int kMaxDevices = 64;
+}  // namespace
+}  // namespace nalu
+
+__device__ __align__(256) unsigned char scratch[nalu::kScratchBytes];
+
+namespace nalu {
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// device address of this GPU's scratch (looked up once per device)
+unsigned char* scratch_ptr() {
+  int dev = 0;
+  NALU_HIP_CHECK(hipGetDevice(&dev));
+  NALU_REQUIRE(dev >= 0 && dev < kMaxDevices, NALU_ERR_UNSUPPORTED, "scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    NALU_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    NALU_REQUIRE(p != nullptr, NALU_ERR_HIP, "scratch: symbol address not found");
+    g_scratch_dev[dev] = p;
+  }
+  return static_cast<unsigned char*>(g_scratch_dev[dev]);
+}
+
+__global__ void k_zero_flag(int* f) { *f = 0; }
+}  // namespace
+
+bool sum_into_queue_empty(const nalu_linsys* ls) {
+  return !ls->sum_into_queue || static_cast<const HostQueue*>(ls->sum_into_queue)->count() == 0;
+}
+}  // namespace nalu
+
+extern "C" int nalu_linsys_sum_into_elements_noalloc(nalu_linsys* ls, int64_t num_elems, int num_entities,
+                                                     const int32_t* entities, const double* rhs, const double* lhs) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls && ls->finalized, NALU_ERR_STATE, "sumInto (no alloc): system not finalized");
+    NALU_REQUIRE(num_elems >= 0 && num_entities > 0 && num_entities * ls->num_dof <= kMaxRows, NALU_ERR_ARG,
+                 "sumInto (no alloc): numEntities * numDof must be in [1, 64]");
+    NALU_REQUIRE(num_elems == 0 || (entities && rhs && lhs), NALU_ERR_ARG, "sumInto (no alloc): null array");
+    const int64_t R = (int64_t)num_entities * ls->num_dof;
+    const int64_t max_elems = (int64_t)((kScratchBytes - kScratchFlag) / (2 * sizeof(int32_t))) / R;
+    NALU_REQUIRE(num_elems <= max_elems, NALU_ERR_ARG,
+                 "sumInto (no alloc): " + std::to_string(num_elems) + " elements exceed the scratch limit of " +
+                     std::to_string(max_elems) + " for " + std::to_string(R) + " rows per element");
+    // applying queued host calls would allocate staging memory
+    NALU_REQUIRE(sum_into_queue_empty(ls), NALU_ERR_STATE,
+                 "sumInto (no alloc): host sumInto calls are queued; apply them first (e.g. loadComplete)");
+    unsigned char* base = scratch_ptr();
+    materialize_pending_zero(ls);
+    if (num_elems == 0) return NALU_OK;
+    int* bad = reinterpret_cast<int*>(base);
+    int32_t* ids = reinterpret_cast<int32_t*>(base + kScratchFlag);
+    int32_t* perm = ids + num_elems * R;
+    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, ls->stream, bad);
+    launch_elements(ls, num_elems, num_entities, entities, rhs, lhs, ids, perm, bad);
+    int h = 0;
+    NALU_HIP_CHECK(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, ls->stream));
+    NALU_HIP_CHECK(hipStreamSynchronize(ls->stream));
+    NALU_REQUIRE(h == 0, NALU_ERR_ARG, "sumInto (no alloc): entity index outside the mesh");
     return NALU_OK;
   });
 }

@@ -135,6 +135,8 @@ int main() {
     EXPECT(nalu_linsys_sum_into(ls, 8, ent, rhs.data(), lhs.data()) < 0);
     EXPECT(nalu_linsys_sum_into(ls, 8, nullptr, rhs.data(), lhs.data()) < 0);
     EXPECT(nalu_linsys_sum_into_elements(ls, 1, 8, ent, rhs.data(), lhs.data(), nullptr, nullptr) < 0);
+    EXPECT(nalu_linsys_sum_into_elements_noalloc(ls, 1, 8, ent, rhs.data(), lhs.data()) < 0);
+    EXPECT(nalu_linsys_sum_into_elements_noalloc(nullptr, 1, 8, ent, rhs.data(), lhs.data()) < 0);
   }
   EXPECT(nalu_linsys_write_to_file(ls, "unused") < 0);
   EXPECT(nalu_linsys_write_to_file(ls, nullptr) < 0);
