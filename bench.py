@@ -49,8 +49,8 @@ def parse():
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--muelu-degree", type=int, default=2, help="Chebyshev degree of the AMG smoother")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-assembly / mixed-precision block")
-    ap.add_argument("--assembly-mode", default="etile", choices=["gather", "atomic", "tile", "etile"],
-                    help="scatter strategy of the continuity and momentum assembly (default: element-visit tiles)")
+    ap.add_argument("--assembly-mode", default="gather", choices=["gather", "atomic", "tile", "etile"],
+                    help="scatter strategy of the continuity and momentum assembly (default: row gather for continuity, row tiles for momentum)")
     ap.add_argument("--learned-apply-only", type=int, default=0, metavar="K",
                     help="only time K applications of an untrained learned preconditioner on the n^3 "
                          "continuity system (profiling aid), print one JSON line and exit")

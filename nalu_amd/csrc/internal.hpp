@@ -189,10 +189,6 @@ struct nalu_linsys {
   // together in time; the 8 XCDs take contiguous eighths of the order
   nalu::DevBuf<int32_t> sell_sched;
   nalu::DevBuf<int32_t> sl_col, su_col, sl_src, su_src;
-  // compressed columns: per 64-entry slot a base column, per entry the 8-bit
-  // offset from it (255 = escape to the int32 column above)
-  nalu::DevBuf<uint8_t> sl_off, su_off;
-  nalu::DevBuf<int32_t> sl_base, su_base;
   nalu::DevBuf<double> sl_val, su_val;
   nalu::DevBuf<float> sl_val32, su_val32;
   nalu::DevBuf<double> pwork;  // permuted in/out vectors of precond_apply

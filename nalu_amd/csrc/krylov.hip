@@ -355,12 +355,27 @@ struct CgState {
 // One block: sum of part[0, nb) in a fixed order into the CgState field
 // `which` (0 p.q, 1 r.r with the convergence test at iteration it, 2 r.z of
 // slot `slot`).
-__global__ __launch_bounds__(kThreads) void k_cg_reduce(const double* __restrict__ part, int nb, int which,
-                                                        int slot, int it, CgState* __restrict__ st) {
+constexpr int kReduceThreads = 1024;
+__global__ __launch_bounds__(kReduceThreads) void k_cg_reduce(const double* __restrict__ part, int nb, int which,
+                                                              int slot, int it, CgState* __restrict__ st) {
+  __shared__ double sh[kReduceThreads / 64];
   if (st->done) return;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
-  s = block_sum(s);
+  // four independent loads in flight per thread, then a fixed-order tree
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * kReduceThreads < nb; i += 4 * kReduceThreads) {
+    a0 += part[i];
+    a1 += part[i + kReduceThreads];
+    a2 += part[i + 2 * kReduceThreads];
+    a3 += part[i + 3 * kReduceThreads];
+  }
+  for (; i < nb; i += kReduceThreads) a0 += part[i];
+  double v = wave_sum((a0 + a1) + (a2 + a3));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  v = wave_sum(threadIdx.x < kReduceThreads / 64 ? sh[threadIdx.x] : 0.0);
+  const double s = v;
   if (threadIdx.x != 0) return;
   if (which == 0) {
     st->pq = s;
@@ -676,17 +691,6 @@ __global__ __launch_bounds__(kThreads) void k_bwd_color_vec(const int32_t* __res
 // ------------------------------------------------------------ SELL-64 kernels
 // Column table of one part (L or U) of the SELL copy: 8-bit offsets from a
 // per-slot base, with the int32 columns as the escape table.
-struct SellCols {
-  const uint8_t* off;
-  const int32_t* base;
-  const int32_t* col;
-  __device__ __forceinline__ int32_t at(int64_t slot, int64_t idx) const {
-    const uint32_t o = __builtin_nontemporal_load(off + idx);
-    int32_t j = base[slot] + (int32_t)o;
-    if (o == 255u) j = col[idx];
-    return j;
-  }
-};
 
 // One wavefront per chunk, one lane per row: every load of column indices and
 // values is a fully coalesced 64-lane access, no reductions or LDS.
@@ -733,45 +737,6 @@ __global__ __launch_bounds__(kThreads) void k_sell_fill(const SellChunk* __restr
   }
 }
 
-// Column compression of one SELL slot (64 entries, one wavefront): base = the
-// real column that starts the window [base, base + 254] holding the most real
-// entries of the slot (smallest on ties); an entry stores col - base in 8 bits,
-// 255 = escape to the int32 column table.  Padding entries point at base (their
-// value is 0, and base is a column the sweeps read as final).  A slot without
-// real entries keeps the padding column of its first lane.
-__global__ __launch_bounds__(kThreads) void k_sell_compress(const int32_t* __restrict__ col,
-                                                            const int32_t* __restrict__ src, int64_t nslots,
-                                                            uint8_t* __restrict__ off, int32_t* __restrict__ base) {
-  const int64_t s = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (s >= nslots) return;  // the whole wavefront leaves together
-  const int64_t at = 64 * s + lane;
-  const int32_t c = col[at];
-  const int real = src[at] >= 0 ? 1 : 0;
-  int cnt = 0;
-  for (int k = 0; k < 64; ++k) {
-    const int32_t ck = __shfl(c, k, 64);
-    const int rk = __shfl(real, k, 64);
-    const int64_t d = (int64_t)ck - (int64_t)c;
-    cnt += (rk && d >= 0 && d <= 254) ? 1 : 0;
-  }
-  int best = real ? cnt : -1;
-  int32_t bcol = c;
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    const int oc = __shfl_xor(best, o, 64);
-    const int32_t ocol = __shfl_xor(bcol, o, 64);
-    if (oc > best || (oc == best && ocol < bcol)) {
-      best = oc;
-      bcol = ocol;
-    }
-  }
-  const int32_t c0 = __shfl(c, 0, 64);
-  const int32_t b = best < 0 ? c0 : bcol;
-  if (lane == 0) base[s] = b;
-  const int64_t d = (int64_t)c - (int64_t)b;
-  off[at] = !real ? (uint8_t)0 : (d >= 0 && d <= 254 ? (uint8_t)d : (uint8_t)255);
-}
 
 template <typename T>
 __global__ void k_take_sell(const int32_t* __restrict__ src, const double* __restrict__ v, int64_t n,
@@ -784,8 +749,8 @@ __global__ void k_take_sell(const int32_t* __restrict__ src, const double* __res
 // entries stream through once (non-temporal loads), the gathered x stays cached
 template <typename T, typename TY>
 __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restrict__ ch, int64_t nch,
-                                                        const SellCols lcs, const T* __restrict__ lval,
-                                                        const SellCols ucs, const T* __restrict__ uval,
+                                                        const int32_t* __restrict__ lcs, const T* __restrict__ lval,
+                                                        const int32_t* __restrict__ ucs, const T* __restrict__ uval,
                                                         const T* __restrict__ dv, const T* __restrict__ x,
                                                         TY* __restrict__ y, const TY* __restrict__ b, int mode) {
   const int64_t c = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
@@ -797,13 +762,13 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restr
   T s = dv[row] * x[row];
 #pragma unroll 4
   for (int q = 0; q < k.lw; ++q) {
-    const int64_t at = k.loff + (int64_t)q * 64 + lane;
-    s += __builtin_nontemporal_load(&lval[at]) * x[lcs.at((k.loff >> 6) + q, at)];
+    s += __builtin_nontemporal_load(&lval[k.loff + lane + q * 64]) *
+         x[__builtin_nontemporal_load(&lcs[k.loff + lane + q * 64])];
   }
 #pragma unroll 4
   for (int q = 0; q < k.uw; ++q) {
-    const int64_t at = k.uoff + (int64_t)q * 64 + lane;
-    s += __builtin_nontemporal_load(&uval[at]) * x[ucs.at((k.uoff >> 6) + q, at)];
+    s += __builtin_nontemporal_load(&uval[k.uoff + lane + q * 64]) *
+         x[__builtin_nontemporal_load(&ucs[k.uoff + lane + q * 64])];
   }
   if (act) y[row] = mode ? (TY)((T)b[row] - s) : (TY)s;
 }
@@ -816,7 +781,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restr
 // color's chunks, so neighbouring rows gather z through the same L2.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
-                                                       const SellCols lcs, const T* __restrict__ lval,
+                                                       const int32_t* __restrict__ lcs, const T* __restrict__ lval,
                                                        const T* __restrict__ invd, const T* __restrict__ r,
                                                        T* __restrict__ z) {
   __shared__ T part[kThreads / 2];
@@ -833,10 +798,9 @@ __global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restri
   T s = T(0);
   if (act) {
 #pragma unroll 4
-    for (int q = half; q < k.lw; q += 2) {
-      const int64_t at = k.loff + (int64_t)q * 64 + lane;
-      s += __builtin_nontemporal_load(&lval[at]) * z[lcs.at((k.loff >> 6) + q, at)];
-    }
+    for (int q = half; q < k.lw; q += 2)
+      s += __builtin_nontemporal_load(&lval[k.loff + lane + q * 64]) *
+           z[__builtin_nontemporal_load(&lcs[k.loff + lane + q * 64])];
   }
   if (half == 1) part[slot] = s;
   __syncthreads();
@@ -849,7 +813,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restri
 // backward sweep over the chunks of one color, in place: z = y - (U z) / d
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
-                                                       const SellCols ucs, const T* __restrict__ uval,
+                                                       const int32_t* __restrict__ ucs, const T* __restrict__ uval,
                                                        const T* __restrict__ invd, T* __restrict__ z) {
   __shared__ T part[kThreads / 2];
   const int64_t nb = (c1 - c0 + 1) / 2;  // blocks of two chunks, XCD-contiguous ranges
@@ -865,10 +829,9 @@ __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restri
   T s = T(0);
   if (act) {
 #pragma unroll 4
-    for (int q = half; q < k.uw; q += 2) {
-      const int64_t at = k.uoff + (int64_t)q * 64 + lane;
-      s += __builtin_nontemporal_load(&uval[at]) * z[ucs.at((k.uoff >> 6) + q, at)];
-    }
+    for (int q = half; q < k.uw; q += 2)
+      s += __builtin_nontemporal_load(&uval[k.uoff + lane + q * 64]) *
+           z[__builtin_nontemporal_load(&ucs[k.uoff + lane + q * 64])];
   }
   if (half == 1) part[slot] = s;
   __syncthreads();
@@ -881,8 +844,8 @@ __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restri
 // q = A p (SELL copy); per-block partials of p.q
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sell_spmv_dot(const SellChunk* __restrict__ ch, int64_t nch,
-                                                            const SellCols lcs, const T* __restrict__ lval,
-                                                            const SellCols ucs, const T* __restrict__ uval,
+                                                            const int32_t* __restrict__ lcs, const T* __restrict__ lval,
+                                                            const int32_t* __restrict__ ucs, const T* __restrict__ uval,
                                                             const T* __restrict__ dv, const T* __restrict__ x,
                                                             T* __restrict__ y, double* __restrict__ part,
                                                             const CgState* __restrict__ st) {
@@ -897,13 +860,13 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_dot(const SellChunk* __r
     T s = dv[row] * x[row];
 #pragma unroll 4
     for (int q = 0; q < k.lw; ++q) {
-      const int64_t at = k.loff + (int64_t)q * 64 + lane;
-      s += __builtin_nontemporal_load(&lval[at]) * x[lcs.at((k.loff >> 6) + q, at)];
+      s += __builtin_nontemporal_load(&lval[k.loff + lane + q * 64]) *
+           x[__builtin_nontemporal_load(&lcs[k.loff + lane + q * 64])];
     }
 #pragma unroll 4
     for (int q = 0; q < k.uw; ++q) {
-      const int64_t at = k.uoff + (int64_t)q * 64 + lane;
-      s += __builtin_nontemporal_load(&uval[at]) * x[ucs.at((k.uoff >> 6) + q, at)];
+      s += __builtin_nontemporal_load(&uval[k.uoff + lane + q * 64]) *
+           x[__builtin_nontemporal_load(&ucs[k.uoff + lane + q * 64])];
     }
     if (act) {
       y[row] = s;
@@ -919,7 +882,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_dot(const SellChunk* __r
 // backward sweep) in part[off + block].  All return at once after convergence.
 template <typename T, bool UPPER>
 __global__ __launch_bounds__(kThreads) void k_sell_sweep_dot(const SellChunk* __restrict__ ch, int64_t c0,
-                                                             int64_t c1, const SellCols cs, const T* __restrict__ val,
+                                                             int64_t c1, const int32_t* __restrict__ cs, const T* __restrict__ val,
                                                              const T* __restrict__ invd, const T* __restrict__ r,
                                                              T* __restrict__ z, int dot, double* __restrict__ part,
                                                              int off, const CgState* __restrict__ st) {
@@ -940,10 +903,8 @@ __global__ __launch_bounds__(kThreads) void k_sell_sweep_dot(const SellChunk* __
     const int w = UPPER ? k.uw : k.lw;
     const int64_t o = UPPER ? k.uoff : k.loff;
 #pragma unroll 4
-    for (int q = half; q < w; q += 2) {
-      const int64_t at = o + (int64_t)q * 64 + lane;
-      s += __builtin_nontemporal_load(&val[at]) * z[cs.at((o >> 6) + q, at)];
-    }
+    for (int q = half; q < w; q += 2)
+      s += __builtin_nontemporal_load(&val[o + lane + q * 64]) * z[__builtin_nontemporal_load(&cs[o + lane + q * 64])];
   }
   if (half == 1) hpart[hslot] = s;
   __syncthreads();
@@ -965,8 +926,8 @@ __global__ __launch_bounds__(kThreads) void k_sell_sweep_dot(const SellChunk* __
 // with the matrix layouts of the kernels above.
 template <typename T, typename TY, int NR>
 __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __restrict__ ch, int64_t nch,
-                                                          const SellCols lcs, const T* __restrict__ lval,
-                                                          const SellCols ucs, const T* __restrict__ uval,
+                                                          const int32_t* __restrict__ lcs, const T* __restrict__ lval,
+                                                          const int32_t* __restrict__ ucs, const T* __restrict__ uval,
                                                           const T* __restrict__ dv, const T* __restrict__ x,
                                                           TY* __restrict__ y, const TY* __restrict__ b, int mode,
                                                           int64_t n) {
@@ -984,7 +945,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __res
   for (int q = 0; q < k.lw; ++q) {
     const int64_t at = k.loff + (int64_t)q * 64 + lane;
     const T a = __builtin_nontemporal_load(&lval[at]);
-    const int64_t j = lcs.at((k.loff >> 6) + q, at);
+    const int64_t j = __builtin_nontemporal_load(&lcs[at]);
 #pragma unroll
     for (int r = 0; r < NR; ++r) s[r] += a * x[j * NR + r];
   }
@@ -992,7 +953,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __res
   for (int q = 0; q < k.uw; ++q) {
     const int64_t at = k.uoff + (int64_t)q * 64 + lane;
     const T a = __builtin_nontemporal_load(&uval[at]);
-    const int64_t j = ucs.at((k.uoff >> 6) + q, at);
+    const int64_t j = __builtin_nontemporal_load(&ucs[at]);
 #pragma unroll
     for (int r = 0; r < NR; ++r) s[r] += a * x[j * NR + r];
   }
@@ -1004,7 +965,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_m(const SellChunk* __res
 // UPPER = false: forward sweep z = (r - L z) / d; true: backward z -= (U z) / d
 template <typename T, int NR, bool UPPER>
 __global__ __launch_bounds__(kThreads) void k_sell_sweep_m(const SellChunk* __restrict__ ch, int64_t c0, int64_t c1,
-                                                           const SellCols cs, const T* __restrict__ val,
+                                                           const int32_t* __restrict__ cs, const T* __restrict__ val,
                                                            const T* __restrict__ invd, const T* __restrict__ rr,
                                                            T* __restrict__ z, int64_t n) {
   __shared__ T part[NR][kThreads / 2];
@@ -1028,7 +989,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_sweep_m(const SellChunk* __re
     for (int q = half; q < w; q += 2) {
       const int64_t at = off + (int64_t)q * 64 + lane;
       const T a = __builtin_nontemporal_load(&val[at]);
-      const int64_t j = cs.at((off >> 6) + q, at);
+      const int64_t j = __builtin_nontemporal_load(&cs[at]);
 #pragma unroll
       for (int r = 0; r < NR; ++r) s[r] += a * z[j * NR + r];
     }
@@ -1180,17 +1141,6 @@ void build_sell(nalu_linsys* ls) {
                      ls->sl_col.p, ls->sl_src.p);
   hipLaunchKernelGGL(k_sell_fill, dim3(nbw), dim3(kThreads), 0, s, dch, nch, ls->urp.p, ls->uci.p, ls->usrc.p, 1,
                      ls->su_col.p, ls->su_src.p);
-  // compressed columns (every chunk offset is a multiple of 64: one slot = 64 entries)
-  ls->sl_off.alloc(std::max<int64_t>(1, lo));
-  ls->su_off.alloc(std::max<int64_t>(1, uo));
-  ls->sl_base.alloc(std::max<int64_t>(1, lo / 64));
-  ls->su_base.alloc(std::max<int64_t>(1, uo / 64));
-  if (lo > 0)
-    hipLaunchKernelGGL(k_sell_compress, dim3((unsigned)((lo / 64 + 3) / 4)), dim3(kThreads), 0, s, ls->sl_col.p,
-                       ls->sl_src.p, lo / 64, ls->sl_off.p, ls->sl_base.p);
-  if (uo > 0)
-    hipLaunchKernelGGL(k_sell_compress, dim3((unsigned)((uo / 64 + 3) / 4)), dim3(kThreads), 0, s, ls->su_col.p,
-                       ls->su_src.p, uo / 64, ls->su_off.p, ls->su_base.p);
   NALU_HIP_CHECK(hipGetLastError());
   ls->sell_ready = true;
 }
@@ -1209,9 +1159,7 @@ Split<T> split_of(nalu_linsys* ls) {
     return {ls->lrp.p, ls->lci.p, ls->urp.p, ls->uci.p, ls->lval32.p, ls->uval32.p, ls->dval32.p, ls->pinvd32.p};
 }
 
-SellCols sell_cols(const nalu_linsys* ls, bool upper) {
-  return upper ? SellCols{ls->su_off.p, ls->su_base.p, ls->su_col.p} : SellCols{ls->sl_off.p, ls->sl_base.p, ls->sl_col.p};
-}
+const int32_t* sell_cols(const nalu_linsys* ls, bool upper) { return upper ? ls->su_col.p : ls->sl_col.p; }
 
 template <typename T>
 const T* sell_vals(const nalu_linsys* ls, bool upper) {
@@ -1635,7 +1583,8 @@ int cg_fused(nalu_linsys* ls, const double* b, double* x, int maxit, double tol)
   ls->cg_host.alloc(sizeof(CgState));
   CgState* hs = static_cast<CgState*>(ls->cg_host.p);
   Red red(ls);
-  const SellCols lcs = sell_cols(ls, false), ucs = sell_cols(ls, true);
+  const int32_t* lcs = sell_cols(ls, false);
+  const int32_t* ucs = sell_cols(ls, true);
   const double* lval = sell_vals<double>(ls, false);
   const double* uval = sell_vals<double>(ls, true);
   const double* invd = split_of<double>(ls).invd;
@@ -1657,10 +1606,10 @@ int cg_fused(nalu_linsys* ls, const double* b, double* x, int maxit, double tol)
     ++it;
     hipLaunchKernelGGL(k_sell_spmv_dot<double>, dim3(g_spmv), dim3(kThreads), 0, s, ch, nch, lcs, lval, ucs, uval,
                        split_of<double>(ls).dv, (const double*)p, q, part, (const CgState*)st);
-    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kThreads), 0, s, (const double*)part, (int)g_spmv, 0, 0, it, st);
+    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kReduceThreads), 0, s, (const double*)part, (int)g_spmv, 0, 0, it, st);
     hipLaunchKernelGGL(k_cg_update_f, dim3(g_upd), dim3(kThreads), 0, s, x, r, (const double*)p,
                        (const double*)q, n, cur, part, (const CgState*)st);
-    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kThreads), 0, s, (const double*)part, (int)g_upd, 1, 0, it, st);
+    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kReduceThreads), 0, s, (const double*)part, (int)g_upd, 1, 0, it, st);
     const int nxt = 1 - cur;
     for (int c = 0; c < nc; ++c)
       hipLaunchKernelGGL((k_sell_sweep_dot<double, false>), dim3(sweep_grid(c)), dim3(kThreads), 0, s, ch, ccp[c],
@@ -1669,7 +1618,7 @@ int cg_fused(nalu_linsys* ls, const double* b, double* x, int maxit, double tol)
     for (int c = nc - 2; c >= 0; --c)
       hipLaunchKernelGGL((k_sell_sweep_dot<double, true>), dim3(sweep_grid(c)), dim3(kThreads), 0, s, ch, ccp[c],
                          ccp[c + 1], ucs, uval, invd, (const double*)r, z, 1, part, off[c], (const CgState*)st);
-    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kThreads), 0, s, (const double*)part, total, 2, nxt, it, st);
+    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kReduceThreads), 0, s, (const double*)part, total, 2, nxt, it, st);
     hipLaunchKernelGGL(k_cg_p_f, dim3(nblocks(n)), dim3(kThreads), 0, s, p, (const double*)z, n, nxt, cur,
                        (const CgState*)st);
     cur = nxt;

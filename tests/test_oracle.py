@@ -221,6 +221,73 @@ def test_momentum_structure(oracle, golds):
     np.testing.assert_allclose(lhs0 @ np.tile([1.0, -2.0, 0.5], 8), 0.0, atol=1e-13)
 
 
+def _scs_centroid(l, r):
+    """Unit-cube coordinates of the centroid of the sub-control surface between
+    nodes l and r: the edge midpoint along the edge, a quarter of the way in
+    from node l across it."""
+    xi = 0.5 * (UNIT_HEX[l] + UNIT_HEX[r])
+    off = UNIT_HEX[r] == UNIT_HEX[l]
+    xi[off] = 0.25 + 0.5 * UNIT_HEX[l][off]
+    return xi
+
+
+@pytest.mark.parametrize("shifted", [False, True])
+def test_momentum_include_divu_closed_form(oracle, golds, shifted):
+    """The includeDivU term completes the stress to mu (grad u + grad u^T - 2/3 div u I),
+    which vanishes for an isotropic dilatation u = c x + b on any Hex8 (a linear
+    field is exact, so grad u = c I at every ip): with no advection the rhs is
+    zero with the term and 2 mu c A per scs without it.  A solenoidal linear
+    field (trace 0) makes the term vanish, so both rhs agree."""
+    X = np.array(golds["perturbed_hex8_coords"]["values"]).reshape(8, 3)
+    rng = np.random.default_rng(11)
+    mu = rng.uniform(0.1, 0.3, 8)
+    c, b = 0.7, rng.normal(size=3)
+    u = c * X + b
+    _, r1 = oracle.momentum_advdiff_elem(X, u, mu, np.zeros(12), include_divu=1.0, shifted=shifted)
+    _, r0 = oracle.momentum_advdiff_elem(X, u, mu, np.zeros(12), include_divu=0.0, shifted=shifted)
+    np.testing.assert_allclose(r1, 0.0, atol=1e-13)
+    # without the term: rhs_L += 2 mu_ip c A_ip, rhs_R -= the same (mu_ip from the standard ips)
+    A = oracle.scs_areav(X)
+    expect = np.zeros((8, 3))
+    for ip, (l, r) in enumerate(oracle.lrscv()):
+        xi = _scs_centroid(l, r)
+        N = np.prod(np.where(UNIT_HEX == 1, xi, 1.0 - xi), axis=1)  # trilinear weights on [0,1]^3
+        f = 2.0 * (N @ mu) * c * A[ip]
+        expect[l] += f
+        expect[r] -= f
+    np.testing.assert_allclose(r0, expect.ravel(), rtol=0, atol=1e-13 * np.abs(expect).max())
+    G = rng.normal(size=(3, 3))
+    G -= np.trace(G) / 3.0 * np.eye(3)
+    u = X @ G.T + b
+    _, s1 = oracle.momentum_advdiff_elem(X, u, mu, np.zeros(12), include_divu=1.0, shifted=shifted)
+    _, s0 = oracle.momentum_advdiff_elem(X, u, mu, np.zeros(12), include_divu=0.0, shifted=shifted)
+    np.testing.assert_allclose(s1, s0, rtol=0, atol=1e-13 * np.abs(s0).max())
+
+
+@pytest.mark.parametrize("skew", [False, True])
+def test_momentum_advection_ip_values_closed_form(oracle, skew):
+    """Advection interpolates u at the scs ip: the centroid of the sub-control
+    surface (edge midpoint, the two face centres on the edge, the element centre)
+    by default, the edge midpoint in the skew-symmetric form.  On an affine
+    image of the unit cube a linear u is exact, so rhs_L -= mdot u(x_ip) and
+    rhs_R += the same, with x_ip from that geometry alone; lhs u + rhs = 0."""
+    rng = np.random.default_rng(17 + skew)
+    Amap = np.eye(3) + 0.2 * rng.normal(size=(3, 3))
+    X = UNIT_HEX @ Amap.T + rng.normal(size=3)
+    G, b = rng.normal(size=(3, 3)), rng.normal(size=3)
+    u = X @ G.T + b
+    md = rng.normal(size=12)
+    lhs, rhs = oracle.momentum_advdiff_elem(X, u, np.zeros(8), md, skew=skew)
+    expect = np.zeros((8, 3))
+    for ip, (l, r) in enumerate(oracle.lrscv()):
+        xi = 0.5 * (UNIT_HEX[l] + UNIT_HEX[r]) if skew else _scs_centroid(l, r)
+        uip = G @ (Amap @ xi + X[0] - Amap @ UNIT_HEX[0]) + b
+        expect[l] -= md[ip] * uip
+        expect[r] += md[ip] * uip
+    np.testing.assert_allclose(rhs, expect.ravel(), rtol=0, atol=1e-12 * np.abs(expect).max())
+    np.testing.assert_allclose(lhs @ u.ravel() + rhs, 0.0, atol=1e-12 * np.abs(rhs).max())
+
+
 # ----------------------------------------------------------------- graph + scatter
 
 def test_tpetra_1x1x2_graph_and_values(oracle, golds):
