@@ -7,6 +7,8 @@ viscosity 1e-3, dt = gamma1 = 1.  One step is the hot path once:
 
   1. mdot at the 12 scs of every element (ComputeMdotElemAlgorithm)
   2. continuity system: zeroSystem, ContinuityAdvElemKernel assembly, loadComplete
+     (by default 1 and 2 are one pass: the continuity row gather evaluates each
+     scs's geometry and writes its mdot too; --separate-mdot runs them apart)
   3. momentum system (3 dof/node, 27-point block rows): zeroSystem, MomentumAdvDiffElemKernel
   4. pressure-Poisson solve: CG + multicolor SGS to 1e-8 FP64 true relative residual
      on the assembled continuity matrix with its zero-mean-projected rhs.
@@ -51,6 +53,8 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-assembly / mixed-precision block")
     ap.add_argument("--assembly-mode", default="gather", choices=["gather", "atomic", "tile", "etile"],
                     help="scatter strategy of the continuity and momentum assembly (default: row gather for continuity, row tiles for momentum)")
+    ap.add_argument("--separate-mdot", action="store_true",
+                    help="compute mdot in its own pass instead of inside the continuity row gather")
     ap.add_argument("--learned-apply-only", type=int, default=0, metavar="K",
                     help="only time K applications of an untrained learned preconditioner on the n^3 "
                          "continuity system (profiling aid), print one JSON line and exit")
@@ -119,7 +123,7 @@ def train_network(args, na, torch, devc, Pt, tag=""):
     iterations (nalu_amd.training).  Returns (net, training record)."""
     from nalu_amd.training import train_pressure_preconditioner
     return train_pressure_preconditioner(
-        Pt, num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=args.train_epochs, batch=16, lr=3e-3, seed=1,
+        Pt, num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=args.train_epochs, batch=16, lr=1e-3, seed=1,
         net_seed=2, device=devc, log=lambda m: print("bench: %s%s" % (tag, m), file=sys.stderr, flush=True))
 
 
@@ -532,16 +536,21 @@ def main():
     ev = lambda: torch.cuda.Event(enable_timing=True)
     cmode = args.assembly_mode
     mmode = "tile" if args.assembly_mode == "gather" else args.assembly_mode  # no row gather for numDof 3
+    fused = cmode == "gather" and not args.separate_mdot
 
     def step(timers=None):
         e = [ev() for _ in range(6)] if timers is not None else None
         if e: e[5].record()
         na.compute_nodal_gradient(mesh, p, dual_vol, gp, stream=stream)
         if e: e[0].record()
-        na.compute_mdot(mesh, rho, p, u, gp, mdot, tau=P["tau"], stream=stream)
+        if not fused:
+            na.compute_mdot(mesh, rho, p, u, gp, mdot, tau=P["tau"], stream=stream)
         if e: e[1].record()
         cont.zeroSystem()
-        na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode=cmode)
+        if fused:  # continuity + mdot from one evaluation of the scs geometry
+            na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode="gather", mdot_out=mdot)
+        else:
+            na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode=cmode)
         cont.loadComplete()
         if e: e[2].record()
         mom.zeroSystem()
@@ -604,6 +613,8 @@ def main():
     ncol = len(cp) - 1
 
     byt = algorithmic_bytes(N, E, Pn, rows, nnz1, nnz3)
+    if fused:  # the continuity pass also writes mdot [E][12]
+        byt["continuity_assembly"] += 12 * E * 8
     per_step = {
         "mdot": (med["mdot"], byt["mdot"], 1),
         "continuity_assembly": (med["continuity_assembly"], byt["continuity_assembly"], 1),
@@ -615,7 +626,9 @@ def main():
              "atomic": ("k_scalar_atomic<ContinuityOp>", "k_momentum_atomic"),
              "tile": ("k_scalar_tile<ContinuityOp>", "k_momentum_tile"),
              "etile": ("k_scalar_items<ContinuityEt> + k_scalar_etile", "k_momentum_items + k_momentum_etile")}
-    kern_name = {"mdot": "k_mdot_elem", "continuity_assembly": kname[cmode][0],
+    if fused:
+        del per_step["mdot"]
+    kern_name = {"mdot": "k_mdot_elem", "continuity_assembly": kname[cmode][0] + (" (+ mdot)" if fused else ""),
                  "momentum_assembly": kname[mmode][1], "sgs_apply": "k_sell_fwd + k_sell_bwd",
                  "spmv": "k_sell_spmv"}
     dom = max(per_step, key=lambda k: per_step[k][0])
@@ -665,6 +678,7 @@ def main():
         "data": "synthetic (seeded LCG mesh jitter and velocity noise, analytic Taylor-Green fields)",
         "config": {"workload": "low-Mach momentum + continuity assembly and CG+SGS pressure-Poisson solve, "
                                "%d^3 Hex8, jitter 0.1h, plain FP64" % n, "assembly_mode": cmode,
+                   "mdot": "written by the continuity row gather" if fused else "separate pass",
                    "elements": E, "nodes": N, "continuity_nnz": nnz1, "momentum_nnz": nnz3,
                    "parallelism": "single GPU"},
         "assembly_s": t_asm,

@@ -294,6 +294,15 @@ int nalu_assemble_continuity(nalu_linsys *ls, const nalu_mesh *m, const double *
                              const double *p_dev, const double *u_dev, const double *gpdx_dev,
                              double proj_time_scale, int shift_mdot, int shift_poisson,
                              int reduced_sens, int mode);
+/* ContinuityAdvElemKernel and ComputeMdotElemAlgorithm
+ * (src/ComputeMdotElemAlgorithm.C:85-250) from one evaluation of the scs
+ * geometry, as the consolidated algorithm's kernels share their element data:
+ * the continuity system as nalu_assemble_continuity (row gather) and mdot_dev
+ * [E][12] as nalu_compute_mdot (same formula; sums in another order). */
+int nalu_assemble_continuity_mdot(nalu_linsys *ls, const nalu_mesh *m, const double *rho_dev,
+                                  const double *p_dev, const double *u_dev, const double *gpdx_dev,
+                                  double proj_time_scale, int shift_mdot, int shift_poisson,
+                                  int reduced_sens, double *mdot_dev);
 /* MomentumAdvDiffElemKernel (src/kernel/MomentumAdvDiffElemKernel.C:72-186), numDof 3. */
 int nalu_assemble_momentum(nalu_linsys *ls, const nalu_mesh *m, const double *u_dev,
                            const double *mu_dev, const double *mdot_dev, double include_divu,

@@ -124,6 +124,8 @@ def lib():
         L.nalu_assemble_scalar_diff.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int]
         L.nalu_compute_mdot.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp]
         L.nalu_compute_mdot_f32.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp]
+        L.nalu_assemble_continuity_mdot.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int,
+                                                    C.c_int, _vp]
         L.nalu_assemble_continuity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int,
                                                C.c_int, C.c_int]
         L.nalu_assemble_momentum.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_int]
@@ -725,7 +727,7 @@ class LearnedPreconditioner:
                                         C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))), "gnn_loss_grad")
         return loss.value, g
 
-    def train(self, ls, pairs, epochs=4, batch=16, lr=3e-3, seed=0, validate=None):
+    def train(self, ls, pairs, epochs=4, batch=16, lr=1e-3, seed=0, validate=None):
         """Adam on the harvested pairs; returns the mean loss of each epoch.
 
         validate: optional callable(net) -> score (lower is better), evaluated
@@ -800,8 +802,20 @@ def compute_nodal_gradient(mesh, q, dual_vol, dqdx, shifted=False, stream=None, 
 
 
 def assemble_continuity(ls, mesh, rho, p, u, gpdx, tau=1.0, shift_mdot=False, shift_poisson=False,
-                        reduced_sens=False, mode="tile"):
+                        reduced_sens=False, mode="tile", mdot_out=None):
+    """ContinuityAdvElemKernel.  mdot_out ([E][12] device tensor, row-gather mode
+    only) also receives mdot from the same scs evaluations
+    (ComputeMdotElemAlgorithm), so no separate compute_mdot pass is needed."""
     N = mesh.num_nodes
+    if mdot_out is not None:
+        if mode != "gather":
+            raise NaluError("assemble_continuity: mdot_out needs mode 'gather'")
+        _check(lib().nalu_assemble_continuity_mdot(
+            ls.handle, mesh.handle, _dev(rho, "continuity rho", "f64", numel=N), _dev(p, "continuity p", "f64", numel=N),
+            _dev(u, "continuity u", "f64", numel=3 * N), _dev(gpdx, "continuity Gpdx", "f64", numel=3 * N),
+            float(tau), int(shift_mdot), int(shift_poisson), int(reduced_sens),
+            _dev(mdot_out, "continuity mdot_out", "f64", numel=12 * mesh.num_elems)), "assemble_continuity")
+        return
     _check(lib().nalu_assemble_continuity(ls.handle, mesh.handle, _dev(rho, "continuity rho", "f64", numel=N),
                                           _dev(p, "continuity p", "f64", numel=N),
                                           _dev(u, "continuity u", "f64", numel=3 * N),

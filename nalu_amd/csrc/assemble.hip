@@ -14,6 +14,8 @@
 //     computing row w of 64 elements, FP64 atomic adds into the CSR.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hex8_device.hpp"
 #include "internal.hpp"
 
@@ -84,12 +86,27 @@ struct ContinuityOp {
   const double* u;
   const double* gp;
   double tau;
+  double* mdot_out = nullptr;  // [E][12]: the mdot of each scs, written by the row of its left node
   template <int LA, class Emit>
   __device__ __forceinline__ R row(const int32_t (&nd)[8], const R (&x)[8][3], Emit&& emit) const {
     const hex8::Gather gt{nd};
     return hex8::row_continuity<LA, SHIFT_MDOT, SHIFT_POISSON, REDUCED, double>(x, gt, rho, p, u, gp, R(tau), emit);
   }
+  // the same row, also delivering the mdot of the scs it owns (mdot_out != nullptr)
+  template <int LA, class Emit>
+  __device__ __forceinline__ R row_e(const int32_t (&nd)[8], const R (&x)[8][3], int64_t e, Emit&& emit) const {
+    const hex8::Gather gt{nd};
+    double* mo = mdot_out + 12 * e;
+    return hex8::row_continuity<LA, SHIFT_MDOT, SHIFT_POISSON, REDUCED, double>(
+        x, gt, rho, p, u, gp, R(tau), emit, [&](int ip, R v) { mo[ip] = (double)v; });
+  }
 };
+
+// ops whose rows can also write per-element outputs (row_e)
+template <class Op, class = void>
+struct HasRowE : std::false_type {};
+template <class Op>
+struct HasRowE<Op, std::void_t<decltype(&Op::mdot_out)>> : std::true_type {};
 
 // dispatch a wave-uniform (or mostly uniform) runtime row index to the
 // compile-time specialisation
@@ -106,6 +123,29 @@ __device__ __forceinline__ typename Op::real dispatch_row(const Op& op, int la, 
     case 6: return op.template row<6>(nd, x, emit);
     default: return op.template row<7>(nd, x, emit);
   }
+}
+
+// dispatch_row with the element index, for ops with per-element outputs when
+// they are requested
+template <class Op, class Emit>
+__device__ __forceinline__ typename Op::real dispatch_row_e(const Op& op, int la, const int32_t (&nd)[8],
+                                                            const typename Op::real (&x)[8][3], int64_t e,
+                                                            Emit&& emit) {
+  if constexpr (HasRowE<Op>::value) {
+    if (op.mdot_out) {
+      switch (la) {
+        case 0: return op.template row_e<0>(nd, x, e, emit);
+        case 1: return op.template row_e<1>(nd, x, e, emit);
+        case 2: return op.template row_e<2>(nd, x, e, emit);
+        case 3: return op.template row_e<3>(nd, x, e, emit);
+        case 4: return op.template row_e<4>(nd, x, e, emit);
+        case 5: return op.template row_e<5>(nd, x, e, emit);
+        case 6: return op.template row_e<6>(nd, x, e, emit);
+        default: return op.template row_e<7>(nd, x, e, emit);
+      }
+    }
+  }
+  return dispatch_row(op, la, nd, x, emit);
 }
 
 __device__ __forceinline__ int32_t pick(const int32_t (&nd)[8], int la) {
@@ -211,8 +251,8 @@ __global__ __launch_bounds__(256) void k_scalar_gather(Op op, const int32_t* __r
       typename Op::real x[8][3];
       load_coords(coords, nd, x);
       const uint64_t pk = elem_pos[t];
-      rr += (double)dispatch_row(op, la, nd, x,
-                                 [&](int c, auto v) { myrow[(int)((pk >> (8 * c)) & 0xff)] += (double)v; });
+      rr += (double)dispatch_row_e(op, la, nd, x, e,
+                                   [&](int c, auto v) { myrow[(int)((pk >> (8 * c)) & 0xff)] += (double)v; });
     }
   }
   __syncthreads();
@@ -897,6 +937,35 @@ extern "C" int nalu_assemble_continuity(nalu_linsys* ls, const nalu_mesh* m, con
       NALU_CT(6, true, true, false)
       NALU_CT(7, true, true, true)
 #undef NALU_CT
+    }
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_assemble_continuity_mdot(nalu_linsys* ls, const nalu_mesh* m, const double* rho,
+                                             const double* p, const double* u, const double* gp, double tau,
+                                             int shift_mdot, int shift_poisson, int reduced_sens, double* mdot) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 1);
+    NALU_REQUIRE(rho && p && u && gp && mdot, NALU_ERR_ARG, "continuity + mdot: null field");
+    NALU_REQUIRE(tau > 0.0, NALU_ERR_ARG, "continuity + mdot: projTimeScale must be positive");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    const int key = (shift_mdot ? 4 : 0) | (shift_poisson ? 2 : 0) | (reduced_sens ? 1 : 0);
+    switch (key) {
+#define NALU_CM(K, A, B, C)                                                                              \
+  case K:                                                                                                \
+    if (fp32_assembly(ls)) launch_scalar(ls, m, ContinuityOp<A, B, C, float>{rho, p, u, gp, tau, mdot}, 0); \
+    else launch_scalar(ls, m, ContinuityOp<A, B, C>{rho, p, u, gp, tau, mdot}, 0);                        \
+    break;
+      NALU_CM(0, false, false, false)
+      NALU_CM(1, false, false, true)
+      NALU_CM(2, false, true, false)
+      NALU_CM(3, false, true, true)
+      NALU_CM(4, true, false, false)
+      NALU_CM(5, true, false, true)
+      NALU_CM(6, true, true, false)
+      NALU_CM(7, true, true, true)
+#undef NALU_CM
     }
     return NALU_OK;
   });

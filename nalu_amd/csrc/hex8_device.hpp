@@ -306,14 +306,24 @@ __device__ __forceinline__ R ip_mdot_regs(const IpGeo<JR, R>& G, const double (&
   return rua - tau * (dpa - gpa);
 }
 
+// no-op receiver of the per-ip mdot values of row_continuity
+struct NoIpOut {
+  template <typename R>
+  __device__ __forceinline__ void operator()(int, R) const {}
+};
+
 // ContinuityAdvElemKernel, row LA (src/kernel/ContinuityAdvElemKernel.C:88-148):
 // lhs(la,c) = sum_ip S (-dN_lhs_c/dx . A); rhs(la) = -sum_ip S mdot_ip / tau,
 // with mdot_ip as in ComputeMdotElemAlgorithm.  Nodal fields are read once per
-// node and folded into the three ip accumulators.
-template <int LA, bool SHIFT_MDOT, bool SHIFT_POISSON, bool REDUCED, typename JR, typename R, class Emit>
+// node and folded into the three ip accumulators.  ipout(ip, mdot_ip) receives
+// the mdot of each adjacent scs whose left node is LA (each scs of an element
+// has exactly one left node, so a pass over the element's rows delivers all 12).
+template <int LA, bool SHIFT_MDOT, bool SHIFT_POISSON, bool REDUCED, typename JR, typename R, class Emit,
+          class IpOut = NoIpOut>
 __device__ __forceinline__ R row_continuity(const R (&x)[8][3], const Gather& gt, const double* __restrict__ rho,
                                             const double* __restrict__ p, const double* __restrict__ u,
-                                            const double* __restrict__ gp, R tau, Emit&& emit) {
+                                            const double* __restrict__ gp, R tau, Emit&& emit,
+                                            IpOut&& ipout = IpOut{}) {
   constexpr bool LHS_SHIFT = SHIFT_POISSON || REDUCED;
   R A[3][3], g[3][3], gl[3][3];
   static_for<3>([&](auto kk) {
@@ -348,6 +358,7 @@ __device__ __forceinline__ R row_continuity(const R (&x)[8][3], const Gather& gt
   static_for<3>([&](auto kk) {
     constexpr R S = R(adj_sign(LA, decltype(kk)::value));
     rhs -= S * (fl[kk] / tau);
+    if constexpr (adj_sign(LA, decltype(kk)::value) > 0) ipout(adj_ip(LA, decltype(kk)::value), fl[kk]);
   });
   static_for<8>([&](auto cc) {
     constexpr int c = decltype(cc)::value;

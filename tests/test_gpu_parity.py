@@ -184,6 +184,41 @@ def test_continuity_and_mdot_parity(oracle, mode):
     np.testing.assert_allclose(mdot.cpu().numpy(), om, rtol=0, atol=TOL * np.abs(om).max())
 
 
+@pytest.mark.parametrize("shift_mdot,shift_poisson,reduced", [(False, False, False), (True, False, False),
+                                                               (False, True, True), (True, True, False)])
+def test_continuity_with_fused_mdot(oracle, shift_mdot, shift_poisson, reduced):
+    """The row gather that also writes mdot: the continuity system bit-identical to
+    the plain row gather, mdot against the oracle and within round-off of the
+    separate mdot pass, every (element, scs) written."""
+    n = 7
+    print("case: continuity + fused mdot n=%d shift_mdot=%d shift_poisson=%d reduced=%d"
+          % (n, shift_mdot, shift_poisson, reduced), flush=True)
+    P = lowmach_problem(n, seed=23, rho_sigma=0.3)
+    mesh, ls = make_system(P["coords"], P["conn"])
+    rho, p, u, gp = dev(P["rho"]), dev(P["p"]), dev(P["u"]), dev(P["gpdx"])
+    E = P["conn"].shape[0]
+    kw = dict(tau=P["tau"], shift_mdot=shift_mdot, shift_poisson=shift_poisson, reduced_sens=reduced)
+    ls.zeroSystem()
+    na.assemble_continuity(ls, mesh, rho, p, u, gp, mode="gather", **kw)
+    _, _, v0, r0 = ls.csr()
+    md = torch.full((E, 12), float("nan"), dtype=torch.float64, device=DEV)
+    ls.zeroSystem()
+    na.assemble_continuity(ls, mesh, rho, p, u, gp, mode="gather", mdot_out=md, **kw)
+    _, _, v1, r1 = ls.csr()
+    np.testing.assert_array_equal(v1, v0)
+    np.testing.assert_array_equal(r1, r0)
+    ref = torch.zeros_like(md)
+    na.compute_mdot(mesh, rho, p, u, gp, ref, tau=P["tau"], shift_mdot=shift_mdot, shift_poisson=shift_poisson)
+    m1, m0 = md.cpu().numpy(), ref.cpu().numpy()
+    assert np.isfinite(m1).all()
+    np.testing.assert_allclose(m1, m0, rtol=0, atol=1e-13 * np.abs(m0).max())
+    om = oracle.compute_mdot(P["conn"], P["coords"], P["rho"], P["p"], P["u"], P["gpdx"], P["tau"],
+                             shift_mdot=shift_mdot, shift_poisson=shift_poisson)
+    np.testing.assert_allclose(m1, om, rtol=0, atol=TOL * np.abs(om).max())
+    with pytest.raises(na.NaluError, match="gather"):
+        na.assemble_continuity(ls, mesh, rho, p, u, gp, mode="tile", mdot_out=md, **kw)
+
+
 @pytest.mark.parametrize("mode", ["atomic", "tile", "etile"])
 def test_momentum_parity(oracle, mode):
     n = 6
