@@ -408,7 +408,15 @@ int nalu_gnn_set_params(nalu_gnn *g, const float *params_host, int64_t n);
 int nalu_gnn_get_grad(const nalu_gnn *g, float *grad_host, int64_t n);
 /* Attach (g != NULL) or detach the network used by a system's solve. */
 int nalu_linsys_set_learned_precond(nalu_linsys *ls, nalu_gnn *g);
-/* 1 when the last solve fell back from the network to SGS. */
+/* No counterpart in the reference.  The network's base preconditioner:
+ * NALU_PREC_SGS (default) or NALU_PREC_MUELU (the AMG V-cycle, built at the
+ * preconditioner set-up with the system's MueLu parameters).  The base gives
+ * the network's second input feature and its initial output (the untrained
+ * network returns the base's output) and is the fallback; a network trained
+ * with one base is meant for that base.  Takes effect at the next set-up; set
+ * it before harvesting pairs too (their stored feature is the base's output). */
+int nalu_linsys_set_learned_base(nalu_linsys *ls, int precond);
+/* 1 when the last solve fell back from the network to its base preconditioner. */
 int nalu_linsys_learned_fallback(const nalu_linsys *ls);
 /* z = net(r) for one right-hand side, FP32 device arrays in row order; needs a
  * set-up preconditioner (nalu_linsys_precond_setup or a solve). */

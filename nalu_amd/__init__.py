@@ -163,6 +163,7 @@ def lib():
         L.nalu_gnn_data_pair.argtypes = [_vp, C.c_int, _vp, _vp]
         L.nalu_gnn_train.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
         L.nalu_linsys_set_learned_precond.argtypes = [_vp, _vp]
+        L.nalu_linsys_set_learned_base.argtypes = [_vp, C.c_int]
         L.nalu_linsys_set_write_matrix_files.argtypes = [_vp, C.c_char_p]
         L.nalu_linsys_learned_fallback.argtypes = [_vp]
         L.nalu_elem_algorithm_create.argtypes = [_vp, _vp, C.POINTER(_vp)]
@@ -636,8 +637,15 @@ class LinearSystem:
         _check(lib().nalu_linsys_set_learned_precond(self._h, None if net is None else net.handle),
                "set_learned_precond")
 
+    def set_learned_base(self, base):
+        """The learned preconditioner's base: "sgs" (default) or "muelu" (the AMG
+        V-cycle with this system's MueLu parameters): the network's input feature,
+        its initial output and its fallback.  Set it before the preconditioner
+        set-up (and before harvesting training pairs)."""
+        _check(lib().nalu_linsys_set_learned_base(self._h, PRECOND[base]), "set_learned_base")
+
     def learned_fallback(self):
-        """True when the last solve fell back from the network to SGS."""
+        """True when the last solve fell back from the network to its base."""
         return bool(_check(lib().nalu_linsys_learned_fallback(self._h), "learned_fallback"))
 
 
@@ -727,7 +735,7 @@ class LearnedPreconditioner:
                                         C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))), "gnn_loss_grad")
         return loss.value, g
 
-    def train(self, ls, pairs, epochs=4, batch=16, lr=1e-3, seed=0, validate=None):
+    def train(self, ls, pairs, epochs=4, batch=16, lr=5e-4, seed=0, validate=None):
         """Adam on the harvested pairs; returns the mean loss of each epoch.
 
         validate: optional callable(net) -> score (lower is better), evaluated

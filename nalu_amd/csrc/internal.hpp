@@ -202,6 +202,9 @@ struct nalu_linsys {
 
   // learned preconditioner (optional)
   void* learned = nullptr;
+  // its base preconditioner: the input feature, the network's initial output
+  // and the fallback (NALU_PREC_SGS or NALU_PREC_MUELU)
+  int learned_base = NALU_PREC_SGS;
   // element-visit tile plans of the assembly (etile.hip)
   void* etile_plans = nullptr;
   // queued calls of the host (std::vector) sumInto overload (suminto.hip)

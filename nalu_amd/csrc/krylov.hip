@@ -1340,7 +1340,9 @@ void pprec(nalu_linsys* ls, const T* r, T* z) {
         else learned_apply_f64(ls, r, z);
         break;
       }
-      psgs<T>(ls, r, z);  // no network attached, or fallen back
+      // no network attached, or fallen back: the base preconditioner
+      if (ls->learned_base == NALU_PREC_MUELU) amg_apply<T>(ls, r, z);
+      else psgs<T>(ls, r, z);
       break;
     case NALU_PREC_SGS:
       psgs<T>(ls, r, z);
@@ -1765,8 +1767,11 @@ void precond_setup(nalu_linsys* ls) {
   ls->split32_ready = false;
   if (mixed(ls) || ls->learned) ensure_split32(ls);
   learned_setup(ls);
-  if (ls->cfg.precond == NALU_PREC_MUELU) amg_setup(ls);
-  else amg_release(ls);
+  if (ls->cfg.precond == NALU_PREC_MUELU ||
+      (ls->cfg.precond == NALU_PREC_LEARNED && ls->learned_base == NALU_PREC_MUELU))
+    amg_setup(ls);
+  else
+    amg_release(ls);
 }
 
 // M^-1 r in the original row order (permutes in and out through two

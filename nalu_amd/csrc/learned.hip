@@ -1342,6 +1342,13 @@ void refresh_inference(nalu_linsys* ls, Learned* st) {
   st->infer_ready = true;
 }
 
+// the network's base preconditioner in the permuted order (FP32): SGS, or the
+// AMG V-cycle when the system's learned base is NALU_PREC_MUELU
+void base32(nalu_linsys* ls, const float* r, float* z) {
+  if (ls->learned_base == NALU_PREC_MUELU) amg_apply<float>(ls, r, z);
+  else perm_sgs32(ls, r, z);
+}
+
 // z = net(r) for one vector in the permuted order (FP32)
 void apply_perm(nalu_linsys* ls, Learned* st, const nalu_gnn* g, const float* r, float* z) {
   hipStream_t s = ls->stream;
@@ -1349,7 +1356,7 @@ void apply_perm(nalu_linsys* ls, Learned* st, const nalu_gnn* g, const float* r,
   const Layout lay{g->L};
   NALU_REQUIRE(st->infer_ready, NALU_ERR_STATE, "learned preconditioner: inference state not set up");
   ensure_one(st, n);
-  perm_sgs32(ls, r, st->zs.p);
+  base32(ls, r, st->zs.p);
   const unsigned nb = std::min<unsigned>(kRedBlocks, nblocks(n));
   hipLaunchKernelGGL((k_bdot<float, float>), dim3(nb, 1), dim3(kThreads), 0, s, r, r, n, st->part.p);
   hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, st->part.p, (int)nb, st->sumsq.p, 1, 0);
@@ -1619,6 +1626,19 @@ extern "C" int nalu_gnn_get_grad(const nalu_gnn* g, float* host, int64_t n) {
   });
 }
 
+extern "C" int nalu_linsys_set_learned_base(nalu_linsys* ls, int precond) {
+  return guarded([&]() -> int {
+    NALU_REQUIRE(ls, NALU_ERR_ARG, "set_learned_base: null system");
+    NALU_REQUIRE(precond == NALU_PREC_SGS || precond == NALU_PREC_MUELU, NALU_ERR_ARG,
+                 "set_learned_base: the base is NALU_PREC_SGS or NALU_PREC_MUELU");
+    if (precond != ls->learned_base) {
+      ls->learned_base = precond;
+      ls->precond_ready = false;  // the next set-up builds (or drops) the hierarchy
+    }
+    return NALU_OK;
+  });
+}
+
 extern "C" int nalu_linsys_set_learned_precond(nalu_linsys* ls, nalu_gnn* g) {
   return guarded([&]() -> int {
     NALU_REQUIRE(ls, NALU_ERR_ARG, "set_learned_precond: null system");
@@ -1685,7 +1705,7 @@ extern "C" int nalu_gnn_loss_grad(nalu_gnn* g, nalu_linsys* ls, int batch, const
                        ls->perm.p, n, batch, w.R.p);
     hipLaunchKernelGGL((nalu::k_gather_perm<float, float>), dim3(nalu::nblocks(T)), dim3(nalu::kThreads), 0, s, e_dev,
                        ls->perm.p, n, batch, w.E.p);
-    for (int b = 0; b < batch; ++b) nalu::perm_sgs32(ls, w.R.p + b * n, w.ZS.p + b * n);
+    for (int b = 0; b < batch; ++b) nalu::base32(ls, w.R.p + b * n, w.ZS.p + b * n);
     const unsigned nb = std::min<unsigned>(nalu::kRedBlocks, nalu::nblocks(n));
     hipLaunchKernelGGL((nalu::k_bdot<float, float>), dim3(nb, batch), dim3(nalu::kThreads), 0, s, w.E.p, w.R.p, n,
                        w.dpart.p);
@@ -1802,7 +1822,7 @@ extern "C" int nalu_gnn_harvest(nalu_linsys* ls, int num_rhs, int pairs_per_rhs,
         hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, part.p, (int)nb, sc.p + 6, 1, 0);
         hipLaunchKernelGGL(k_pair, dim3(nblocks(n)), dim3(kThreads), 0, s, Rr.p + j * n, q.p, sc.p + 6, sc.p + 5, n,
                            d->R.p + (int64_t)pair * n, d->E.p + (int64_t)pair * n);
-        perm_sgs32(ls, d->R.p + (int64_t)pair * n, d->ZS.p + (int64_t)pair * n);
+        base32(ls, d->R.p + (int64_t)pair * n, d->ZS.p + (int64_t)pair * n);
         hipLaunchKernelGGL((k_bdot<float, float>), dim3(nb, 1), dim3(kThreads), 0, s, d->E.p + (int64_t)pair * n,
                            d->R.p + (int64_t)pair * n, n, part.p);
         hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, part.p, (int)nb, d->S.p + pair, 1, 0);

@@ -6,7 +6,7 @@
  *
  * It is the CPU oracle's arithmetic (oracle.c: the element kernels of the
  * ElemKernel.C files under src/kernel and ComputeMdotElemAlgorithm.C) compiled for speed
- * (-O3 -march=native -fopenmp, FP contraction allowed) and driven the way the
+ * (-O3 -march=x86-64-v3 -fopenmp, FP contraction allowed) and driven the way the
  * reference runs on a multicore host:
  *   - graph: the Tpetra-identical node graph, built per row in parallel from
  *     the node -> element incidence (rows sorted, columns sorted), numDof

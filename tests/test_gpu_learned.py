@@ -152,6 +152,82 @@ def test_untrained_network_is_sgs():
     assert rel(z.cpu().numpy(), zs[0].cpu().numpy()) <= 1e-6
 
 
+def amg_base_system(n, method="fgmres", prec="learned"):
+    P = lowmach_problem(n, rho_sigma=0.3, seed=4321)
+    mesh = na.Mesh(P["coords"], P["conn"])
+    ls = na.LinearSystem(1, na.SolverConfig(method, prec, tolerance=1e-8, max_iterations=2000, kspace=30,
+                                            precision="mixed"))
+    ls.set_muelu_params(na.MueLuParams(max_coarse_size=30, cheb_degree=2))
+    if prec == "learned":
+        ls.set_learned_base("muelu")
+    ls.buildElemToNodeGraph(mesh)
+    ls.finalizeLinearSystem()
+    ls.zeroSystem()
+    na.assemble_continuity(ls, mesh, dev(P["rho"]), dev(P["p"]), dev(P["u"]), dev(P["gpdx"]), tau=1.0,
+                           mode="gather")
+    ls.loadComplete()
+    ls.removeRhsMean()
+    ls.precond_setup()
+    return P, mesh, ls
+
+
+def test_untrained_network_is_the_amg_base():
+    """With the AMG base the network's feature is the FP32 V-cycle of r (within
+    FP32 rounding of the FP64 V-cycle of a muelu system), the untrained network
+    returns it, and a non-finite network falls back to the V-cycle."""
+    print("case: learned preconditioner on the AMG base, n=10", flush=True)
+    P, mesh, ls = amg_base_system(10)
+    _, _, lm = amg_base_system(10, prec="muelu")
+    n = ls.sizes()[0]
+    net = na.LearnedPreconditioner(seed=1)
+    r64 = np.random.default_rng(3).standard_normal(n)
+    r64 -= r64.mean()
+    r = dev(r64, torch.float32)
+    z = torch.zeros_like(r)
+    net.apply(ls, r, z)
+    zs = torch.zeros(1, n, dtype=torch.float32, device=DEV)
+    net.loss_grad(ls, r[None].contiguous(), r[None].contiguous(), zs=zs)
+    zamg = torch.zeros(n, dtype=torch.float64, device=DEV)
+    lm.precond_apply(dev(r.double().cpu().numpy()), zamg)
+    z_np, zs_np, za = z.cpu().numpy(), zs[0].cpu().numpy(), zamg.cpu().numpy()
+    print("  |net - feature| %.2e, |feature - FP64 V-cycle| %.2e" % (rel(z_np, zs_np), rel(zs_np, za)), flush=True)
+    assert rel(z_np, zs_np) <= 1e-6
+    assert rel(zs_np, za) <= 1e-4
+    th = net.get_params()
+    th[:] = np.nan
+    net.set_params(th)
+    ls.set_learned_preconditioner(net)
+    b = dev(poisson_rhs(n, 5))
+    x = torch.zeros(n, dtype=torch.float64, device=DEV)
+    st, it, rr = ls.solve_rhs(b, x)
+    print("  NaN network: status %d iters %d resid %.2e fallback %s" % (st, it, rr, ls.learned_fallback()),
+          flush=True)
+    assert st == 0 and rr <= 1e-8 and ls.learned_fallback() and it <= 60
+
+
+def test_training_on_the_amg_base():
+    """Pairs harvested with the AMG base store the V-cycle feature; a network
+    trained on them solves inside FGMRES without falling back."""
+    print("case: train on the AMG base, n=12", flush=True)
+    P, mesh, ls = amg_base_system(12)
+    n = ls.sizes()[0]
+    data = na.HarvestedPairs(ls, num_rhs=12, pairs_per_rhs=10, seed=11, tol=1e-10)
+    net = na.LearnedPreconditioner(seed=2)
+    hist = net.train(ls, data, epochs=6, batch=8, seed=1)
+    print("  epoch losses", np.round(hist, 4).tolist(), flush=True)
+    assert np.all(np.isfinite(hist)) and hist[-1] <= hist[0]
+    b = dev(poisson_rhs(n, 99))
+    ls.set_learned_preconditioner(net)
+    x = torch.zeros(n, dtype=torch.float64, device=DEV)
+    st1, it1, r1 = ls.solve_rhs(b, x)
+    _, _, lm = amg_base_system(12, prec="muelu")
+    x.zero_()
+    st0, it0, r0 = lm.solve_rhs(b, x)
+    print("  FGMRES iterations: AMG %d, learned on AMG %d (fallback %s)" % (it0, it1, ls.learned_fallback()),
+          flush=True)
+    assert st0 == 0 and st1 == 0 and r1 <= 1e-8 and not ls.learned_fallback()
+
+
 def test_harvest_pairs_are_consistent(oracle):
     print("case: harvest n=8", flush=True)
     _, _, ls = poisson_system(8, na.SolverConfig("cg", "sgs"))  # plain FP64 system, as the pairs are defined
