@@ -117,7 +117,7 @@ def cpu_baseline(n_cpu):
             "pressure_true_rel_resid": res}
 
 
-def train_network(args, na, torch, devc, Pt, tag="", base="sgs", muelu_params=None):
+def train_network(args, na, torch, devc, Pt, tag="", base="sgs", muelu_params=None, lr=3e-3):
     """Harvest 2000 (residual, error) pairs from plain FP64 CG + SGS on the
     pressure system of the training problem Pt (400 seeded Gaussian zero-mean
     right-hand sides x 5 pairs, CG to 1e-6: pairs are kept while the residual is
@@ -127,7 +127,7 @@ def train_network(args, na, torch, devc, Pt, tag="", base="sgs", muelu_params=No
     iterations (nalu_amd.training).  Returns (net, training record)."""
     from nalu_amd.training import train_pressure_preconditioner
     return train_pressure_preconditioner(
-        Pt, num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=args.train_epochs, batch=16, lr=5e-4, seed=1,
+        Pt, num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=args.train_epochs, batch=16, lr=lr, seed=1,
         net_seed=2, device=devc, log=lambda m: print("bench: %s%s" % (tag, m), file=sys.stderr, flush=True),
         base=base, muelu_params=muelu_params)
 
@@ -440,7 +440,8 @@ def stretched_compare(args, na, torch, devc):
     ratio_t = 1.05 ** (n / args.train_n)
     net, rec = train_network(args, na, torch, devc,
                              lowmach_problem(args.train_n, jitter=0.05, rho_sigma=0.3, seed=778, stretch=ratio_t,
-                                             max_aspect=100.0), tag="stretched ", base="muelu", muelu_params=mp)
+                                             max_aspect=100.0), tag="stretched ", base="muelu", muelu_params=mp,
+                             lr=5e-4)
     rec["stretch_ratio"] = ratio_t
     P = lowmach_problem(n, jitter=0.05, rho_sigma=0.3, seed=777, stretch=1.05, max_aspect=100.0)
     hx = np.diff(P["coords"][:n + 1, 0])  # the boundary line j = k = 0 carries the unjittered axis positions

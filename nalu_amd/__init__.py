@@ -735,7 +735,7 @@ class LearnedPreconditioner:
                                         C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))), "gnn_loss_grad")
         return loss.value, g
 
-    def train(self, ls, pairs, epochs=4, batch=16, lr=5e-4, seed=0, validate=None):
+    def train(self, ls, pairs, epochs=4, batch=16, lr=3e-3, seed=0, validate=None):
         """Adam on the harvested pairs; returns the mean loss of each epoch.
 
         validate: optional callable(net) -> score (lower is better), evaluated

@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_scalar_gather(Op op, const int32_t* __r
     for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
     if (r < nrows) rhs[r] += rr;
   } else {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] = seg[i];
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store(seg[i], &vals[s0 + i]);
     if (r < nrows) rhs[r] = rr;
   }
 }
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scalar_tile(Op op, const int32
     for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
     for (int i = threadIdx.x; i < r1 - r0; i += blockDim.x) rhs[r0 + i] += rseg[i];
   } else {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] = seg[i];
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store(seg[i], &vals[s0 + i]);
     for (int i = threadIdx.x; i < r1 - r0; i += blockDim.x) rhs[r0 + i] = rseg[i];
   }
 }
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kTileThreads) void k_momentum_tile(
     for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
     for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] += rseg[i];
   } else {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] = seg[i];
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store(seg[i], &vals[s0 + i]);
     for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] = rseg[i];
   }
 }
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(256) void k_momentum_seg_gather(
   if (accumulate) {
     for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
   } else {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] = seg[i];
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store(seg[i], &vals[s0 + i]);
   }
   if (r < nrows) {
 #pragma unroll

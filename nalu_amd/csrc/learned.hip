@@ -10,7 +10,9 @@
 //   encoder   h0 = W_in f + b_in                               3 -> 32
 //   layer l   m_i = sum_{j != i} w_ij h_j,  w_ij = -a_ij / a_ii (BF16)
 //             h' = h + relu(W_l [h ; m] + b_l)                 64 -> 32
-//   output    out = w_out . h + c0 f0 + c1 f1,  z = out s / dbar
+//   output    out = w_out . h + c0 f0 + c1 f1,  z = out s / dbar, minus its
+//             mean when every row of A sums to zero (pure Neumann: the loss
+//             below cannot see the constant part of the output)
 // Weights are BF16, hidden states are stored as BF16, every accumulation is
 // FP32; the 64 -> 32 layer transforms run on the MFMA units
 // (v_mfma_f32_32x32x16_bf16, one 32-node tile per wavefront).  With w_out = 0,
@@ -1059,7 +1061,11 @@ struct Learned {
   DevBuf<uint16_t> h0, h1;
   DevBuf<float> zs, r32, z32, scal;
   DevBuf<float2> F;
-  DevBuf<double> part, sumsq;
+  DevBuf<double> part, sumsq, zsum;
+  // every row of A sums to zero (pure-Neumann pressure system): constants are
+  // A's null space, and the network output is projected off them
+  bool neumann = false;
+  DevBuf<int> rs_flag;
   bool fallback = false;
 };
 
@@ -1315,6 +1321,26 @@ void build_tiles(nalu_linsys* ls, Learned* st) {
   st->tiles_n = n;
 }
 
+// flag = 1 when some row sum of A = D + L + U exceeds 1e-5 of its diagonal
+// (a pure-Neumann operator assembled in FP32 arithmetic sums to zero only to
+// that precision)
+__global__ void k_rowsum_check(const int32_t* __restrict__ lrp, const double* __restrict__ lv,
+                               const int32_t* __restrict__ urp, const double* __restrict__ uv,
+                               const double* __restrict__ dv, int64_t n, int* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double t = dv[i];
+  for (int32_t q = lrp[i]; q < lrp[i + 1]; ++q) t += lv[q];
+  for (int32_t q = urp[i]; q < urp[i + 1]; ++q) t += uv[q];
+  if (!(std::fabs(t) <= 1e-5 * std::fabs(dv[i]))) flag[0] = 1;
+}
+
+// z -= mean(z) (sum[0] = sum of z)
+__global__ void k_sub_mean(float* __restrict__ z, int64_t n, const double* __restrict__ sum) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) z[k] -= (float)(sum[0] / (double)n);
+}
+
 // diagonal sum and delta_i = dbar / a_ii of the current matrix (solver order)
 void refresh_diag(nalu_linsys* ls, Learned* st) {
   hipStream_t s = ls->stream;
@@ -1326,6 +1352,16 @@ void refresh_diag(nalu_linsys* ls, Learned* st) {
   hipLaunchKernelGGL(k_bsum<double>, dim3(nb, 1), dim3(kThreads), 0, s, ls->dval.p, n, st->part.p);
   hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, st->part.p, (int)nb, st->dsum.p, 1, 0);
   hipLaunchKernelGGL(k_graph_delta, dim3(nblocks(n)), dim3(kThreads), 0, s, ls->dval.p, n, st->dsum.p, st->delta.p);
+  // pure-Neumann test of the current matrix
+  if (st->rs_flag.n < 1) st->rs_flag.alloc(1);
+  if (st->zsum.n < 1) st->zsum.alloc(1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, st->rs_flag.p, (int64_t)1, 0);
+  hipLaunchKernelGGL(k_rowsum_check, dim3(nblocks(n)), dim3(kThreads), 0, s, ls->lrp.p, ls->lval.p, ls->urp.p,
+                     ls->uval.p, ls->dval.p, n, st->rs_flag.p);
+  int h = 1;
+  NALU_HIP_CHECK(hipMemcpyAsync(&h, st->rs_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  NALU_HIP_CHECK(hipStreamSynchronize(s));
+  st->neumann = h == 0;
 }
 
 // Inference state for the current matrix: tiles (built once per system), message
@@ -1380,6 +1416,13 @@ void apply_perm(nalu_linsys* ls, Learned* st, const nalu_gnn* g, const float* r,
                        st->tfoot.p, st->tedge.p, hin, hout, g->wbf.p + lay.w(l), g->theta.p + lay.b(l),
                        g->wbf.p + lay.w_out(), st->F.p, st->scal.p, trow, z, enc);
     std::swap(hin, hout);
+  }
+  if (st->neumann) {
+    // the training loss cannot see the output's constant part (A 1 = 0 and the
+    // residuals are zero-mean), so it is left undetermined; drop it
+    hipLaunchKernelGGL(k_bsum<float>, dim3(nb, 1), dim3(kThreads), 0, s, (const float*)z, n, st->part.p);
+    hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, st->part.p, (int)nb, st->zsum.p, 1, 0);
+    hipLaunchKernelGGL(k_sub_mean, dim3(nblocks(n)), dim3(kThreads), 0, s, z, n, (const double*)st->zsum.p);
   }
 }
 

@@ -33,7 +33,7 @@ def _continuity_system(mesh, fields, cfg, dev, base="sgs", muelu_params=None):
 
 
 def train_pressure_preconditioner(problem, num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=10, batch=16,
-                                  lr=5e-4, seed=1, net_seed=2, layers=4, hidden=32, validate_max_iterations=600,
+                                  lr=3e-3, seed=1, net_seed=2, layers=4, hidden=32, validate_max_iterations=600,
                                   device="cuda:0", log=None, base="sgs", muelu_params=None):
     """problem: dict with coords, conn, rho, p, u, gpdx (a lowmach_problem).
     base: the network's base preconditioner, "sgs" or "muelu" (the AMG V-cycle

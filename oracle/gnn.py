@@ -82,6 +82,10 @@ class Graph:
         self.d = d.astype(np.float32).astype(np.float64)
         self.dbar = float(np.mean(d))
         self.delta = (self.dbar / d).astype(np.float32).astype(np.float64)
+        # every row sums to zero (pure-Neumann system): constants span A's null space
+        rs = np.zeros(n)
+        np.add.at(rs, rows, vals)
+        self.neumann = bool(np.all(np.abs(rs) <= 1e-5 * np.abs(d)))  # FP32-assembled sums vanish to ~1e-7
 
     def aop(self, x):
         """A x = D (I - W) x for x [B, n]."""
@@ -145,9 +149,13 @@ def forward(th, G, r, zs, num_layers=4, rnd=True):
 
 
 def apply(th, G, r, zs, num_layers=4, rnd=True):
-    """Preconditioner output z = out s / dbar."""
+    """Preconditioner output z = out s / dbar, projected off the constants when
+    the system is pure Neumann (the loss does not determine that component)."""
     out, c = forward(th, G, r, zs, num_layers, rnd)
-    return out * c["s"] / G.dbar
+    z = out * c["s"] / G.dbar
+    if G.neumann:
+        z = z - z.mean(axis=-1, keepdims=True)
+    return z
 
 
 def loss(out, G, r, e):

@@ -59,6 +59,28 @@ def test_zero_correction_is_sgs(system, rnd):
     th = G.init_params(4, seed=5)
     r, zs, _ = _inputs(gr)
     z = G.apply(th, gr, r, zs, rnd=rnd)
+    # the pressure system is pure Neumann: the output is projected off the constants
+    assert gr.neumann
+    want = zs - zs.mean(axis=-1, keepdims=True)
+    assert np.allclose(z, want, rtol=1e-12, atol=1e-12 * np.abs(zs).max())
+
+
+def test_no_projection_without_a_null_space(system):
+    """Only a pure-Neumann system (every row summing to zero) has its output
+    projected: with the diagonal raised the zero-correction network is the
+    base feature exactly."""
+    g, vals, _ = system
+    v2 = vals.copy()
+    rp, ci = g.row_ptr, g.col_idx
+    for i in range(len(rp) - 1):
+        for k in range(rp[i], rp[i + 1]):
+            if ci[k] == i:
+                v2[k] *= 1.5
+    gr2 = G.Graph(rp, ci, v2)
+    assert not gr2.neumann
+    th = G.init_params(4, seed=5)
+    r, zs, _ = _inputs(gr2)
+    z = G.apply(th, gr2, r, zs, rnd=False)
     assert np.allclose(z, zs, rtol=1e-12, atol=1e-12 * np.abs(zs).max())
 
 
@@ -82,6 +104,7 @@ def test_one_layer_neighbour_average(system):
             j = ci[k]
             if j != i:
                 want[i] += (-vals[k] / d[i]) * r[0, j] / d[j]
+    want -= want.mean()  # pure-Neumann system: projected off the constants
     assert np.linalg.norm(z - want) <= 1e-6 * np.linalg.norm(want)
 
 

@@ -337,7 +337,7 @@ def test_config5_stretched_256(oracle):
     mp = na.MueLuParams(cheb_degree=2)
     net, rec = train_pressure_preconditioner(
         lowmach_problem(n_t, jitter=0.05, rho_sigma=0.3, seed=778, stretch=ratio_t, max_aspect=100.0),
-        num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=10, log=say, base="muelu", muelu_params=mp)
+        num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=10, lr=5e-4, log=say, base="muelu", muelu_params=mp)
     ls = na.LinearSystem(1, cfg("learned"))
     ls.set_muelu_params(mp)
     ls.set_learned_base("muelu")
