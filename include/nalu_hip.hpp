@@ -202,6 +202,8 @@ class LinearSystem {
   }
   // MueLu parameter list (muelu_xml_file_name, src/LinearSolverConfig.C:86-100) as a struct
   void setMueLuParams(const nalu_muelu_params& p) { check(nalu_linsys_set_muelu_params(ls_, &p), "setMueLuParams"); }
+  // base of the learned preconditioner: NALU_PREC_SGS or NALU_PREC_MUELU (no reference counterpart)
+  void setLearnedBase(int precond) { check(nalu_linsys_set_learned_base(ls_, precond), "setLearnedBase"); }
 
   // ---- accessors (include/LinearSystem.h:128-141)
   unsigned numDof() const { return (unsigned)numDof_; }
@@ -361,6 +363,17 @@ inline void computeMdot(const Mesh& m, const double* rhoDev, const double* pDev,
   auto fn = fp32Arithmetic ? nalu_compute_mdot_f32 : nalu_compute_mdot;
   check(fn(m.get(), rhoDev, pDev, uDev, gpdxDev, projTimeScale, shiftMdot, shiftPoisson, mdotDev, nullptr),
         "ComputeMdotElemAlgorithm");
+}
+
+// ContinuityAdvElemKernel and ComputeMdotElemAlgorithm from one pass over the
+// scs geometry (row gather): the continuity system of `linsys` and mdot [E][12].
+inline void assembleContinuityAndMdot(LinearSystem& linsys, const Mesh& m, const double* rhoDev, const double* pDev,
+                                      const double* uDev, const double* gpdxDev, double projTimeScale,
+                                      double* mdotDev, bool shiftMdot = false, bool shiftPoisson = false,
+                                      bool reducedSensitivities = false) {
+  check(nalu_assemble_continuity_mdot(linsys.get(), m.get(), rhoDev, pDev, uDev, gpdxDev, projTimeScale, shiftMdot,
+                                      shiftPoisson, reducedSensitivities, mdotDev),
+        "ContinuityAdvElemKernel + ComputeMdotElemAlgorithm");
 }
 
 }  // namespace nalu_hip

@@ -229,18 +229,22 @@ __global__ __launch_bounds__(256) void k_scalar_gather(Op op, const int32_t* __r
                                                        const uint64_t* __restrict__ elem_pos, int64_t nrows,
                                                        double* __restrict__ vals, double* __restrict__ rhs,
                                                        int accumulate) {
-  extern __shared__ double seg[];
+  // the row segments accumulate in FP64 (FP32 LDS accumulation for the mixed
+  // path halved the LDS per row but made the pass slower)
+  using A = double;
+  extern __shared__ __align__(16) unsigned char seg_raw[];
+  A* seg = reinterpret_cast<A*>(seg_raw);
   const int64_t r0 = (int64_t)blockIdx.x * blockDim.x;
   const int64_t rend = min(r0 + (int64_t)blockDim.x, nrows);
   const int64_t r = r0 + threadIdx.x;
   const int32_t s0 = nrp[r0], s1 = nrp[rend];
   const int nseg = s1 - s0;
-  for (int i = threadIdx.x; i < nseg; i += blockDim.x) seg[i] = 0.0;
+  for (int i = threadIdx.x; i < nseg; i += blockDim.x) seg[i] = A(0);
   __syncthreads();
   double rr = 0.0;
   if (r < nrows) {
     const int32_t a = row_node ? row_node[r] : (int32_t)r;
-    double* myrow = seg + (nrp[r] - s0);
+    A* myrow = seg + (nrp[r] - s0);
     const int32_t q1 = inc_ptr[a + 1];
     for (int32_t q = inc_ptr[a]; q < q1; ++q) {
       const int32_t t = inc_elem[q];
@@ -252,15 +256,15 @@ __global__ __launch_bounds__(256) void k_scalar_gather(Op op, const int32_t* __r
       load_coords(coords, nd, x);
       const uint64_t pk = elem_pos[t];
       rr += (double)dispatch_row_e(op, la, nd, x, e,
-                                   [&](int c, auto v) { myrow[(int)((pk >> (8 * c)) & 0xff)] += (double)v; });
+                                   [&](int c, auto v) { myrow[(int)((pk >> (8 * c)) & 0xff)] += (A)v; });
     }
   }
   __syncthreads();
   if (accumulate) {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += (double)seg[i];
     if (r < nrows) rhs[r] += rr;
   } else {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store(seg[i], &vals[s0 + i]);
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store((double)seg[i], &vals[s0 + i]);
     if (r < nrows) rhs[r] = rr;
   }
 }
@@ -394,11 +398,11 @@ __global__ __launch_bounds__(kTileThreads) void k_momentum_tile(
   __syncthreads();
   const int nr = (int)(3 * (a1 - a0));
   if (accumulate) {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] += rseg[i];
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += (double)seg[i];
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] += (double)rseg[i];
   } else {
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store(seg[i], &vals[s0 + i]);
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] = rseg[i];
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store((double)seg[i], &vals[s0 + i]);
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] = (double)rseg[i];
   }
 }
 
@@ -552,6 +556,10 @@ __global__ void k_dirichlet(const int32_t* __restrict__ nodes, int64_t nbc, int6
   rhs[row] = reset_mode ? rhs_residual : bcv[(int64_t)nd * node + d] - sol[(int64_t)nd * node + d];
 }
 
+// rows (threads) per block of the row gather (128^3 continuity: 1.31 ms at
+// 256, 1.34 ms at 64, 1.40 ms at 128)
+int gather_rows() { return 256; }
+
 template <class Op>
 void launch_scalar(nalu_linsys* ls, const nalu_mesh* m, const Op& op, int mode) {
   hipStream_t s = ls->stream;
@@ -561,9 +569,10 @@ void launch_scalar(nalu_linsys* ls, const nalu_mesh* m, const Op& op, int mode) 
     const int accumulate = ls->values_pending_zero ? 0 : 1;
     NALU_REQUIRE(ls->values_pending_zero == ls->rhs_pending_zero, NALU_ERR_STATE,
                  "gather assembly: values/rhs zero state diverged");
-    int B = 256;
-    while (B > 32 && (size_t)B * ls->max_node_row_len * sizeof(double) > 64 * 1024) B >>= 1;
-    const size_t lds = (size_t)B * ls->max_node_row_len * sizeof(double);
+    const size_t esz = sizeof(double);
+    int B = gather_rows();
+    while (B > 32 && (size_t)B * ls->max_node_row_len * esz > 64 * 1024) B >>= 1;
+    const size_t lds = (size_t)B * ls->max_node_row_len * esz;
     hipLaunchKernelGGL((k_scalar_gather<Op>), dim3(blocks_for(ls->nrows, B)), dim3(B), lds, s, op, m->conn.p,
                        m->coords.p, (const int32_t*)ls->row_node.p, m->inc_ptr.p, m->inc_elem.p, ls->nrp.p,
                        ls->elem_pos.p, ls->nrows, ls->values.p, ls->rhs.p, accumulate);
