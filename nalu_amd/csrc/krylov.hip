@@ -24,6 +24,11 @@
 
 #include "internal.hpp"
 
+// unroll depth of the SELL entry loops (loads in flight per lane)
+#ifndef NALU_SPARSE_UNROLL
+#define NALU_SPARSE_UNROLL 4
+#endif
+
 namespace nalu {
 namespace {
 
@@ -760,12 +765,12 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv(const SellChunk* __restr
   const bool act = lane < k.nrows;
   const int row = k.row0 + (act ? lane : 0);
   T s = dv[row] * x[row];
-#pragma unroll 4
+#pragma unroll NALU_SPARSE_UNROLL
   for (int q = 0; q < k.lw; ++q) {
     s += __builtin_nontemporal_load(&lval[k.loff + lane + q * 64]) *
          x[__builtin_nontemporal_load(&lcs[k.loff + lane + q * 64])];
   }
-#pragma unroll 4
+#pragma unroll NALU_SPARSE_UNROLL
   for (int q = 0; q < k.uw; ++q) {
     s += __builtin_nontemporal_load(&uval[k.uoff + lane + q * 64]) *
          x[__builtin_nontemporal_load(&ucs[k.uoff + lane + q * 64])];
@@ -797,7 +802,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_fwd(const SellChunk* __restri
   const bool act = ok && lane < k.nrows;
   T s = T(0);
   if (act) {
-#pragma unroll 4
+#pragma unroll NALU_SPARSE_UNROLL
     for (int q = half; q < k.lw; q += 2)
       s += __builtin_nontemporal_load(&lval[k.loff + lane + q * 64]) *
            z[__builtin_nontemporal_load(&lcs[k.loff + lane + q * 64])];
@@ -828,7 +833,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_bwd(const SellChunk* __restri
   const bool act = ok && lane < k.nrows;
   T s = T(0);
   if (act) {
-#pragma unroll 4
+#pragma unroll NALU_SPARSE_UNROLL
     for (int q = half; q < k.uw; q += 2)
       s += __builtin_nontemporal_load(&uval[k.uoff + lane + q * 64]) *
            z[__builtin_nontemporal_load(&ucs[k.uoff + lane + q * 64])];
@@ -858,12 +863,12 @@ __global__ __launch_bounds__(kThreads) void k_sell_spmv_dot(const SellChunk* __r
     const bool act = lane < k.nrows;
     const int row = k.row0 + (act ? lane : 0);
     T s = dv[row] * x[row];
-#pragma unroll 4
+#pragma unroll NALU_SPARSE_UNROLL
     for (int q = 0; q < k.lw; ++q) {
       s += __builtin_nontemporal_load(&lval[k.loff + lane + q * 64]) *
            x[__builtin_nontemporal_load(&lcs[k.loff + lane + q * 64])];
     }
-#pragma unroll 4
+#pragma unroll NALU_SPARSE_UNROLL
     for (int q = 0; q < k.uw; ++q) {
       s += __builtin_nontemporal_load(&uval[k.uoff + lane + q * 64]) *
            x[__builtin_nontemporal_load(&ucs[k.uoff + lane + q * 64])];
@@ -902,7 +907,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_sweep_dot(const SellChunk* __
   if (act) {
     const int w = UPPER ? k.uw : k.lw;
     const int64_t o = UPPER ? k.uoff : k.loff;
-#pragma unroll 4
+#pragma unroll NALU_SPARSE_UNROLL
     for (int q = half; q < w; q += 2)
       s += __builtin_nontemporal_load(&val[o + lane + q * 64]) * z[__builtin_nontemporal_load(&cs[o + lane + q * 64])];
   }
