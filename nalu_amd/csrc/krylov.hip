@@ -694,8 +694,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_color_vec(const int32_t* __res
 }
 
 // ------------------------------------------------------------ SELL-64 kernels
-// Column table of one part (L or U) of the SELL copy: 8-bit offsets from a
-// per-slot base, with the int32 columns as the escape table.
+// Each part (L or U) of the SELL copy stores, per chunk, int32 columns and
+// values slot-major: entry q of the chunk's 64 rows at off + q * 64 + lane.
 
 // One wavefront per chunk, one lane per row: every load of column indices and
 // values is a fully coalesced 64-lane access, no reductions or LDS.
