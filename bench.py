@@ -82,6 +82,23 @@ def algorithmic_bytes(N, E, P, rows, nnz1, nnz3):
     }
 
 
+# Algorithmic FP64 flops per unit of work (the same accounting as the byte
+# counts: geometry ~5.4k flops per element, plus the kernel's own arithmetic;
+# 2 flops per stored entry for a product, every off-diagonal entry once plus the
+# diagonal scaling for one SGS application)
+FP64_PEAK_TFLOPS = 78.6
+
+
+def algorithmic_flops(E, rows, nnz1):
+    return {
+        "mdot": E * (5400 + 3000),
+        "continuity_assembly": E * (5400 + 3000),
+        "momentum_assembly": E * (5400 + 13000),
+        "spmv": 2 * nnz1,
+        "sgs_apply": 2 * (nnz1 - rows) + 4 * rows,
+    }
+
+
 def cpu_baseline(n_cpu):
     """The oracle's arithmetic compiled -O3 -march=x86-64-v3 -fopenmp (oracle/cpu_baseline.c),
     on the n_cpu^3 low-Mach case: assembly (zeroSystem + mdot + continuity +
@@ -628,6 +645,7 @@ def main():
     ncol = len(cp) - 1
 
     byt = algorithmic_bytes(N, E, Pn, rows, nnz1, nnz3)
+    flp = algorithmic_flops(E, rows, nnz1)
     if fused:  # the continuity pass also writes mdot [E][12]
         byt["continuity_assembly"] += 12 * E * 8
     per_step = {
@@ -646,6 +664,9 @@ def main():
     kern_name = {"mdot": "k_mdot_elem", "continuity_assembly": kname[cmode][0] + (" (+ mdot)" if fused else ""),
                  "momentum_assembly": kname[mmode][1], "sgs_apply": "k_sell_fwd + k_sell_bwd",
                  "spmv": "k_sell_spmv"}
+    def unit_t(k):
+        return {"sgs_apply": t_sgs, "spmv": t_spmv}.get(k, per_step[k][0])
+
     dom = max(per_step, key=lambda k: per_step[k][0])
     unit_time = {"mdot": med["mdot"], "continuity_assembly": med["continuity_assembly"],
                  "momentum_assembly": med["momentum_assembly"], "sgs_apply": t_sgs, "spmv": t_spmv}[dom]
@@ -708,9 +729,13 @@ def main():
         "mixed_precision": mixed,
         "stretched_pressure_compare": stretched,
         "kernels": {
-            k: {"time_s": per_step[k][0], "unit_time_s": {"sgs_apply": t_sgs, "spmv": t_spmv}.get(k, per_step[k][0]),
+            k: {"time_s": per_step[k][0], "unit_time_s": unit_t(k),
                 "algorithmic_bytes": byt[k],
-                "GBps": byt[k] / {"sgs_apply": t_sgs, "spmv": t_spmv}.get(k, per_step[k][0]) / 1e9}
+                "GBps": byt[k] / unit_t(k) / 1e9,
+                "hbm_frac": byt[k] / unit_t(k) / 1e9 / HBM_PEAK_GBS,
+                "algorithmic_fp64_flops": flp[k],
+                "fp64_TFLOPs": flp[k] / unit_t(k) / 1e12,
+                "fp64_frac": flp[k] / unit_t(k) / 1e12 / FP64_PEAK_TFLOPS}
             for k in per_step},
         "roofline": {"bound": "hbm", "kernel": kern_name[dom], "unit_of_work": dom,
                      "launches_per_unit": launches,
