@@ -76,6 +76,28 @@ struct DevBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Sub-buffers carved from one device workspace at 256-byte boundaries.  To size
+// a workspace, carve the sequence from a null base first (off is then the byte
+// count), then carve the same sequence from the real base.
+struct Carve {
+  unsigned char* base = nullptr;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t count) {
+    off = (off + 255) / 256 * 256;
+    T* q = reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(base) + off);
+    off += count * sizeof(T);
+    return q;
+  }
+};
+
+// Grows a byte workspace owned by a handle to at least `bytes` (never shrinks;
+// growing frees the old buffer, which synchronizes the device).
+inline unsigned char* ensure_ws(DevBuf<unsigned char>& ws, size_t bytes) {
+  if (ws.n < bytes) ws.alloc(bytes);
+  return ws.p;
+}
+
 // Page-locked host buffer (released with its owner).
 struct HostPinned {
   void* p = nullptr;
@@ -196,6 +218,13 @@ struct nalu_linsys {
   // across solves, and its page-locked host mirror of the state
   nalu::DevBuf<unsigned char> cg_work;
   nalu::HostPinned cg_host;
+  // workspaces kept across solves: the permuted outer vectors of a solve
+  // (right-hand side, solution, residual, refinement correction), and the
+  // inner solver's (Krylov basis, residual, Hessenberg buffers; or the
+  // vectors of the kernel-per-step CG) with its page-locked host buffer
+  nalu::DevBuf<unsigned char> solve_ws;
+  nalu::DevBuf<unsigned char> krylov_ws;
+  nalu::HostPinned krylov_host;
   // layout of the vectors of a stacked multi-rhs solve while it runs: false =
   // [nrhs][rows] slices, true = interleaved [rows][nrhs] (batched kernels)
   bool stack_interleaved = false;

@@ -1398,14 +1398,6 @@ struct Hessenberg {
   }
 };
 
-struct PinnedDoubles {
-  double* p = nullptr;
-  explicit PinnedDoubles(size_t n) { NALU_HIP_CHECK(hipHostMalloc((void**)&p, sizeof(double) * n, hipHostMallocDefault)); }
-  ~PinnedDoubles() {
-    if (p) (void)hipHostFree(p);
-  }
-};
-
 // Restarted (F)GMRES on A x = b in the permuted space (all vectors permuted).
 // TV is the storage/arithmetic type of the Krylov basis, SpMV and
 // preconditioner (double, or float in the mixed-precision inner solve); inner
@@ -1420,8 +1412,20 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
   // basis columns padded to a multiple of 64 entries (16-byte aligned columns)
   const int64_t ldv = (n + 63) / 64 * 64;
   const size_t need = (size_t)ldv * ((m + 1) + (flexible ? m : 1) + 2);
-  DevBuf<TV> ws(need);
-  TV* V = ws.p;
+  // basis, FP64 residual, Hessenberg column buffers: the handle's workspace
+  TV* V = nullptr;
+  double *r64 = nullptr, *hdev = nullptr, *hbuf = nullptr;
+  auto layout = [&](unsigned char* base) {
+    Carve c{base};
+    V = c.take<TV>(need);
+    r64 = c.take<double>(n);
+    hdev = c.take<double>(m + 2);
+    hbuf = c.take<double>(2 * (m + 2) + 2);
+    return c.off;
+  };
+  layout(ensure_ws(ls->krylov_ws, layout(nullptr)));
+  ls->krylov_host.alloc(sizeof(double) * (2 * (m + 2) + 2));
+  double* hpin = static_cast<double*>(ls->krylov_host.p);
   TV* Z = V + (size_t)ldv * (m + 1);
   TV* tmp = Z + (size_t)ldv * (flexible ? m : 1);
   TV* xt = tmp + ldv;  // x in TV precision for the residual of the inner SpMV path
@@ -1438,26 +1442,23 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
   const size_t lds_dots = (size_t)R_dots * sizeof(TV) + (size_t)(m + 2) * 8;
   const unsigned nb_upd = red_blocks((n + 3) / 4);
   NALU_REQUIRE((size_t)(m + 2) * kGsBlocks <= ls->red.n, NALU_ERR_STATE, "gmres: reduction workspace too small");
-  DevBuf<double> r64(n), hdev(m + 2);
   // hA: pass-1 coefficients (+ ||w||^2 at k+1), hB: pass-2 coefficients, nrm: ||w_new||^2
-  DevBuf<double> hbuf(2 * (m + 2) + 2);
-  double* hA = hbuf.p;
-  double* hB = hbuf.p + (m + 2);
-  double* nrm = hbuf.p + 2 * (m + 2);
+  double* hA = hbuf;
+  double* hB = hbuf + (m + 2);
+  double* nrm = hbuf + 2 * (m + 2);
   // learned preconditioner: FGMRES falls back to SGS for the rest of the solve
   // when a cycle stagnates or the network output is not finite
   const bool watch = flexible && ls->cfg.precond == NALU_PREC_LEARNED;
   Hessenberg H(m);
   std::vector<double> hcol(m + 2), y;
-  PinnedDoubles hpin(2 * (m + 2) + 2);
   (void)xt;
 
   int iters = 0;
   double r0 = -1.0, rel = 1.0;
   while (iters < maxit) {
     // r = b - A x in FP64, v0 = r / ||r||
-    spmv_m<double, double>(ls, nrhs, x, r64.p, b, 1);
-    const double beta = std::sqrt(red.dot(r64.p, r64.p, n));
+    spmv_m<double, double>(ls, nrhs, x, r64, b, 1);
+    const double beta = std::sqrt(red.dot(r64, r64, n));
     if (r0 < 0) r0 = beta;
     if (r0 == 0.0 || beta <= tol * r0) {
       rel = r0 > 0 ? beta / r0 : 0.0;
@@ -1465,7 +1466,7 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
     }
     NALU_REQUIRE(std::isfinite(beta), NALU_ERR_NUMERIC, "gmres: non-finite residual");
     hipLaunchKernelGGL((k_fill<TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, V, n, TV(0));
-    hipLaunchKernelGGL((k_axpy<double, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, 1.0 / beta, r64.p, V, n);
+    hipLaunchKernelGGL((k_axpy<double, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, 1.0 / beta, r64, V, n);
     H.reset(beta);
     int k = 0;
     bool done = false;
@@ -1499,20 +1500,20 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
                            ls->red.p);
         hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kThreads), 0, s, ls->red.p, (int)nb_upd, nrm, 0);
       }
-      NALU_HIP_CHECK(hipMemcpyAsync(hpin.p, hbuf.p, hbytes, hipMemcpyDeviceToHost, s));
+      NALU_HIP_CHECK(hipMemcpyAsync(hpin, hbuf, hbytes, hipMemcpyDeviceToHost, s));
       NALU_HIP_CHECK(hipStreamSynchronize(s));
       bool two = always2;
       if (!always2) {
-        const double w2 = hpin.p[cnt], n2 = hpin.p[2 * (m + 2)];
+        const double w2 = hpin[cnt], n2 = hpin[2 * (m + 2)];
         if (!(n2 >= 0.01 * w2)) {  // severe cancellation (or NaN): reorthogonalize once
           second_pass();
-          NALU_HIP_CHECK(hipMemcpyAsync(hpin.p, hbuf.p, hbytes, hipMemcpyDeviceToHost, s));
+          NALU_HIP_CHECK(hipMemcpyAsync(hpin, hbuf, hbytes, hipMemcpyDeviceToHost, s));
           NALU_HIP_CHECK(hipStreamSynchronize(s));
           two = true;
         }
       }
-      for (int i = 0; i <= k; ++i) hcol[i] = hpin.p[i] + (two ? hpin.p[(m + 2) + i] : 0.0);
-      hcol[k + 1] = std::sqrt(hpin.p[2 * (m + 2)]);
+      for (int i = 0; i <= k; ++i) hcol[i] = hpin[i] + (two ? hpin[(m + 2) + i] : 0.0);
+      hcol[k + 1] = std::sqrt(hpin[2 * (m + 2)]);
       hipLaunchKernelGGL(k_scale_invnorm<TV>, dim3(nblocks(n)), dim3(kThreads), 0, s, vk1, n, nrm);
       if (learned_cycle && !std::isfinite(hcol[k + 1])) {
         learned_set_fallback(ls, true);  // drop this direction, finish the cycle with k columns
@@ -1529,14 +1530,14 @@ int gmres_t(nalu_linsys* ls, const double* b, double* x, int m, int maxit, doubl
     }
     // x += M^-1 V y (right preconditioning) or Z y (flexible)
     H.solve(k, y);
-    for (int i = 0; i < k; ++i) hpin.p[i] = y[i];
-    NALU_HIP_CHECK(hipMemcpyAsync(hdev.p, hpin.p, sizeof(double) * k, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < k; ++i) hpin[i] = y[i];
+    NALU_HIP_CHECK(hipMemcpyAsync(hdev, hpin, sizeof(double) * k, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((k_fill<TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, tmp, n, TV(0));
     if (flexible) {
-      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, Z, ldv, k, hdev.p, tmp, n, 1.0);
+      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, Z, ldv, k, hdev, tmp, n, 1.0);
       hipLaunchKernelGGL((k_axpy<TV, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, 1.0, tmp, x, n);
     } else {
-      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, V, ldv, k, hdev.p, tmp, n, 1.0);
+      hipLaunchKernelGGL((k_maxpy<TV, TV>), dim3(nblocks(n)), dim3(kThreads), 0, s, V, ldv, k, hdev, tmp, n, 1.0);
       prec_m<TV>(ls, nrhs, tmp, Z);
       hipLaunchKernelGGL((k_axpy<TV, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, 1.0, Z, x, n);
     }
@@ -1651,34 +1652,45 @@ int cg(nalu_linsys* ls, const double* b, double* x, int maxit, double tol, int n
   hipStream_t s = ls->stream;
   const int64_t n = ls->nrows * nrhs;  // nrhs > 1: the block-diagonal stacked system
   Red red(ls);
-  DevBuf<double> r(n), z(n), p(n), q(n), sc(8);
-  spmv_m<double, double>(ls, nrhs, x, r.p, b, 1);
-  const double r0 = std::sqrt(red.dot(r.p, r.p, n));
+  double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *sc = nullptr;  // the handle's workspace
+  auto layout = [&](unsigned char* base) {
+    Carve c{base};
+    r = c.take<double>(n);
+    z = c.take<double>(n);
+    p = c.take<double>(n);
+    q = c.take<double>(n);
+    sc = c.take<double>(8);
+    return c.off;
+  };
+  layout(ensure_ws(ls->krylov_ws, layout(nullptr)));
+  ls->krylov_host.alloc(sizeof(double));
+  double* pin = static_cast<double*>(ls->krylov_host.p);
+  spmv_m<double, double>(ls, nrhs, x, r, b, 1);
+  const double r0 = std::sqrt(red.dot(r, r, n));
   if (r0 == 0.0) return 0;
   NALU_REQUIRE(std::isfinite(r0), NALU_ERR_NUMERIC, "cg: non-finite residual");
-  prec_m<double>(ls, nrhs, r.p, z.p);
-  red.dot_dev(r.p, z.p, n, sc.p + 0);  // rz (slot alternates 0/3)
-  hipLaunchKernelGGL((k_copy<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, z.p, p.p, n);
-  PinnedDoubles pin(1);
+  prec_m<double>(ls, nrhs, r, z);
+  red.dot_dev(r, z, n, sc + 0);  // rz (slot alternates 0/3)
+  hipLaunchKernelGGL((k_copy<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, z, p, n);
   int it = 0;
   int cur = 0;
   while (it < maxit) {
     ++it;
-    spmv_m<double, double>(ls, nrhs, p.p, q.p, (const double*)nullptr, 0);
-    red.dot_dev(p.p, q.p, n, sc.p + 1);
+    spmv_m<double, double>(ls, nrhs, p, q, (const double*)nullptr, 0);
+    red.dot_dev(p, q, n, sc + 1);
     const unsigned nb = red_blocks(n);
-    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, x, r.p, p.p, q.p, sc.p + cur, sc.p + 1, n,
+    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, x, r, p, q, sc + cur, sc + 1, n,
                        ls->red.p);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kThreads), 0, s, ls->red.p, (int)nb, sc.p + 2, 0);
-    NALU_HIP_CHECK(hipMemcpyAsync(pin.p, sc.p + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kThreads), 0, s, ls->red.p, (int)nb, sc + 2, 0);
+    NALU_HIP_CHECK(hipMemcpyAsync(pin, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
     NALU_HIP_CHECK(hipStreamSynchronize(s));
-    const double rn = std::sqrt(*pin.p);
+    const double rn = std::sqrt(*pin);
     NALU_REQUIRE(std::isfinite(rn), NALU_ERR_NUMERIC, "cg: non-finite residual");
     if (rn <= tol * r0) break;
-    prec_m<double>(ls, nrhs, r.p, z.p);
+    prec_m<double>(ls, nrhs, r, z);
     const int nxt = cur == 0 ? 3 : 0;
-    red.dot_dev(r.p, z.p, n, sc.p + nxt);
-    hipLaunchKernelGGL(k_cg_p, dim3(nblocks(n)), dim3(kThreads), 0, s, p.p, z.p, sc.p + nxt, sc.p + cur, n);
+    red.dot_dev(r, z, n, sc + nxt);
+    hipLaunchKernelGGL(k_cg_p, dim3(nblocks(n)), dim3(kThreads), 0, s, p, z, sc + nxt, sc + cur, n);
     cur = nxt;
   }
   return it;
@@ -1810,8 +1822,22 @@ int krylov_solve_k(nalu_linsys* ls, int nrhs, const double* b, double* x, int* i
   const int64_t n = n1 * nrhs;
   hipStream_t s = ls->stream;
   Red red(ls);
-  DevBuf<double> bp(n), xp(n), r(n);
   const bool inter = stacked_interleaved(ls, nrhs);
+  // permuted right-hand side, solution and residual, the refinement correction
+  // (mixed) and the per-component slices (interleaved stacks): the handle's workspace
+  struct Vec {
+    double* p = nullptr;
+  } bp, xp, r, d, sl;
+  auto layout = [&](unsigned char* base) {
+    Carve c{base};
+    bp.p = c.take<double>(n);
+    xp.p = c.take<double>(n);
+    r.p = c.take<double>(n);
+    d.p = c.take<double>(mixed(ls) ? n : 0);
+    sl.p = c.take<double>(rel_each && inter ? 2 * n1 : 0);
+    return c.off;
+  };
+  layout(ensure_ws(ls->solve_ws, layout(nullptr)));
   struct LayoutReset {  // the flag describes this solve's vectors only
     nalu_linsys* ls;
     ~LayoutReset() { ls->stack_interleaved = false; }
@@ -1841,7 +1867,6 @@ int krylov_solve_k(nalu_linsys* ls, int nrhs, const double* b, double* x, int* i
     // FP64 iterative refinement around an FP32 inner (F)GMRES: r = b - A x in
     // FP64, d ~ A^-1 r in FP32 arithmetic with FP64 Arnoldi inner products and
     // least squares, x += d in FP64.
-    DevBuf<double> d(n);
     spmv_m<double, double>(ls, nrhs, xp.p, r.p, bp.p, 1);
     double rn = std::sqrt(red.dot(r.p, r.p, n));
     const double target = tol * (bn > 0 ? bn : 1.0);
@@ -1862,7 +1887,6 @@ int krylov_solve_k(nalu_linsys* ls, int nrhs, const double* b, double* x, int* i
   spmv_m<double, double>(ls, nrhs, xp.p, r.p, bp.p, 1);
   const double rn = std::sqrt(red.dot(r.p, r.p, n));
   if (rel_each) {
-    DevBuf<double> sl(inter ? 2 * n1 : 0);
     for (int q = 0; q < nrhs; ++q) {
       const double *bq_p = bp.p + q * n1, *rq_p = r.p + q * n1;
       if (inter) {  // contiguous copies of component q
