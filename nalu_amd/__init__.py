@@ -258,6 +258,8 @@ class Mesh:
         self.num_nodes = coords.shape[0]
         self.num_elems = conn.shape[0]
         ids = None if node_ids is None else _np(node_ids, np.int64)
+        if ids is not None and ids.shape != (self.num_nodes,):
+            raise NaluError("Mesh: node_ids must hold one id per node")
         self.has_ids = ids is not None
         h = _vp()
         _check(lib().nalu_mesh_create(self.num_nodes, self.num_elems, conn.ctypes.data_as(_i32p),

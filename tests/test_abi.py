@@ -80,6 +80,28 @@ def test_config_validation_without_gpu():
         ls.finalizeLinearSystem()
 
 
+def test_mesh_validation_without_gpu():
+    # every check runs on the host before any device call
+    coords, conn = na.lattice(2)
+    with pytest.raises(na.NaluError, match="bad arguments"):
+        na.Mesh(coords, conn[:0])  # no elements
+    bad = conn.copy()
+    bad[3, 5] = coords.shape[0]
+    with pytest.raises(na.NaluError, match="out of range"):
+        na.Mesh(coords, bad)
+    bad[3, 5] = -1
+    with pytest.raises(na.NaluError, match="out of range"):
+        na.Mesh(coords, bad)
+    ids = np.arange(1, coords.shape[0] + 1, dtype=np.int64)
+    with pytest.raises(na.NaluError, match="one id per node"):
+        na.Mesh(coords, conn, node_ids=ids[:-1])
+    ids[0] = 0
+    with pytest.raises(na.NaluError, match="ids must be in"):
+        na.Mesh(coords, conn, node_ids=ids)
+    with pytest.raises(na.NaluError, match=r"\[N,3\]"):
+        na.Mesh(coords[:, :2], conn)
+
+
 def test_product_does_not_import_oracle():
     # the shipped package never reaches the checker
     for dirpath, _, files in os.walk(os.path.join(ROOT, "nalu_amd")):
