@@ -436,6 +436,12 @@ int nalu_gnn_data_destroy(nalu_gnn_data *d);
 int nalu_gnn_data_size(const nalu_gnn_data *d);
 int nalu_gnn_data_cg_iterations(const nalu_gnn_data *d, int *iters_host, int count);
 int nalu_gnn_data_pair(const nalu_gnn_data *d, int idx, float *r_dev, float *e_dev);
+/* No counterpart in the reference (training data of the learned preconditioner).
+ * Overwrites pair idx with the caller's (r, e), FP64 device vectors in row
+ * order with A e = r, stored like a harvested pair (scaled by 1/|r|, e off its
+ * mean, the base preconditioner's output of r as the feature); ls is the system
+ * the pairs were harvested from. */
+int nalu_gnn_data_set_pair(nalu_gnn_data *d, nalu_linsys *ls, int idx, const double *r_dev, const double *e_dev);
 /* Adam (cosine learning-rate decay to lr/10) on the pairs harvested from the
  * same system; epoch_loss_host[epochs] receives the mean loss per epoch. */
 int nalu_gnn_train(nalu_gnn *g, nalu_linsys *ls, const nalu_gnn_data *d, int epochs, int batch, double lr,

@@ -161,6 +161,7 @@ def lib():
         L.nalu_gnn_data_size.argtypes = [_vp]
         L.nalu_gnn_data_cg_iterations.argtypes = [_vp, C.POINTER(C.c_int), C.c_int]
         L.nalu_gnn_data_pair.argtypes = [_vp, C.c_int, _vp, _vp]
+        L.nalu_gnn_data_set_pair.argtypes = [_vp, _vp, C.c_int, _vp, _vp]
         L.nalu_gnn_train.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
         L.nalu_linsys_set_learned_precond.argtypes = [_vp, _vp]
         L.nalu_linsys_set_learned_base.argtypes = [_vp, C.c_int]
@@ -682,6 +683,15 @@ class HarvestedPairs:
             raise NaluError("pair: index out of range")
         _check(lib().nalu_gnn_data_pair(self._h, int(idx), _dev(r, "pair r", "f32", numel=n),
                                         _dev(e, "pair e", "f32", numel=n)), "gnn_data_pair")
+
+    def set_pair(self, idx, r, e):
+        """Overwrite pair idx with (r, e), float64 device tensors in the original
+        row order with A e = r (stored scaled by 1/|r|, e off its mean)."""
+        n = self.ls._rows()
+        if not 0 <= int(idx) < len(self):
+            raise NaluError("set_pair: index out of range")
+        _check(lib().nalu_gnn_data_set_pair(self._h, self.ls.handle, int(idx), _dev(r, "set_pair r", "f64", numel=n),
+                                            _dev(e, "set_pair e", "f64", numel=n)), "gnn_data_set_pair")
 
     def __del__(self):
         h = getattr(self, "_h", None)

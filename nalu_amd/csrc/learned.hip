@@ -1909,6 +1909,47 @@ extern "C" int nalu_gnn_data_pair(const nalu_gnn_data* d, int idx, float* r_dev,
   });
 }
 
+// Overwrites pair idx with a caller's (r, e), FP64 device vectors in the row
+// order with A e = r (for instance the residual and error of an iterate of a
+// solve whose solution is known), stored like a harvested pair: solver order,
+// both scaled by 1/|r|, e off its mean, the base preconditioner's output of r
+// as the stored feature.
+extern "C" int nalu_gnn_data_set_pair(nalu_gnn_data* d, nalu_linsys* ls, int idx, const double* r_dev,
+                                      const double* e_dev) {
+  return guarded([&]() -> int {
+    using namespace nalu;
+    NALU_REQUIRE(d && ls && r_dev && e_dev && idx >= 0 && idx < d->npairs, NALU_ERR_ARG,
+                 "gnn_data_set_pair: bad arguments");
+    NALU_REQUIRE(d->owner == ls && d->n == ls->nrows, NALU_ERR_ARG, "gnn_data_set_pair: data of another system");
+    hipStream_t s = ls->stream;
+    const int64_t n = d->n;
+    const unsigned nb = std::min<unsigned>(kRedBlocks, nblocks(n));
+    DevBuf<double> rp(n), ep(n), part(kRedBlocks), sc(2);
+    hipLaunchKernelGGL((k_gather_perm<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, r_dev, ls->perm.p, n,
+                       1, rp.p);
+    hipLaunchKernelGGL((k_gather_perm<double, double>), dim3(nblocks(n)), dim3(kThreads), 0, s, e_dev, ls->perm.p, n,
+                       1, ep.p);
+    hipLaunchKernelGGL((k_bdot<double, double>), dim3(nb, 1), dim3(kThreads), 0, s, rp.p, rp.p, n, part.p);
+    hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, part.p, (int)nb, sc.p + 0, 1, 0);
+    hipLaunchKernelGGL(k_bsum<double>, dim3(nb, 1), dim3(kThreads), 0, s, ep.p, n, part.p);
+    hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, part.p, (int)nb, sc.p + 1, 1, 0);
+    double nr2 = 0.0;
+    NALU_HIP_CHECK(hipMemcpyAsync(&nr2, sc.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    NALU_HIP_CHECK(hipStreamSynchronize(s));
+    NALU_REQUIRE(nr2 > 0 && std::isfinite(nr2), NALU_ERR_NUMERIC, "gnn_data_set_pair: zero or non-finite residual");
+    float* R = d->R.p + (int64_t)idx * n;
+    float* E = d->E.p + (int64_t)idx * n;
+    hipLaunchKernelGGL(k_pair, dim3(nblocks(n)), dim3(kThreads), 0, s, rp.p, ep.p, sc.p + 1, sc.p + 0, n, R, E);
+    if (!ls->precond_ready) precond_setup(ls);
+    ensure_split32(ls);
+    base32(ls, R, d->ZS.p + (int64_t)idx * n);
+    hipLaunchKernelGGL((k_bdot<float, float>), dim3(nb, 1), dim3(kThreads), 0, s, E, R, n, part.p);
+    hipLaunchKernelGGL(k_brow_sum, dim3(1), dim3(kThreads), 0, s, part.p, (int)nb, d->S.p + idx, 1, 0);
+    NALU_HIP_CHECK(hipStreamSynchronize(s));
+    return NALU_OK;
+  });
+}
+
 // Adam training on the harvested pairs of the same system; epoch_loss_host[e]
 // receives the mean batch loss of epoch e.  The learning rate decays from lr to
 // lr/10 on a cosine over all steps.

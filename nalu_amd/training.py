@@ -32,12 +32,56 @@ def _continuity_system(mesh, fields, cfg, dev, base="sgs", muelu_params=None):
     return ls
 
 
+def _self_pairs(net, pairs, mesh, problem, device, fraction, caps, seed, base, muelu_params):
+    """Replace a fraction of the pairs by the residuals and errors of the
+    network's own FGMRES iterates: seeded zero-mean Gaussian right-hand sides b,
+    x* from FP64 CG + SGS to 1e-11, x_k after k FGMRES(50) iterations with the
+    network (FP64) for each cap k, pair (b - A x_k, x* - x_k).  These are the
+    residuals the network leaves behind in its own solves."""
+    import torch
+    lx = _continuity_system(mesh, problem, SolverConfig("cg", "sgs", tolerance=1e-11, max_iterations=5000), device)
+    lfs = []
+    for k in caps:
+        lf = _continuity_system(mesh, problem, SolverConfig("fgmres", "learned", tolerance=1e-12, max_iterations=int(k),
+                                                            kspace=50), device, base=base, muelu_params=muelu_params)
+        lf.set_learned_preconditioner(net)
+        lfs.append(lf)
+    n = lx.sizes()[0]
+    g = torch.Generator(device="cpu").manual_seed(int(seed))
+    nrep = int(round(fraction * len(pairs)))
+    idx = torch.randperm(len(pairs), generator=g)[:nrep].tolist()
+    q = 0
+    y = torch.zeros(n, dtype=torch.float64, device=device)
+    while q < nrep:
+        b = torch.randn(n, generator=g, dtype=torch.float64).to(device)
+        b -= b.mean()
+        xs = torch.zeros_like(b)
+        lx.solve_rhs(b, xs)
+        for lf in lfs:
+            if q >= nrep:
+                break
+            xk = torch.zeros_like(b)
+            lf.solve_rhs(b, xk)
+            lx.spmv(xk, y)
+            pairs.set_pair(idx[q], (b - y).contiguous(), (xs - xk).contiguous())
+            q += 1
+    for lf in lfs:
+        lf.set_learned_preconditioner(None)
+    return nrep
+
+
 def train_pressure_preconditioner(problem, num_rhs=400, pairs_per_rhs=5, harvest_tol=1e-6, epochs=10, batch=16,
                                   lr=3e-3, seed=1, net_seed=2, layers=4, hidden=32, validate_max_iterations=600,
-                                  device="cuda:0", log=None, base="sgs", muelu_params=None):
+                                  device="cuda:0", log=None, base="sgs", muelu_params=None, self_pair_rounds=0,
+                                  self_pair_fraction=0.5, self_pair_caps=(3, 6, 10, 15, 25), self_pair_epochs=6,
+                                  self_pair_lr=1e-3):
     """problem: dict with coords, conn, rho, p, u, gpdx (a lowmach_problem).
     base: the network's base preconditioner, "sgs" or "muelu" (the AMG V-cycle
     with muelu_params); the network is then meant for systems with that base.
+    self_pair_rounds > 0: after the first training, that many rounds of
+    replacing self_pair_fraction of the pairs by the network's own FGMRES
+    residuals / errors (_self_pairs) and training self_pair_epochs more epochs
+    at self_pair_lr, each round validated and selected like the first.
     Returns (network, record).  The record holds the harvest and training
     statistics and the validation FGMRES iterations of every epoch (10000 is
     added when the solve fell back to the base)."""
@@ -77,11 +121,24 @@ def train_pressure_preconditioner(problem, num_rhs=400, pairs_per_rhs=5, harvest
     t_train = time.perf_counter() - t0
     say("validation iterations per epoch %s, kept epoch %d" % (net.validation, net.best_epoch))
     say("trained %d epochs in %.2f s, losses %s" % (epochs, t_train, np.round(hist, 4).tolist()))
+    rounds = []
+    for rnd in range(int(self_pair_rounds)):
+        t0 = time.perf_counter()
+        nrep = _self_pairs(net, pairs, mesh, problem, device, self_pair_fraction, self_pair_caps, 1000 + rnd, base,
+                           muelu_params)
+        h2 = net.train(lt, pairs, epochs=self_pair_epochs, batch=batch, lr=self_pair_lr, seed=seed + 1 + rnd,
+                       validate=validate)
+        torch.cuda.synchronize()
+        rounds.append({"pairs_replaced": nrep, "validation_fgmres_iters_per_epoch": net.validation,
+                       "kept_epoch": net.best_epoch, "epoch_loss": h2.tolist(), "s": time.perf_counter() - t0})
+        say("self-pair round %d: %d pairs replaced, validation %s, kept epoch %d" % (rnd + 1, nrep, net.validation,
+                                                                                   net.best_epoch))
     rec = {"train_mesh": "%d^3" % n_t, "pairs": len(pairs), "rhs": num_rhs, "pairs_per_rhs": pairs_per_rhs,
            "cg_tol": harvest_tol, "validation_fgmres_iters_per_epoch": net.validation, "kept_epoch": net.best_epoch,
            "harvest_s": t_harvest, "harvest_cg_iters_mean": float(np.mean(pairs.cg_iterations())),
            "train_s": t_train, "epochs": epochs, "batch": batch, "lr": lr, "epoch_loss": hist.tolist(),
-           "layers": layers, "hidden": hidden, "weights": "bf16", "accumulate": "fp32", "base": base}
+           "layers": layers, "hidden": hidden, "weights": "bf16", "accumulate": "fp32", "base": base,
+           "self_pair_rounds": rounds}
     lv.set_learned_preconditioner(None)
     del pairs, lv, lt
     return net, rec

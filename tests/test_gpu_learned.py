@@ -251,6 +251,51 @@ def test_harvest_pairs_are_consistent(oracle):
         assert float(eh @ rh) > 0
 
 
+def test_set_pair_stores_a_caller_pair(oracle):
+    """A caller's (r = A e, e) is stored like a harvested pair: scaled by 1/|r|,
+    e off its mean; the stored feature is the base preconditioner's output."""
+    print("case: set_pair n=8", flush=True)
+    _, _, ls = poisson_system(8, na.SolverConfig("cg", "sgs"))
+    rp, ci, v, _ = ls.csr()
+    n = len(rp) - 1
+    data = na.HarvestedPairs(ls, num_rhs=2, pairs_per_rhs=2, seed=7, tol=1e-10)
+    e = np.random.default_rng(3).standard_normal(n) + 5.0  # mean far from zero
+    r = oracle.spmv(rp, ci, v, e)
+    data.set_pair(2, dev(r), dev(e))
+    rg = torch.zeros(n, dtype=torch.float32, device=DEV)
+    eg = torch.zeros_like(rg)
+    data.pair(2, rg, eg)
+    nr = np.linalg.norm(r)
+    assert rel(rg.cpu().numpy().astype(np.float64), r / nr) <= 1e-6
+    assert rel(eg.cpu().numpy().astype(np.float64), (e - e.mean()) / nr) <= 1e-6
+    # the other pairs are untouched and the set trains
+    data.pair(0, rg, eg)
+    assert abs(np.linalg.norm(rg.cpu().numpy().astype(np.float64)) - 1.0) < 1e-6
+    net = na.LearnedPreconditioner(seed=2)
+    hist = net.train(ls, data, epochs=2, batch=2, lr=1e-3, seed=1)
+    assert np.all(np.isfinite(hist))
+    with pytest.raises(na.NaluError, match="index out of range"):
+        data.set_pair(4, dev(r), dev(e))
+    with pytest.raises(na.NaluError, match="zero or non-finite"):
+        data.set_pair(1, dev(np.zeros(n)), dev(e))
+
+
+def test_self_pair_rounds():
+    """The training recipe's self-pair rounds replace pairs with the network's
+    own FGMRES residuals / errors and retrain with validation."""
+    print("case: self-pair rounds n=10", flush=True)
+    from nalu_amd.training import train_pressure_preconditioner
+    P = lowmach_problem(10, rho_sigma=0.3, seed=4321)
+    net, rec = train_pressure_preconditioner(P, num_rhs=16, pairs_per_rhs=4, harvest_tol=1e-6, epochs=2, batch=8,
+                                             lr=3e-3, device=DEV, self_pair_rounds=1, self_pair_fraction=0.5,
+                                             self_pair_caps=(2, 5), self_pair_epochs=2)
+    rounds = rec["self_pair_rounds"]
+    assert len(rounds) == 1 and rounds[0]["pairs_replaced"] == 32
+    assert len(rounds[0]["validation_fgmres_iters_per_epoch"]) == 2
+    assert all(np.isfinite(rounds[0]["epoch_loss"]))
+    assert 0 <= rounds[0]["kept_epoch"] < 2
+
+
 def test_training_gains_over_sgs():
     print("case: train n=12 and FGMRES learned vs SGS", flush=True)
     cfg = na.SolverConfig("fgmres", "learned", tolerance=1e-8, max_iterations=2000, kspace=30, precision="mixed")
