@@ -279,6 +279,7 @@ __global__ __launch_bounds__(256) void k_scalar_gather(Op op, const int32_t* __r
 // then written to HBM once, coalesced -- no global atomics, no separate zero pass.
 constexpr int kTileNodes = 64;
 constexpr int kTileThreads = kTileNodes * 8;
+constexpr size_t kMaxLds = 160 * 1024;  // LDS of one gfx950 CU, the most one block can hold
 
 template <class Op>
 __global__ __launch_bounds__(kTileThreads) void k_scalar_tile(Op op, const int32_t* __restrict__ conn,
@@ -776,7 +777,8 @@ struct MassArgs {
   int balanced;          // densFac = rhoNp1 (balanced force) or 1
 };
 
-template <int LA, bool MOM, bool LUMPED, bool SEG>
+// ATOMIC: the row is shared with other threads (LDS row segment of a node tile)
+template <int LA, bool MOM, bool LUMPED, bool SEG, bool ATOMIC = false>
 __device__ __forceinline__ void mass_row(const MassArgs& A, const int32_t (&nd)[8], const double (&x)[8][3],
                                          uint64_t pk, double* __restrict__ row, int32_t L, double (&rh)[3]) {
   const double scV = hex8::scv_volume<LA>(x);
@@ -809,6 +811,9 @@ __device__ __forceinline__ void mass_row(const MassArgs& A, const int32_t (&nd)[
       const int p = (int)((pk >> (8 * c)) & 0xff);
       if (SEG) {
         row[p] += lf;
+      } else if (ATOMIC) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) unsafeAtomicAdd(row + 3 * j * L + 3 * p + j, lf);
       } else {
 #pragma unroll
         for (int j = 0; j < 3; ++j) row[3 * j * L + 3 * p + j] += lf;
@@ -858,6 +863,71 @@ __global__ __launch_bounds__(256) void k_mass_gather(MassArgs A, const int32_t* 
     for (int j = 0; j < 3; ++j) rhs[3 * r + j] += rh[j];
   } else {
     rhs[r] += rh[0];
+  }
+}
+
+// Momentum mass (numDof 3) as a node tile, the structure of k_momentum_tile:
+// a block owns kTileNodes consecutive rows; thread (node j, slot s) adds the
+// sub-control volume of local node s of each element in which node j has local
+// index s (the 3x3 blocks' diagonal entries, LDS atomics: the 8 slots share
+// the row), and the block adds its LDS segment to the matrix in one coalesced
+// pass -- or stores it, folding a pending zeroSystem.  Replaces the
+// thread-per-row form, whose scattered read-modify-writes of 81 diagonal
+// entries per row went to HBM one 8-byte value at a time.
+template <bool LUMPED>
+__global__ __launch_bounds__(kTileThreads) void k_mass_tile3(MassArgs A, const int32_t* __restrict__ conn,
+                                                             const double* __restrict__ coords,
+                                                             const int32_t* __restrict__ row_node,
+                                                             const int32_t* __restrict__ inc_ptr,
+                                                             const int32_t* __restrict__ inc_elem,
+                                                             const int32_t* __restrict__ nrp,
+                                                             const uint64_t* __restrict__ elem_pos, int64_t nnodes,
+                                                             double* __restrict__ vals, double* __restrict__ rhs,
+                                                             int accumulate) {
+  extern __shared__ double seg[];
+  const int64_t a0 = (int64_t)blockIdx.x * kTileNodes;
+  const int64_t a1 = min(a0 + (int64_t)kTileNodes, nnodes);
+  const int64_t s0 = 9 * (int64_t)nrp[a0], s1 = 9 * (int64_t)nrp[a1];
+  const int nseg = (int)(s1 - s0);
+  double* rseg = seg + nseg;
+  for (int i = threadIdx.x; i < nseg + 3 * kTileNodes; i += blockDim.x) seg[i] = 0.0;
+  __syncthreads();
+  const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileNodes);
+  const int j = threadIdx.x % kTileNodes;
+  const int64_t r = a0 + j;
+  if (r < a1) {
+    const int32_t a = row_node ? row_node[r] : (int32_t)r;
+    const int32_t L = nrp[r + 1] - nrp[r];
+    double* myrow = seg + (9 * (int64_t)nrp[r] - s0);
+    double rh[3] = {0.0, 0.0, 0.0};
+    for (int32_t q = inc_ptr[a]; q < inc_ptr[a + 1]; ++q) {
+      const int32_t t = inc_elem[q];
+      if ((t & 7) != slot) continue;
+      const int64_t e = t >> 3;
+      int32_t nd[8];
+      load_nodes(conn, e, nd);
+      double x[8][3];
+      load_vec3(coords, nd, x);
+      const uint64_t pk = elem_pos[t];
+      switch (slot) {
+#define NALU_MS(Lx) \
+  case Lx: mass_row<Lx, true, LUMPED, false, true>(A, nd, x, pk, myrow, L, rh); break;
+        NALU_MS(0) NALU_MS(1) NALU_MS(2) NALU_MS(3) NALU_MS(4) NALU_MS(5) NALU_MS(6) NALU_MS(7)
+#undef NALU_MS
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (rh[i] != 0.0) unsafeAtomicAdd(rseg + 3 * j + i, rh[i]);
+  }
+  __syncthreads();
+  const int nr = (int)(3 * (a1 - a0));
+  if (accumulate) {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) vals[s0 + i] += seg[i];
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] += rseg[i];
+  } else {
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) __builtin_nontemporal_store(seg[i], &vals[s0 + i]);
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) rhs[3 * a0 + i] = rseg[i];
   }
 }
 
@@ -1118,6 +1188,21 @@ namespace {
 void launch_mass(nalu_linsys* ls, const nalu_mesh* m, const MassArgs& A, bool mom, bool lumped,
                  double* rhs3 = nullptr) {
   build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
+  const size_t lds3 = ((size_t)kTileNodes * ls->max_node_row_len * 9 + 3 * kTileNodes) * sizeof(double);
+  if (mom && !rhs3 && lds3 <= kMaxLds && ls->values_pending_zero == ls->rhs_pending_zero) {
+    // numDof 3: node tile with the row segments in LDS
+    const int accumulate = ls->values_pending_zero ? 0 : 1;
+    auto go3 = [&](auto kern) {
+      set_lds_limit((const void*)kern, lds3);
+      hipLaunchKernelGGL(kern, dim3(blocks_for(ls->num_nodes, kTileNodes)), dim3(kTileThreads), lds3, ls->stream, A,
+                         m->conn.p, m->coords.p, (const int32_t*)ls->row_node.p, m->inc_ptr.p, m->inc_elem.p,
+                         ls->nrp.p, ls->elem_pos.p, ls->num_nodes, ls->values.p, ls->rhs.p, accumulate);
+    };
+    lumped ? go3(k_mass_tile3<true>) : go3(k_mass_tile3<false>);
+    ls->values_pending_zero = ls->rhs_pending_zero = false;
+    NALU_HIP_CHECK(hipGetLastError());
+    return;
+  }
   materialize_pending_zero(ls);
   const dim3 g(blocks_for(ls->num_nodes, 256)), b(256);
   auto go = [&](auto kern) {
