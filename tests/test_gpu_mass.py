@@ -108,6 +108,38 @@ def test_mass_whole_mesh_parity(oracle, lumped, balanced):
     assert not np.any(v1)
 
 
+@pytest.mark.parametrize("lumped", [False, True])
+def test_momentum_mass_tile_alone_with_global_ids(oracle, lumped):
+    """The numDof 3 mass tile as the first assembly after zeroSystem (its
+    store path) on a mesh with permuted global ids (rows are not nodes)."""
+    n = 5
+    print("case: momentum mass alone, permuted ids, n=%d lumped=%s" % (n, lumped), flush=True)
+    P = lowmach_problem(n, seed=21, rho_sigma=0.3)
+    X, conn = P["coords"], P["conn"]
+    N = X.shape[0]
+    ids = np.random.default_rng(8).permutation(N).astype(np.int64) + 1
+    rng = np.random.default_rng(23)
+    rho = [P["rho"] * (1 + 0.1 * rng.random(N)) for _ in range(3)]
+    u = [P["u"] + 0.1 * rng.standard_normal((N, 3)) for _ in range(3)]
+    gjp = P["gpdx"] + 0.1 * rng.standard_normal((N, 3))
+    dt, gam = 0.02, (1.5, -2.0, 0.5)
+    mesh = na.Mesh(X, conn, node_ids=ids)
+    ls = na.LinearSystem(3, na.SolverConfig())
+    ls.buildElemToNodeGraph(mesh)
+    ls.finalizeLinearSystem()
+    ls.zeroSystem()
+    na.assemble_momentum_mass(ls, mesh, [dev(a) for a in rho], [dev(a) for a in u], dev(gjp), dt, gam,
+                              lumped=lumped)
+    rp, ci, v, r = ls.csr()
+    g3 = oracle.graph_build(N, conn, 3, node_ids=ids)
+    np.testing.assert_array_equal(ci, g3.col_idx)
+    mv, mr = oracle.assemble_momentum_mass(g3, conn, X, rho, u, gjp, dt, gam, lumped=lumped)
+    e_mat = rel_err_rowwise(v, mv, g3.row_ptr)
+    e_rhs = np.abs(r - mr).max() / np.abs(mr).max()
+    print("  rel err: matrix %.2e rhs %.2e" % (e_mat, e_rhs), flush=True)
+    assert e_mat <= TOL and e_rhs <= TOL
+
+
 def test_mass_rejects_bad_arguments():
     mesh, ls = system(UNIT_HEX, np.arange(8, dtype=np.int32)[None, :], 1)
     z = dev(np.zeros(8))
