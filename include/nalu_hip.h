@@ -317,6 +317,18 @@ int nalu_assemble_momentum_mass(nalu_linsys *ls, const nalu_mesh *m, const doubl
                                 const double *rho_n, const double *rho_np1, const double *u_nm1,
                                 const double *u_n, const double *u_np1, const double *gjp, double dt,
                                 const double *gamma, int lumped, int balanced_force);
+/* MomentumAdvDiffElemKernel and MomentumMassElemKernel in one pass over the
+ * elements, as AssembleElemSolverAlgorithm::execute runs its kernel list
+ * (src/AssembleElemSolverAlgorithm.C:75-100 with both kernels registered):
+ * the arguments of nalu_assemble_momentum (node-tile scatter) and of
+ * nalu_assemble_momentum_mass; one LDS row segment per tile, one write of the
+ * matrix.  FP32-assembly systems run the two passes one after the other. */
+int nalu_assemble_momentum_with_mass(nalu_linsys *ls, const nalu_mesh *m, const double *u, const double *mu,
+                                     const double *mdot, double include_divu, int skew, int shifted,
+                                     const double *rho_nm1, const double *rho_n, const double *rho_np1,
+                                     const double *u_nm1, const double *u_n, const double *u_np1,
+                                     const double *gjp, double dt, const double *gamma, int lumped,
+                                     int balanced_force);
 /* Momentum as three right-hand sides sharing one scalar matrix (the segregated
  * form of MomentumAdvDiffElemKernel, src/kernel/MomentumAdvDiffElemKernel.C:72-186):
  * ls is a numDof 1 system on the mesh; the matrix gets the advection-diffusion

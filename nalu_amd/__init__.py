@@ -132,6 +132,9 @@ def lib():
         L.nalu_assemble_momentum_mass.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _dp,
                                                   C.c_int, C.c_int]
         L.nalu_assemble_continuity_mass.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, _dp, C.c_int]
+        L.nalu_assemble_momentum_with_mass.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp,
+                                                       _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _dp, C.c_int,
+                                                       C.c_int]
         L.nalu_assemble_momentum_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp]
         L.nalu_assemble_momentum_mass_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
                                                              _dp, C.c_int, C.c_int, _vp]
@@ -863,6 +866,23 @@ def _gammas(gamma):
     if g.shape != (3,):
         raise NaluError("gamma: three BDF coefficients expected")
     return g
+
+
+def assemble_momentum_with_mass(ls, mesh, u, mu, mdot, rho, u_states, gjp, dt, gamma=(1.0, -1.0, 0.0), lumped=False,
+                                balanced_force=False, include_divu=0.0, skew=False, shifted=False):
+    """MomentumAdvDiffElemKernel + MomentumMassElemKernel in one element pass
+    (the kernel list of one element algorithm): u, mu, mdot as for
+    assemble_momentum; rho = (rhoNm1, rhoN, rhoNp1), u_states = (uNm1, uN, uNp1),
+    gjp, dt, gamma as for assemble_momentum_mass."""
+    N, E = mesh.num_nodes, mesh.num_elems
+    r, v, g = _states(rho, "momentum_with_mass rho", N), _states(u_states, "momentum_with_mass u", 3 * N), \
+        _gammas(gamma)
+    _check(lib().nalu_assemble_momentum_with_mass(
+        ls.handle, mesh.handle, _dev(u, "momentum_with_mass u", "f64", numel=3 * N),
+        _dev(mu, "momentum_with_mass mu", "f64", numel=N), _dev(mdot, "momentum_with_mass mdot", "f64", numel=12 * E),
+        float(include_divu), int(skew), int(shifted), r[0], r[1], r[2], v[0], v[1], v[2],
+        _dev(gjp, "momentum_with_mass Gjp", "f64", numel=3 * N), float(dt), g.ctypes.data_as(_dp), int(lumped),
+        int(balanced_force)), "assemble_momentum_with_mass")
 
 
 def assemble_momentum_mass(ls, mesh, rho, u, gjp, dt, gamma=(1.0, -1.0, 0.0), lumped=False, balanced_force=False):

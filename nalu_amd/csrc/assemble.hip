@@ -269,6 +269,71 @@ __global__ __launch_bounds__(256) void k_scalar_gather(Op op, const int32_t* __r
   }
 }
 
+// ---------------------------------------------------------------- mass (time derivative)
+// MomentumMassElemKernel (src/kernel/MomentumMassElemKernel.C:95-170) and
+// ContinuityMassElemKernel (src/kernel/ContinuityMassElemKernel.C:84-118).  The
+// scv ip of an element belongs to one node (HexSCV::ipNodeMap_), so a node's
+// rows receive exactly the contributions of the scvs at that node: k_mass_gather
+// gives each node one thread that walks its incident elements (ip = the node's
+// local index) and adds into its rows and rhs in place (no atomics, no LDS);
+// the numDof 3 forms (k_mass_tile3, and the mass option of k_momentum_tile)
+// accumulate node tiles in LDS.
+struct MassArgs {
+  const double* rho[3];  // Nm1, N, Np1 (nodal)
+  const double* u[3];    // Nm1, N, Np1 (nodal, 3 components; momentum only)
+  const double* gjp;     // projected pressure gradient (momentum only)
+  double dt, g1, g2, g3;
+  int balanced;          // densFac = rhoNp1 (balanced force) or 1
+};
+
+// ATOMIC: the row is shared with other threads (LDS row segment of a node tile)
+template <int LA, bool MOM, bool LUMPED, bool SEG, bool ATOMIC = false>
+__device__ __forceinline__ void mass_row(const MassArgs& A, const int32_t (&nd)[8], const double (&x)[8][3],
+                                         uint64_t pk, double* __restrict__ row, int32_t L, double (&rh)[3]) {
+  const double scV = hex8::scv_volume<LA>(x);
+  double rs[3] = {0.0, 0.0, 0.0}, us[3][3] = {}, g[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const double r = hex8::scv_shape<LA, LUMPED>(c);
+    if (LUMPED && c != LA) continue;
+#pragma unroll
+    for (int st = 0; st < 3; ++st) rs[st] += r * A.rho[st][nd[c]];
+    if (MOM) {
+#pragma unroll
+      for (int st = 0; st < 3; ++st)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) us[st][j] += r * A.u[st][3 * (int64_t)nd[c] + j];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) g[j] += r * A.gjp[3 * (int64_t)nd[c] + j];
+    }
+  }
+  if (MOM) {
+    const double densFac = A.balanced ? rs[2] : 1.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      rh[j] += -(A.g1 * rs[2] * us[2][j] + A.g2 * rs[1] * us[1][j] + A.g3 * rs[0] * us[0][j]) * scV / A.dt -
+               g[j] * scV * densFac;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (LUMPED && c != LA) continue;
+      const double lf = hex8::scv_shape<LA, LUMPED>(c) * A.g1 * rs[2] * scV / A.dt;
+      const int p = (int)((pk >> (8 * c)) & 0xff);
+      if (SEG) {
+        row[p] += lf;
+      } else if (ATOMIC) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) unsafeAtomicAdd(row + 3 * j * L + 3 * p + j, lf);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) row[3 * j * L + 3 * p + j] += lf;
+      }
+    }
+  } else {
+    const double projTimeScale = A.dt / A.g1;
+    rh[0] += -(A.g1 * rs[2] + A.g2 * rs[1] + A.g3 * rs[0]) * scV / A.dt / projTimeScale;
+  }
+}
+
 // ---------------------------------------------------------------- node-tile gather
 // Owner-computes with one thread per (row node, local index): a block owns
 // kTileNodes consecutive rows; thread (node j, slot s) evaluates row s of every
@@ -351,14 +416,17 @@ __device__ __forceinline__ void momentum_row_lds(const int32_t (&nd)[8], const R
 }
 
 // Momentum (numDof 3) node-tile gather: the block's segment holds the 3 rows of
-// each of its nodes (rows 3a..3a+2 are contiguous in the CSR).
-template <bool SKEW, bool SHIFT, typename R>
+// each of its nodes (rows 3a..3a+2 are contiguous in the CSR).  MASS (1
+// consistent, 2 lumped) adds the momentum mass kernel of the same element and
+// scv in the same pass, as one element algorithm runs its kernel list over an
+// element: one LDS segment, one write of the matrix.
+template <bool SKEW, bool SHIFT, typename R, int MASS = 0>
 __global__ __launch_bounds__(kTileThreads) void k_momentum_tile(
     const int32_t* __restrict__ conn, const double* __restrict__ coords, const double* __restrict__ u,
     const double* __restrict__ mu, const double* __restrict__ mdot, double incl_divu,
     const int32_t* __restrict__ row_node, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_elem,
     const int32_t* __restrict__ nrp, const uint64_t* __restrict__ elem_pos, int64_t nnodes,
-    double* __restrict__ vals, double* __restrict__ rhs, int accumulate) {
+    double* __restrict__ vals, double* __restrict__ rhs, int accumulate, MassArgs MA) {
   extern __shared__ double seg[];
   const int64_t a0 = (int64_t)blockIdx.x * kTileNodes;
   const int64_t a1 = min(a0 + (int64_t)kTileNodes, nnodes);
@@ -390,6 +458,16 @@ __global__ __launch_bounds__(kTileThreads) void k_momentum_tile(
   case Lx: momentum_row_lds<Lx, SKEW, SHIFT, R>(nd, x, u, mu, md, incl_divu, myrow, L, pk, rh); break;
         NALU_MT(0) NALU_MT(1) NALU_MT(2) NALU_MT(3) NALU_MT(4) NALU_MT(5) NALU_MT(6) NALU_MT(7)
 #undef NALU_MT
+      }
+      if (MASS) {
+        double xd[8][3];
+        load_vec3(coords, nd, xd);
+        switch (slot) {
+#define NALU_MM(Lx) \
+  case Lx: mass_row<Lx, true, MASS == 2, false, true>(MA, nd, xd, pk, myrow, L, rh); break;
+          NALU_MM(0) NALU_MM(1) NALU_MM(2) NALU_MM(3) NALU_MM(4) NALU_MM(5) NALU_MM(6) NALU_MM(7)
+#undef NALU_MM
+        }
       }
     }
 #pragma unroll
@@ -762,69 +840,6 @@ __global__ __launch_bounds__(256) void k_momentum_seg_gather(
   }
 }
 
-// ---------------------------------------------------------------- mass (time derivative)
-// MomentumMassElemKernel (src/kernel/MomentumMassElemKernel.C:95-170) and
-// ContinuityMassElemKernel (src/kernel/ContinuityMassElemKernel.C:84-118).  The
-// scv ip of an element belongs to one node (HexSCV::ipNodeMap_), so the node's
-// thread owns every contribution to its rows: one thread per node walks its
-// incident elements (ip = the node's local index) and adds into its rows and
-// rhs in place -- no atomics, no LDS.
-struct MassArgs {
-  const double* rho[3];  // Nm1, N, Np1 (nodal)
-  const double* u[3];    // Nm1, N, Np1 (nodal, 3 components; momentum only)
-  const double* gjp;     // projected pressure gradient (momentum only)
-  double dt, g1, g2, g3;
-  int balanced;          // densFac = rhoNp1 (balanced force) or 1
-};
-
-// ATOMIC: the row is shared with other threads (LDS row segment of a node tile)
-template <int LA, bool MOM, bool LUMPED, bool SEG, bool ATOMIC = false>
-__device__ __forceinline__ void mass_row(const MassArgs& A, const int32_t (&nd)[8], const double (&x)[8][3],
-                                         uint64_t pk, double* __restrict__ row, int32_t L, double (&rh)[3]) {
-  const double scV = hex8::scv_volume<LA>(x);
-  double rs[3] = {0.0, 0.0, 0.0}, us[3][3] = {}, g[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const double r = hex8::scv_shape<LA, LUMPED>(c);
-    if (LUMPED && c != LA) continue;
-#pragma unroll
-    for (int st = 0; st < 3; ++st) rs[st] += r * A.rho[st][nd[c]];
-    if (MOM) {
-#pragma unroll
-      for (int st = 0; st < 3; ++st)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) us[st][j] += r * A.u[st][3 * (int64_t)nd[c] + j];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) g[j] += r * A.gjp[3 * (int64_t)nd[c] + j];
-    }
-  }
-  if (MOM) {
-    const double densFac = A.balanced ? rs[2] : 1.0;
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-      rh[j] += -(A.g1 * rs[2] * us[2][j] + A.g2 * rs[1] * us[1][j] + A.g3 * rs[0] * us[0][j]) * scV / A.dt -
-               g[j] * scV * densFac;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      if (LUMPED && c != LA) continue;
-      const double lf = hex8::scv_shape<LA, LUMPED>(c) * A.g1 * rs[2] * scV / A.dt;
-      const int p = (int)((pk >> (8 * c)) & 0xff);
-      if (SEG) {
-        row[p] += lf;
-      } else if (ATOMIC) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) unsafeAtomicAdd(row + 3 * j * L + 3 * p + j, lf);
-      } else {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) row[3 * j * L + 3 * p + j] += lf;
-      }
-    }
-  } else {
-    const double projTimeScale = A.dt / A.g1;
-    rh[0] += -(A.g1 * rs[2] + A.g2 * rs[1] + A.g3 * rs[0]) * scV / A.dt / projTimeScale;
-  }
-}
-
 template <bool MOM, bool LUMPED, bool SEG>
 __global__ __launch_bounds__(256) void k_mass_gather(MassArgs A, const int32_t* __restrict__ conn,
                                                      const double* __restrict__ coords,
@@ -1072,6 +1087,49 @@ extern "C" int nalu_compute_mdot_f32(const nalu_mesh* m, const double* rho, cons
   });
 }
 
+namespace {
+// The node-tile momentum pass; ma != nullptr adds the momentum mass kernel in
+// the same pass (FP64 assembly only).
+void launch_momentum_tile(nalu_linsys* ls, const nalu_mesh* m, const double* u, const double* mu, const double* mdot,
+                          double include_divu, bool skew, bool shifted, const MassArgs* ma, bool lumped) {
+  build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
+  NALU_REQUIRE(ls->values_pending_zero == ls->rhs_pending_zero, NALU_ERR_STATE,
+               "tile assembly: values/rhs zero state diverged");
+  const int accumulate = ls->values_pending_zero ? 0 : 1;
+  const size_t lds = ((size_t)kTileNodes * ls->max_node_row_len * 9 + 3 * kTileNodes) * sizeof(double);
+  const MassArgs none{};
+  const MassArgs& A = ma ? *ma : none;
+  auto go = [&](auto kern) {
+    set_lds_limit((const void*)kern, lds);
+    hipLaunchKernelGGL(kern, dim3(blocks_for(ls->num_nodes, kTileNodes)), dim3(kTileThreads), lds, ls->stream,
+                       m->conn.p, m->coords.p, u, mu, mdot, include_divu, (const int32_t*)ls->row_node.p,
+                       m->inc_ptr.p, m->inc_elem.p, ls->nrp.p, ls->elem_pos.p, ls->num_nodes, ls->values.p,
+                       ls->rhs.p, accumulate, A);
+  };
+  const bool f32 = fp32_assembly(ls);
+  NALU_REQUIRE(!(ma && f32), NALU_ERR_UNSUPPORTED, "momentum tile: the fused mass pass is FP64 only");
+  if (ma) {
+    if (lumped) {
+      if (skew && shifted) go(k_momentum_tile<true, true, double, 2>);
+      else if (skew) go(k_momentum_tile<true, false, double, 2>);
+      else if (shifted) go(k_momentum_tile<false, true, double, 2>);
+      else go(k_momentum_tile<false, false, double, 2>);
+    } else {
+      if (skew && shifted) go(k_momentum_tile<true, true, double, 1>);
+      else if (skew) go(k_momentum_tile<true, false, double, 1>);
+      else if (shifted) go(k_momentum_tile<false, true, double, 1>);
+      else go(k_momentum_tile<false, false, double, 1>);
+    }
+  } else {
+    if (skew && shifted) f32 ? go(k_momentum_tile<true, true, float>) : go(k_momentum_tile<true, true, double>);
+    else if (skew) f32 ? go(k_momentum_tile<true, false, float>) : go(k_momentum_tile<true, false, double>);
+    else if (shifted) f32 ? go(k_momentum_tile<false, true, float>) : go(k_momentum_tile<false, true, double>);
+    else f32 ? go(k_momentum_tile<false, false, float>) : go(k_momentum_tile<false, false, double>);
+  }
+  ls->values_pending_zero = ls->rhs_pending_zero = false;
+}
+}  // namespace
+
 extern "C" int nalu_assemble_momentum(nalu_linsys* ls, const nalu_mesh* m, const double* u, const double* mu,
                                       const double* mdot, double include_divu, int skew, int shifted, int mode) {
   return guarded([&]() -> int {
@@ -1086,24 +1144,7 @@ extern "C" int nalu_assemble_momentum(nalu_linsys* ls, const nalu_mesh* m, const
       build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
       etile_momentum(ls, m, u, mu, mdot, include_divu, skew != 0, shifted != 0, fp32_assembly(ls));
     } else if (mode == 2) {
-      build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
-      NALU_REQUIRE(ls->values_pending_zero == ls->rhs_pending_zero, NALU_ERR_STATE,
-                   "tile assembly: values/rhs zero state diverged");
-      const int accumulate = ls->values_pending_zero ? 0 : 1;
-      const size_t lds = ((size_t)kTileNodes * ls->max_node_row_len * 9 + 3 * kTileNodes) * sizeof(double);
-      auto go = [&](auto kern) {
-        set_lds_limit((const void*)kern, lds);
-        hipLaunchKernelGGL(kern, dim3(blocks_for(ls->num_nodes, kTileNodes)), dim3(kTileThreads), lds, ls->stream,
-                           m->conn.p, m->coords.p, u, mu, mdot, include_divu, (const int32_t*)ls->row_node.p,
-                           m->inc_ptr.p, m->inc_elem.p, ls->nrp.p, ls->elem_pos.p, ls->num_nodes, ls->values.p,
-                           ls->rhs.p, accumulate);
-      };
-      const bool f32 = fp32_assembly(ls);
-      if (skew && shifted) f32 ? go(k_momentum_tile<true, true, float>) : go(k_momentum_tile<true, true, double>);
-      else if (skew) f32 ? go(k_momentum_tile<true, false, float>) : go(k_momentum_tile<true, false, double>);
-      else if (shifted) f32 ? go(k_momentum_tile<false, true, float>) : go(k_momentum_tile<false, true, double>);
-      else f32 ? go(k_momentum_tile<false, false, float>) : go(k_momentum_tile<false, false, double>);
-      ls->values_pending_zero = ls->rhs_pending_zero = false;
+      launch_momentum_tile(ls, m, u, mu, mdot, include_divu, skew != 0, shifted != 0, nullptr, false);
     } else {
       materialize_pending_zero(ls);
       const dim3 g(chunk_blocks<8>(E)), b(kChunk);
@@ -1237,6 +1278,34 @@ extern "C" int nalu_assemble_momentum_mass(nalu_linsys* ls, const nalu_mesh* m, 
     const MassArgs A{{rho_nm1, rho_n, rho_np1}, {u_nm1, u_n, u_np1}, gjp, dt, gamma[0], gamma[1], gamma[2],
                      balanced_force ? 1 : 0};
     launch_mass(ls, m, A, true, lumped != 0);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_assemble_momentum_with_mass(nalu_linsys* ls, const nalu_mesh* m, const double* u, const double* mu,
+                                                const double* mdot, double include_divu, int skew, int shifted,
+                                                const double* rho_nm1, const double* rho_n, const double* rho_np1,
+                                                const double* u_nm1, const double* u_n, const double* u_np1,
+                                                const double* gjp, double dt, const double* gamma, int lumped,
+                                                int balanced_force) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 3);
+    check_mass_args(dt, gamma);
+    NALU_REQUIRE(u && mu && mdot, NALU_ERR_ARG, "momentum_with_mass: null advection-diffusion field");
+    NALU_REQUIRE(rho_nm1 && rho_n && rho_np1 && u_nm1 && u_n && u_np1 && gjp, NALU_ERR_ARG,
+                 "momentum_with_mass: null mass field");
+    const size_t lds = ((size_t)kTileNodes * ls->max_node_row_len * 9 + 3 * kTileNodes) * sizeof(double);
+    if (fp32_assembly(ls) || lds > kMaxLds) {  // the two passes one after the other
+      int rc = nalu_assemble_momentum(ls, m, u, mu, mdot, include_divu, skew, shifted, fp32_assembly(ls) ? 2 : 1);
+      if (rc != NALU_OK) return rc;
+      return nalu_assemble_momentum_mass(ls, m, rho_nm1, rho_n, rho_np1, u_nm1, u_n, u_np1, gjp, dt, gamma, lumped,
+                                         balanced_force);
+    }
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    const MassArgs A{{rho_nm1, rho_n, rho_np1}, {u_nm1, u_n, u_np1}, gjp, dt, gamma[0], gamma[1], gamma[2],
+                     balanced_force ? 1 : 0};
+    launch_momentum_tile(ls, m, u, mu, mdot, include_divu, skew != 0, shifted != 0, &A, lumped != 0);
+    NALU_HIP_CHECK(hipGetLastError());
     return NALU_OK;
   });
 }

@@ -13,7 +13,10 @@
 // kernel is one fused gather + geometry + physics + scatter pass (the
 // element-visit tile assembly for the advection / diffusion kernels, the
 // owner-computes node pass for the mass kernels), all adding into the same
-// rows, which gives the same sums up to rounding order.
+// rows, which gives the same sums up to rounding order -- except that a
+// momentum advection-diffusion kernel and a momentum mass kernel of the same
+// list share one node-tile pass (nalu_assemble_momentum_with_mass).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -127,8 +130,28 @@ extern "C" int nalu_elem_algorithm_set_time_integrator(nalu_elem_algorithm* a, d
 extern "C" int nalu_elem_algorithm_execute(nalu_elem_algorithm* a) {
   return guarded([&]() -> int {
     NALU_REQUIRE(a, NALU_ERR_ARG, "elem_algorithm_execute: null handle");
-    for (const auto& k : a->kernels) {
-      const int rc = run(a, k);
+    // a momentum advection-diffusion kernel and a momentum mass kernel in the
+    // same list evaluate in one pass over the elements (one LDS row segment per
+    // node tile, one write of the matrix); every other kernel runs as its own pass
+    int adv = -1, mass = -1;
+    for (int i = 0; i < (int)a->kernels.size(); ++i) {
+      if (a->kernels[i].kind == NALU_KERNEL_MOMENTUM_ADVDIFF && adv < 0) adv = i;
+      if (a->kernels[i].kind == NALU_KERNEL_MOMENTUM_MASS && mass < 0) mass = i;
+    }
+    for (int i = 0; i < (int)a->kernels.size(); ++i) {
+      int rc;
+      if (adv >= 0 && mass >= 0 && (i == adv || i == mass)) {
+        if (i != std::min(adv, mass)) continue;
+        const nalu_kernel_desc& ka = a->kernels[adv];
+        const nalu_kernel_desc& km = a->kernels[mass];
+        const double* const* fa = ka.field;
+        const double* const* fm = km.field;
+        rc = nalu_assemble_momentum_with_mass(a->ls, a->mesh, fa[0], fa[1], fa[2], ka.include_divu,
+                                              ka.skew_symmetric, ka.shifted_grad_op, fm[0], fm[1], fm[2], fm[3],
+                                              fm[4], fm[5], fm[6], a->dt, a->gamma, km.lumped, km.balanced_force);
+      } else {
+        rc = run(a, a->kernels[i]);
+      }
       if (rc < 0) return rc;  // the failing kernel's message is the last error
     }
     return NALU_OK;

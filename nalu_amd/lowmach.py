@@ -24,8 +24,8 @@ node ids (row = node).
 import numpy as np
 
 from . import (LinearSystem, NaluError, SolverConfig, assemble_continuity, assemble_continuity_mass,
-               assemble_momentum, assemble_momentum_mass, assemble_momentum_mass_segregated,
-               assemble_momentum_segregated, compute_dual_volume, compute_mdot, compute_nodal_gradient)
+               assemble_momentum_mass_segregated, assemble_momentum_segregated, assemble_momentum_with_mass,
+               compute_dual_volume, compute_mdot, compute_nodal_gradient)
 
 
 class LowMachStep:
@@ -131,9 +131,9 @@ class LowMachStep:
         """The numDof-3 system (rows interleaved node-major), Dirichlet on all
         three components of the wall nodes, delta = du."""
         self.mom.zeroSystem()
-        assemble_momentum(self.mom, self.mesh, self.u, self.mu, self.mdot)
-        assemble_momentum_mass(self.mom, self.mesh, (self.rho, self.rho, self.rho), (uN, uN, self.u), self.Gjp,
-                               self.dt, gamma=self.g)
+        # advection-diffusion and mass in one element pass (the momentum kernel list)
+        assemble_momentum_with_mass(self.mom, self.mesh, self.u, self.mu, self.mdot, (self.rho, self.rho, self.rho),
+                                    (uN, uN, self.u), self.Gjp, self.dt, gamma=self.g)
         u_flat = self.u.reshape(-1).contiguous()
         bc = self.u.clone()
         bc[self.wall] = self.wall_u

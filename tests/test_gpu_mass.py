@@ -108,6 +108,47 @@ def test_mass_whole_mesh_parity(oracle, lumped, balanced):
     assert not np.any(v1)
 
 
+@pytest.mark.parametrize("lumped,balanced,skew,ids", [(False, False, False, False), (True, False, False, False),
+                                                      (False, True, True, False), (False, False, False, True)])
+def test_momentum_with_mass_one_pass(oracle, lumped, balanced, skew, ids):
+    """Advection-diffusion and mass in one node-tile pass against the oracle's
+    two kernels summed (first assembly after zeroSystem, and on top of an
+    earlier assembly)."""
+    n = 6
+    print("case: momentum + mass one pass n=%d lumped=%s balanced=%s skew=%s ids=%s" % (n, lumped, balanced, skew, ids),
+          flush=True)
+    P = lowmach_problem(n, seed=31, rho_sigma=0.3)
+    X, conn = P["coords"], P["conn"]
+    N = X.shape[0]
+    rng = np.random.default_rng(37)
+    rho = [P["rho"] * (1 + 0.1 * rng.random(N)) for _ in range(3)]
+    u = [P["u"] + 0.1 * rng.standard_normal((N, 3)) for _ in range(3)]
+    gjp = P["gpdx"] + 0.1 * rng.standard_normal((N, 3))
+    dt, gam = 0.017, (1.5, -2.0, 0.5)
+    node_ids = np.random.default_rng(5).permutation(N).astype(np.int64) + 1 if ids else None
+    mesh = na.Mesh(X, conn, node_ids=node_ids)
+    ls = na.LinearSystem(3, na.SolverConfig())
+    ls.buildElemToNodeGraph(mesh)
+    ls.finalizeLinearSystem()
+    mdot = torch.zeros((conn.shape[0], 12), dtype=torch.float64, device=DEV)
+    na.compute_mdot(mesh, dev(rho[2]), dev(P["p"]), dev(u[2]), dev(gjp), mdot, tau=P["tau"])
+    g3 = oracle.graph_build(N, conn, 3, node_ids=node_ids)
+    ov, orr = oracle.assemble_momentum(g3, conn, X, u[2], P["mu"], mdot.cpu().numpy(), skew=skew)
+    mv, mr = oracle.assemble_momentum_mass(g3, conn, X, rho, u, gjp, dt, gam, lumped=lumped, balanced=balanced)
+    for repeat in (1, 2):  # 1: stores into a pending zero; 2: adds onto the first pass
+        if repeat == 1:
+            ls.zeroSystem()
+        na.assemble_momentum_with_mass(ls, mesh, dev(u[2]), dev(P["mu"]), mdot, [dev(a) for a in rho],
+                                       [dev(a) for a in u], dev(gjp), dt, gam, lumped=lumped, balanced_force=balanced,
+                                       skew=skew)
+        rp, ci, v, r = ls.csr()
+        np.testing.assert_array_equal(ci, g3.col_idx)
+        e_mat = rel_err_rowwise(v, repeat * (ov + mv), g3.row_ptr)
+        e_rhs = np.abs(r - repeat * (orr + mr)).max() / np.abs(repeat * (orr + mr)).max()
+        print("  pass %d rel err: matrix %.2e rhs %.2e" % (repeat, e_mat, e_rhs), flush=True)
+        assert e_mat <= TOL and e_rhs <= TOL
+
+
 @pytest.mark.parametrize("lumped", [False, True])
 def test_momentum_mass_tile_alone_with_global_ids(oracle, lumped):
     """The numDof 3 mass tile as the first assembly after zeroSystem (its
