@@ -359,11 +359,11 @@ def momentum_3rhs(args, na, torch, mesh, P, devc, stream, md):
         ls.set_stream(stream)
         rhs3 = torch.zeros((3, N), dtype=torch.float64, device=devc)
 
-        def asm():
+        def asm():  # advection-diffusion and mass in one pass
             ls.zeroSystem()
             rhs3.zero_()
-            na.assemble_momentum_segregated(ls, mesh, u, mu, md[prec], rhs3)
-            na.assemble_momentum_mass_segregated(ls, mesh, (rho, rho, rho), (u, u, u), gp, dt, rhs3)
+            na.assemble_momentum_with_mass_segregated(ls, mesh, u, mu, md[prec], (rho, rho, rho), (u, u, u), gp, dt,
+                                                      rhs3)
             ls.loadComplete()
 
         asm()

@@ -322,7 +322,8 @@ int nalu_assemble_momentum_mass(nalu_linsys *ls, const nalu_mesh *m, const doubl
  * (src/AssembleElemSolverAlgorithm.C:75-100 with both kernels registered):
  * the arguments of nalu_assemble_momentum (node-tile scatter) and of
  * nalu_assemble_momentum_mass; one LDS row segment per tile, one write of the
- * matrix.  FP32-assembly systems run the two passes one after the other. */
+ * matrix (the mass part is FP64 arithmetic on FP32-assembly systems too, as in
+ * nalu_assemble_momentum_mass). */
 int nalu_assemble_momentum_with_mass(nalu_linsys *ls, const nalu_mesh *m, const double *u, const double *mu,
                                      const double *mdot, double include_divu, int skew, int shifted,
                                      const double *rho_nm1, const double *rho_n, const double *rho_np1,
@@ -345,6 +346,18 @@ int nalu_assemble_momentum_mass_segregated(nalu_linsys *ls, const nalu_mesh *m, 
                                            const double *u_n, const double *u_np1, const double *gjp,
                                            double dt, const double *gamma, int lumped, int balanced_force,
                                            double *rhs3_dev);
+/* The two segregated passes above in one pass over each row's elements (the
+ * advection-diffusion and mass kernels of one element algorithm): the
+ * arguments of nalu_assemble_momentum_segregated and of
+ * nalu_assemble_momentum_mass_segregated (the mass part is FP64 arithmetic on
+ * FP32-assembly systems too). */
+int nalu_assemble_momentum_with_mass_segregated(nalu_linsys *ls, const nalu_mesh *m, const double *u,
+                                                const double *mu, const double *mdot, double include_divu,
+                                                int skew, int shifted, const double *rho_nm1, const double *rho_n,
+                                                const double *rho_np1, const double *u_nm1, const double *u_n,
+                                                const double *u_np1, const double *gjp, double dt,
+                                                const double *gamma, int lumped, int balanced_force,
+                                                double *rhs3_dev);
 /* ContinuityMassElemKernel (src/kernel/ContinuityMassElemKernel.C:27-118), numDof 1:
  * rhs only, scaled by 1 / (dt / g1) like the continuity system. */
 int nalu_assemble_continuity_mass(nalu_linsys *ls, const nalu_mesh *m, const double *rho_nm1,

@@ -135,6 +135,9 @@ def lib():
         L.nalu_assemble_momentum_with_mass.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp,
                                                        _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _dp, C.c_int,
                                                        C.c_int]
+        L.nalu_assemble_momentum_with_mass_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int,
+                                                                  C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                                  C.c_double, _dp, C.c_int, C.c_int, _vp]
         L.nalu_assemble_momentum_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp]
         L.nalu_assemble_momentum_mass_segregated.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
                                                              _dp, C.c_int, C.c_int, _vp]
@@ -883,6 +886,24 @@ def assemble_momentum_with_mass(ls, mesh, u, mu, mdot, rho, u_states, gjp, dt, g
         float(include_divu), int(skew), int(shifted), r[0], r[1], r[2], v[0], v[1], v[2],
         _dev(gjp, "momentum_with_mass Gjp", "f64", numel=3 * N), float(dt), g.ctypes.data_as(_dp), int(lumped),
         int(balanced_force)), "assemble_momentum_with_mass")
+
+
+def assemble_momentum_with_mass_segregated(ls, mesh, u, mu, mdot, rho, u_states, gjp, dt, rhs3,
+                                           gamma=(1.0, -1.0, 0.0), lumped=False, balanced_force=False,
+                                           include_divu=0.0, skew=False, shifted=False):
+    """assemble_momentum_segregated + assemble_momentum_mass_segregated in one
+    pass over each row's elements; rhs3 [3][N] as for those two."""
+    N, E = mesh.num_nodes, mesh.num_elems
+    r, v, g = _states(rho, "momentum_with_mass_segregated rho", N), \
+        _states(u_states, "momentum_with_mass_segregated u", 3 * N), _gammas(gamma)
+    _check(lib().nalu_assemble_momentum_with_mass_segregated(
+        ls.handle, mesh.handle, _dev(u, "momentum_with_mass_segregated u", "f64", numel=3 * N),
+        _dev(mu, "momentum_with_mass_segregated mu", "f64", numel=N),
+        _dev(mdot, "momentum_with_mass_segregated mdot", "f64", numel=12 * E), float(include_divu), int(skew),
+        int(shifted), r[0], r[1], r[2], v[0], v[1], v[2],
+        _dev(gjp, "momentum_with_mass_segregated Gjp", "f64", numel=3 * N), float(dt), g.ctypes.data_as(_dp),
+        int(lumped), int(balanced_force), _dev(rhs3, "momentum_with_mass_segregated rhs3", "f64", shape=(3, ls._rows()))),
+        "assemble_momentum_with_mass_segregated")
 
 
 def assemble_momentum_mass(ls, mesh, rho, u, gjp, dt, gamma=(1.0, -1.0, 0.0), lumped=False, balanced_force=False):

@@ -784,13 +784,13 @@ __global__ __launch_bounds__(256) void k_nodal_grad(const int32_t* __restrict__ 
 // full residual, so converged velocities are unchanged).  Owner-computes row
 // gather on a numDof 1 graph: the row segment accumulates in LDS, the three
 // residual components go to rhs3[i][row].
-template <bool SKEW, bool SHIFT, typename R>
+template <bool SKEW, bool SHIFT, typename R, int MASS = 0>
 __global__ __launch_bounds__(256) void k_momentum_seg_gather(
     const int32_t* __restrict__ conn, const double* __restrict__ coords, const double* __restrict__ u,
     const double* __restrict__ mu, const double* __restrict__ mdot, double incl_divu,
     const int32_t* __restrict__ row_node, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_elem,
     const int32_t* __restrict__ nrp, const uint64_t* __restrict__ elem_pos, int64_t nrows,
-    double* __restrict__ vals, double* __restrict__ rhs3, int accumulate) {
+    double* __restrict__ vals, double* __restrict__ rhs3, int accumulate, MassArgs MA) {
   extern __shared__ double seg[];
   const int64_t r0 = (int64_t)blockIdx.x * blockDim.x;
   const int64_t rend = min(r0 + (int64_t)blockDim.x, nrows);
@@ -826,6 +826,16 @@ __global__ __launch_bounds__(256) void k_momentum_seg_gather(
       }
 #pragma unroll
       for (int i = 0; i < 3; ++i) rr[i] += (double)rl[i];
+      if (MASS) {  // the momentum mass kernel of the same element and scv (MASS 1 consistent, 2 lumped)
+        double xd[8][3];
+        load_vec3(coords, nd, xd);
+        switch (t & 7) {
+#define NALU_SM(Lx) \
+  case Lx: mass_row<Lx, true, MASS == 2, true>(MA, nd, xd, pk, myrow, 0, rr); break;
+          NALU_SM(0) NALU_SM(1) NALU_SM(2) NALU_SM(3) NALU_SM(4) NALU_SM(5) NALU_SM(6) NALU_SM(7)
+#undef NALU_SM
+        }
+      }
     }
   }
   __syncthreads();
@@ -1089,7 +1099,7 @@ extern "C" int nalu_compute_mdot_f32(const nalu_mesh* m, const double* rho, cons
 
 namespace {
 // The node-tile momentum pass; ma != nullptr adds the momentum mass kernel in
-// the same pass (FP64 assembly only).
+// the same pass.
 void launch_momentum_tile(nalu_linsys* ls, const nalu_mesh* m, const double* u, const double* mu, const double* mdot,
                           double include_divu, bool skew, bool shifted, const MassArgs* ma, bool lumped) {
   build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
@@ -1107,18 +1117,17 @@ void launch_momentum_tile(nalu_linsys* ls, const nalu_mesh* m, const double* u, 
                        ls->rhs.p, accumulate, A);
   };
   const bool f32 = fp32_assembly(ls);
-  NALU_REQUIRE(!(ma && f32), NALU_ERR_UNSUPPORTED, "momentum tile: the fused mass pass is FP64 only");
-  if (ma) {
+  if (ma) {  // the mass part is FP64 arithmetic at either precision of the momentum part
     if (lumped) {
-      if (skew && shifted) go(k_momentum_tile<true, true, double, 2>);
-      else if (skew) go(k_momentum_tile<true, false, double, 2>);
-      else if (shifted) go(k_momentum_tile<false, true, double, 2>);
-      else go(k_momentum_tile<false, false, double, 2>);
+      if (skew && shifted) f32 ? go(k_momentum_tile<true, true, float, 2>) : go(k_momentum_tile<true, true, double, 2>);
+      else if (skew) f32 ? go(k_momentum_tile<true, false, float, 2>) : go(k_momentum_tile<true, false, double, 2>);
+      else if (shifted) f32 ? go(k_momentum_tile<false, true, float, 2>) : go(k_momentum_tile<false, true, double, 2>);
+      else f32 ? go(k_momentum_tile<false, false, float, 2>) : go(k_momentum_tile<false, false, double, 2>);
     } else {
-      if (skew && shifted) go(k_momentum_tile<true, true, double, 1>);
-      else if (skew) go(k_momentum_tile<true, false, double, 1>);
-      else if (shifted) go(k_momentum_tile<false, true, double, 1>);
-      else go(k_momentum_tile<false, false, double, 1>);
+      if (skew && shifted) f32 ? go(k_momentum_tile<true, true, float, 1>) : go(k_momentum_tile<true, true, double, 1>);
+      else if (skew) f32 ? go(k_momentum_tile<true, false, float, 1>) : go(k_momentum_tile<true, false, double, 1>);
+      else if (shifted) f32 ? go(k_momentum_tile<false, true, float, 1>) : go(k_momentum_tile<false, true, double, 1>);
+      else f32 ? go(k_momentum_tile<false, false, float, 1>) : go(k_momentum_tile<false, false, double, 1>);
     }
   } else {
     if (skew && shifted) f32 ? go(k_momentum_tile<true, true, float>) : go(k_momentum_tile<true, true, double>);
@@ -1295,8 +1304,8 @@ extern "C" int nalu_assemble_momentum_with_mass(nalu_linsys* ls, const nalu_mesh
     NALU_REQUIRE(rho_nm1 && rho_n && rho_np1 && u_nm1 && u_n && u_np1 && gjp, NALU_ERR_ARG,
                  "momentum_with_mass: null mass field");
     const size_t lds = ((size_t)kTileNodes * ls->max_node_row_len * 9 + 3 * kTileNodes) * sizeof(double);
-    if (fp32_assembly(ls) || lds > kMaxLds) {  // the two passes one after the other
-      int rc = nalu_assemble_momentum(ls, m, u, mu, mdot, include_divu, skew, shifted, fp32_assembly(ls) ? 2 : 1);
+    if (lds > kMaxLds) {  // rows too long for a node tile: the two passes one after the other
+      int rc = nalu_assemble_momentum(ls, m, u, mu, mdot, include_divu, skew, shifted, 1);
       if (rc != NALU_OK) return rc;
       return nalu_assemble_momentum_mass(ls, m, rho_nm1, rho_n, rho_np1, u_nm1, u_n, u_np1, gjp, dt, gamma, lumped,
                                          balanced_force);
@@ -1325,6 +1334,55 @@ extern "C" int nalu_assemble_continuity_mass(nalu_linsys* ls, const nalu_mesh* m
   });
 }
 
+namespace {
+// The segregated momentum row gather; ma != nullptr adds the momentum mass
+// kernel in the same pass.
+void launch_momentum_seg(nalu_linsys* ls, const nalu_mesh* m, const double* u, const double* mu, const double* mdot,
+                         double include_divu, bool skew, bool shifted, double* rhs3, const MassArgs* ma, bool lumped) {
+  build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
+  if (ls->rhs_pending_zero) NALU_HIP_CHECK(hipMemsetAsync(ls->rhs.p, 0, ls->rhs.bytes(), ls->stream));
+  ls->rhs_pending_zero = false;
+  const int accumulate = ls->values_pending_zero ? 0 : 1;
+  constexpr int B = 256;
+  const size_t lds = (size_t)B * ls->max_node_row_len * sizeof(double);
+  const MassArgs none{};
+  const MassArgs& A = ma ? *ma : none;
+  auto go = [&](auto kern) {
+    set_lds_limit((const void*)kern, lds);
+    hipLaunchKernelGGL(kern, dim3(blocks_for(ls->nrows, B)), dim3(B), lds, ls->stream, m->conn.p, m->coords.p, u, mu,
+                       mdot, include_divu, (const int32_t*)ls->row_node.p, m->inc_ptr.p, m->inc_elem.p, ls->nrp.p,
+                       ls->elem_pos.p, ls->nrows, ls->values.p, rhs3, accumulate, A);
+  };
+  const bool f32 = fp32_assembly(ls);
+  if (ma) {  // the mass part is FP64 arithmetic at either precision of the momentum part
+    if (lumped) {
+      if (skew && shifted)
+        f32 ? go(k_momentum_seg_gather<true, true, float, 2>) : go(k_momentum_seg_gather<true, true, double, 2>);
+      else if (skew)
+        f32 ? go(k_momentum_seg_gather<true, false, float, 2>) : go(k_momentum_seg_gather<true, false, double, 2>);
+      else if (shifted)
+        f32 ? go(k_momentum_seg_gather<false, true, float, 2>) : go(k_momentum_seg_gather<false, true, double, 2>);
+      else f32 ? go(k_momentum_seg_gather<false, false, float, 2>) : go(k_momentum_seg_gather<false, false, double, 2>);
+    } else {
+      if (skew && shifted)
+        f32 ? go(k_momentum_seg_gather<true, true, float, 1>) : go(k_momentum_seg_gather<true, true, double, 1>);
+      else if (skew)
+        f32 ? go(k_momentum_seg_gather<true, false, float, 1>) : go(k_momentum_seg_gather<true, false, double, 1>);
+      else if (shifted)
+        f32 ? go(k_momentum_seg_gather<false, true, float, 1>) : go(k_momentum_seg_gather<false, true, double, 1>);
+      else f32 ? go(k_momentum_seg_gather<false, false, float, 1>) : go(k_momentum_seg_gather<false, false, double, 1>);
+    }
+  } else {
+    if (skew && shifted) f32 ? go(k_momentum_seg_gather<true, true, float>) : go(k_momentum_seg_gather<true, true, double>);
+    else if (skew) f32 ? go(k_momentum_seg_gather<true, false, float>) : go(k_momentum_seg_gather<true, false, double>);
+    else if (shifted) f32 ? go(k_momentum_seg_gather<false, true, float>) : go(k_momentum_seg_gather<false, true, double>);
+    else f32 ? go(k_momentum_seg_gather<false, false, float>) : go(k_momentum_seg_gather<false, false, double>);
+  }
+  ls->values_pending_zero = false;
+  NALU_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
 extern "C" int nalu_assemble_momentum_segregated(nalu_linsys* ls, const nalu_mesh* m, const double* u,
                                                  const double* mu, const double* mdot, double include_divu,
                                                  int skew, int shifted, double* rhs3) {
@@ -1332,25 +1390,7 @@ extern "C" int nalu_assemble_momentum_segregated(nalu_linsys* ls, const nalu_mes
     check_assembly_args(ls, m, 1);
     NALU_REQUIRE(u && mu && mdot && rhs3, NALU_ERR_ARG, "momentum_segregated: null field");
     PhaseTimer t(ls, PH_ASSEMBLE);
-    build_node_incidence(const_cast<nalu_mesh*>(m), ls->stream);
-    if (ls->rhs_pending_zero) NALU_HIP_CHECK(hipMemsetAsync(ls->rhs.p, 0, ls->rhs.bytes(), ls->stream));
-    ls->rhs_pending_zero = false;
-    const int accumulate = ls->values_pending_zero ? 0 : 1;
-    constexpr int B = 256;
-    const size_t lds = (size_t)B * ls->max_node_row_len * sizeof(double);
-    auto go = [&](auto kern) {
-      set_lds_limit((const void*)kern, lds);
-      hipLaunchKernelGGL(kern, dim3(blocks_for(ls->nrows, B)), dim3(B), lds, ls->stream, m->conn.p, m->coords.p, u,
-                         mu, mdot, include_divu, (const int32_t*)ls->row_node.p, m->inc_ptr.p, m->inc_elem.p,
-                         ls->nrp.p, ls->elem_pos.p, ls->nrows, ls->values.p, rhs3, accumulate);
-    };
-    const bool f32 = fp32_assembly(ls);
-    if (skew && shifted) f32 ? go(k_momentum_seg_gather<true, true, float>) : go(k_momentum_seg_gather<true, true, double>);
-    else if (skew) f32 ? go(k_momentum_seg_gather<true, false, float>) : go(k_momentum_seg_gather<true, false, double>);
-    else if (shifted) f32 ? go(k_momentum_seg_gather<false, true, float>) : go(k_momentum_seg_gather<false, true, double>);
-    else f32 ? go(k_momentum_seg_gather<false, false, float>) : go(k_momentum_seg_gather<false, false, double>);
-    ls->values_pending_zero = false;
-    NALU_HIP_CHECK(hipGetLastError());
+    launch_momentum_seg(ls, m, u, mu, mdot, include_divu, skew != 0, shifted != 0, rhs3, nullptr, false);
     return NALU_OK;
   });
 }
@@ -1369,6 +1409,25 @@ extern "C" int nalu_assemble_momentum_mass_segregated(nalu_linsys* ls, const nal
     const MassArgs A{{rho_nm1, rho_n, rho_np1}, {u_nm1, u_n, u_np1}, gjp, dt, gamma[0], gamma[1], gamma[2],
                      balanced_force ? 1 : 0};
     launch_mass(ls, m, A, true, lumped != 0, rhs3);
+    return NALU_OK;
+  });
+}
+
+extern "C" int nalu_assemble_momentum_with_mass_segregated(
+    nalu_linsys* ls, const nalu_mesh* m, const double* u, const double* mu, const double* mdot, double include_divu,
+    int skew, int shifted, const double* rho_nm1, const double* rho_n, const double* rho_np1, const double* u_nm1,
+    const double* u_n, const double* u_np1, const double* gjp, double dt, const double* gamma, int lumped,
+    int balanced_force, double* rhs3) {
+  return guarded([&]() -> int {
+    check_assembly_args(ls, m, 1);
+    check_mass_args(dt, gamma);
+    NALU_REQUIRE(u && mu && mdot && rhs3, NALU_ERR_ARG, "momentum_with_mass_segregated: null field");
+    NALU_REQUIRE(rho_nm1 && rho_n && rho_np1 && u_nm1 && u_n && u_np1 && gjp, NALU_ERR_ARG,
+                 "momentum_with_mass_segregated: null mass field");
+    PhaseTimer t(ls, PH_ASSEMBLE);
+    const MassArgs A{{rho_nm1, rho_n, rho_np1}, {u_nm1, u_n, u_np1}, gjp, dt, gamma[0], gamma[1], gamma[2],
+                     balanced_force ? 1 : 0};
+    launch_momentum_seg(ls, m, u, mu, mdot, include_divu, skew != 0, shifted != 0, rhs3, &A, lumped != 0);
     return NALU_OK;
   });
 }

@@ -24,8 +24,8 @@ node ids (row = node).
 import numpy as np
 
 from . import (LinearSystem, NaluError, SolverConfig, assemble_continuity, assemble_continuity_mass,
-               assemble_momentum_mass_segregated, assemble_momentum_segregated, assemble_momentum_with_mass,
-               compute_dual_volume, compute_mdot, compute_nodal_gradient)
+               assemble_momentum_with_mass, assemble_momentum_with_mass_segregated, compute_dual_volume, compute_mdot,
+               compute_nodal_gradient)
 
 
 class LowMachStep:
@@ -107,9 +107,10 @@ class LowMachStep:
         """Three right-hand sides sharing one scalar matrix."""
         self.mom.zeroSystem()
         self.rhs3.zero_()
-        assemble_momentum_segregated(self.mom, self.mesh, self.u, self.mu, self.mdot, self.rhs3)
-        assemble_momentum_mass_segregated(self.mom, self.mesh, (self.rho, self.rho, self.rho), (uN, uN, self.u),
-                                          self.Gjp, self.dt, self.rhs3, gamma=self.g)
+        # advection-diffusion and mass in one pass over each row's elements
+        assemble_momentum_with_mass_segregated(self.mom, self.mesh, self.u, self.mu, self.mdot,
+                                               (self.rho, self.rho, self.rho), (uN, uN, self.u), self.Gjp, self.dt,
+                                               self.rhs3, gamma=self.g)
         # Dirichlet rows (the field arrays are nodal: component 0 of the velocity
         # and its boundary value; rhs3 then takes all three components)
         u0 = self.u[:, 0].contiguous()

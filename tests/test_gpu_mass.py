@@ -232,6 +232,49 @@ def test_momentum_segregated_parity(oracle, skew, shifted):
     assert e_mat <= TOL and e_rhs <= TOL and e_self <= TOL
 
 
+@pytest.mark.parametrize("lumped,balanced,precision", [(False, False, "fp64"), (True, True, "fp64"),
+                                                       (False, False, "mixed")])
+def test_segregated_momentum_with_mass_one_pass(lumped, balanced, precision):
+    """The segregated advection-diffusion and mass kernels in one pass give the
+    sums of the two oracle-checked passes (to rounding order; with FP32
+    assembly too, whose contributions are summed in FP64 either way)."""
+    n = 6
+    print("case: segregated momentum + mass one pass n=%d lumped=%s balanced=%s %s" % (n, lumped, balanced, precision),
+          flush=True)
+    P = lowmach_problem(n, seed=41, mu=0.05, rho_sigma=0.3)
+    X, conn = P["coords"], P["conn"]
+    N = X.shape[0]
+    rng = np.random.default_rng(43)
+    rho = [dev(P["rho"] * (1 + 0.1 * rng.random(N))) for _ in range(3)]
+    us = [dev(P["u"] + 0.1 * rng.standard_normal((N, 3))) for _ in range(3)]
+    gjp = dev(P["gpdx"] + 0.1 * rng.standard_normal((N, 3)))
+    dt, gam = 0.02, (1.5, -2.0, 0.5)
+    mesh = na.Mesh(X, conn)
+    mdot = torch.zeros((conn.shape[0], 12), dtype=torch.float64, device=DEV)
+    na.compute_mdot(mesh, rho[2], dev(P["p"]), us[2], gjp, mdot, tau=P["tau"])
+    out = []
+    for fused in (True, False):
+        ls = na.LinearSystem(1, na.SolverConfig(precision=precision))
+        ls.buildElemToNodeGraph(mesh)
+        ls.finalizeLinearSystem()
+        ls.zeroSystem()
+        rhs3 = torch.zeros((3, N), dtype=torch.float64, device=DEV)
+        if fused:
+            na.assemble_momentum_with_mass_segregated(ls, mesh, us[2], dev(P["mu"]), mdot, rho, us, gjp, dt, rhs3,
+                                                      gamma=gam, lumped=lumped, balanced_force=balanced)
+        else:
+            na.assemble_momentum_segregated(ls, mesh, us[2], dev(P["mu"]), mdot, rhs3)
+            na.assemble_momentum_mass_segregated(ls, mesh, rho, us, gjp, dt, rhs3, gamma=gam, lumped=lumped,
+                                                 balanced_force=balanced)
+        rp, ci, v, _ = ls.csr()
+        out.append((rp, v, rhs3.cpu().numpy()))
+    (rp, vf, bf), (_, v2, b2) = out
+    e_mat = rel_err_rowwise(vf, v2, rp)
+    e_rhs = np.abs(bf - b2).max() / np.abs(b2).max()
+    print("  one pass vs two: matrix %.2e rhs %.2e" % (e_mat, e_rhs), flush=True)
+    assert e_mat <= 1e-13 and e_rhs <= 1e-13
+
+
 def test_segregated_transient_momentum_solve(oracle):
     """Backward-Euler momentum step in the segregated form: mass + advection-
     diffusion, three right-hand sides solved with one matrix and one SGS set-up
