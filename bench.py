@@ -551,8 +551,9 @@ def stretched_compare(args, na, torch, devc):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the domain-decomposition design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (the path would shard by domain "
+              "decomposition: element slabs per GPU, owned / shared rows, one halo exchange per SpMV).",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
