@@ -305,6 +305,19 @@ def mixed_block(args, na, torch, mesh, P, devc, stream):
         na.assemble_momentum(mom, mesh, u, mu, md[prec], mode="tile")
         mom.loadComplete()
 
+    def assemble_fused(prec):  # as the step does: mdot written by the continuity row gather
+        cont, mom = sys_[prec]
+        cont.zeroSystem()
+        na.assemble_continuity(cont, mesh, rho, p, u, gp, tau=P["tau"], mode="gather", mdot_out=md[prec])
+        cont.loadComplete()
+        mom.zeroSystem()
+        na.assemble_momentum(mom, mesh, u, mu, md[prec], mode="tile")
+        mom.loadComplete()
+
+    for prec in ("fp64", "mixed"):
+        assemble_fused(prec)
+        res["assembly_s_fused_mdot_" + prec] = timed(torch, lambda: assemble_fused(prec), reps=5)
+        res["assembly_melem_s_fused_mdot_" + prec] = E / res["assembly_s_fused_mdot_" + prec] / 1e6
     for prec in ("fp64", "mixed"):
         assemble(prec)
         res["assembly_s_" + prec] = timed(torch, lambda: assemble(prec), reps=5)
