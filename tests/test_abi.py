@@ -102,6 +102,18 @@ def test_mesh_validation_without_gpu():
         na.Mesh(coords[:, :2], conn)
 
 
+def test_new_entry_points_reject_null_handles_without_gpu():
+    # the argument checks run before any device call
+    L = na.lib()
+    g = np.array([1.0, -1.0, 0.0])
+    gp = g.ctypes.data_as(na._dp)
+    assert L.nalu_assemble_momentum_with_mass(None, None, None, None, None, 0.0, 0, 0, None, None, None, None, None,
+                                              None, None, 1.0, gp, 0, 0) < 0
+    assert L.nalu_assemble_momentum_with_mass_segregated(None, None, None, None, None, 0.0, 0, 0, None, None, None,
+                                                         None, None, None, None, 1.0, gp, 0, 0, None) < 0
+    assert L.nalu_gnn_data_set_pair(None, None, 0, None, None) < 0
+
+
 def test_product_does_not_import_oracle():
     # the shipped package never reaches the checker
     for dirpath, _, files in os.walk(os.path.join(ROOT, "nalu_amd")):
