@@ -121,6 +121,7 @@ def train_pressure_preconditioner(problem, num_rhs=400, pairs_per_rhs=5, harvest
     t_train = time.perf_counter() - t0
     say("validation iterations per epoch %s, kept epoch %d" % (net.validation, net.best_epoch))
     say("trained %d epochs in %.2f s, losses %s" % (epochs, t_train, np.round(hist, 4).tolist()))
+    first_validation, first_kept = list(net.validation), net.best_epoch  # the rounds below overwrite both
     rounds = []
     for rnd in range(int(self_pair_rounds)):
         t0 = time.perf_counter()
@@ -134,7 +135,7 @@ def train_pressure_preconditioner(problem, num_rhs=400, pairs_per_rhs=5, harvest
         say("self-pair round %d: %d pairs replaced, validation %s, kept epoch %d" % (rnd + 1, nrep, net.validation,
                                                                                    net.best_epoch))
     rec = {"train_mesh": "%d^3" % n_t, "pairs": len(pairs), "rhs": num_rhs, "pairs_per_rhs": pairs_per_rhs,
-           "cg_tol": harvest_tol, "validation_fgmres_iters_per_epoch": net.validation, "kept_epoch": net.best_epoch,
+           "cg_tol": harvest_tol, "validation_fgmres_iters_per_epoch": first_validation, "kept_epoch": first_kept,
            "harvest_s": t_harvest, "harvest_cg_iters_mean": float(np.mean(pairs.cg_iterations())),
            "train_s": t_train, "epochs": epochs, "batch": batch, "lr": lr, "epoch_loss": hist.tolist(),
            "layers": layers, "hidden": hidden, "weights": "bf16", "accumulate": "fp32", "base": base,

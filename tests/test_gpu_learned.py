@@ -291,6 +291,9 @@ def test_self_pair_rounds():
                                              self_pair_caps=(2, 5), self_pair_epochs=2)
     rounds = rec["self_pair_rounds"]
     assert len(rounds) == 1 and rounds[0]["pairs_replaced"] == 32
+    # the first training's record is kept apart from the round's
+    assert len(rec["validation_fgmres_iters_per_epoch"]) == 2 and 0 <= rec["kept_epoch"] < 2
+    assert rec["validation_fgmres_iters_per_epoch"] is not rounds[0]["validation_fgmres_iters_per_epoch"]
     assert len(rounds[0]["validation_fgmres_iters_per_epoch"]) == 2
     assert all(np.isfinite(rounds[0]["epoch_loss"]))
     assert 0 <= rounds[0]["kept_epoch"] < 2
