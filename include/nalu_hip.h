@@ -323,7 +323,8 @@ int nalu_assemble_momentum_mass(nalu_linsys *ls, const nalu_mesh *m, const doubl
  * the arguments of nalu_assemble_momentum (node-tile scatter) and of
  * nalu_assemble_momentum_mass; one LDS row segment per tile, one write of the
  * matrix (the mass part is FP64 arithmetic on FP32-assembly systems too, as in
- * nalu_assemble_momentum_mass). */
+ * nalu_assemble_momentum_mass).  With skew or shifted set, or rows too long for
+ * a node tile, the two kernels run as two passes. */
 int nalu_assemble_momentum_with_mass(nalu_linsys *ls, const nalu_mesh *m, const double *u, const double *mu,
                                      const double *mdot, double include_divu, int skew, int shifted,
                                      const double *rho_nm1, const double *rho_n, const double *rho_np1,
@@ -350,7 +351,8 @@ int nalu_assemble_momentum_mass_segregated(nalu_linsys *ls, const nalu_mesh *m, 
  * advection-diffusion and mass kernels of one element algorithm): the
  * arguments of nalu_assemble_momentum_segregated and of
  * nalu_assemble_momentum_mass_segregated (the mass part is FP64 arithmetic on
- * FP32-assembly systems too). */
+ * FP32-assembly systems too).  With skew or shifted set, the two kernels run as
+ * two passes. */
 int nalu_assemble_momentum_with_mass_segregated(nalu_linsys *ls, const nalu_mesh *m, const double *u,
                                                 const double *mu, const double *mdot, double include_divu,
                                                 int skew, int shifted, const double *rho_nm1, const double *rho_n,

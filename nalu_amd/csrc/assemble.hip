@@ -1118,17 +1118,10 @@ void launch_momentum_tile(nalu_linsys* ls, const nalu_mesh* m, const double* u, 
   };
   const bool f32 = fp32_assembly(ls);
   if (ma) {  // the mass part is FP64 arithmetic at either precision of the momentum part
-    if (lumped) {
-      if (skew && shifted) f32 ? go(k_momentum_tile<true, true, float, 2>) : go(k_momentum_tile<true, true, double, 2>);
-      else if (skew) f32 ? go(k_momentum_tile<true, false, float, 2>) : go(k_momentum_tile<true, false, double, 2>);
-      else if (shifted) f32 ? go(k_momentum_tile<false, true, float, 2>) : go(k_momentum_tile<false, true, double, 2>);
-      else f32 ? go(k_momentum_tile<false, false, float, 2>) : go(k_momentum_tile<false, false, double, 2>);
-    } else {
-      if (skew && shifted) f32 ? go(k_momentum_tile<true, true, float, 1>) : go(k_momentum_tile<true, true, double, 1>);
-      else if (skew) f32 ? go(k_momentum_tile<true, false, float, 1>) : go(k_momentum_tile<true, false, double, 1>);
-      else if (shifted) f32 ? go(k_momentum_tile<false, true, float, 1>) : go(k_momentum_tile<false, true, double, 1>);
-      else f32 ? go(k_momentum_tile<false, false, float, 1>) : go(k_momentum_tile<false, false, double, 1>);
-    }
+    // one-pass forms for the default advection options only (compile time)
+    NALU_REQUIRE(!skew && !shifted, NALU_ERR_UNSUPPORTED, "momentum tile: one pass with mass needs skew = shifted = 0");
+    if (lumped) f32 ? go(k_momentum_tile<false, false, float, 2>) : go(k_momentum_tile<false, false, double, 2>);
+    else f32 ? go(k_momentum_tile<false, false, float, 1>) : go(k_momentum_tile<false, false, double, 1>);
   } else {
     if (skew && shifted) f32 ? go(k_momentum_tile<true, true, float>) : go(k_momentum_tile<true, true, double>);
     else if (skew) f32 ? go(k_momentum_tile<true, false, float>) : go(k_momentum_tile<true, false, double>);
@@ -1304,8 +1297,10 @@ extern "C" int nalu_assemble_momentum_with_mass(nalu_linsys* ls, const nalu_mesh
     NALU_REQUIRE(rho_nm1 && rho_n && rho_np1 && u_nm1 && u_n && u_np1 && gjp, NALU_ERR_ARG,
                  "momentum_with_mass: null mass field");
     const size_t lds = ((size_t)kTileNodes * ls->max_node_row_len * 9 + 3 * kTileNodes) * sizeof(double);
-    if (lds > kMaxLds) {  // rows too long for a node tile: the two passes one after the other
-      int rc = nalu_assemble_momentum(ls, m, u, mu, mdot, include_divu, skew, shifted, 1);
+    // the two passes one after the other: rows too long for a node tile, or the
+    // skew-symmetric / shifted advection options (one-pass forms exist for the defaults)
+    if (lds > kMaxLds || skew || shifted) {
+      int rc = nalu_assemble_momentum(ls, m, u, mu, mdot, include_divu, skew, shifted, lds > kMaxLds ? 1 : 2);
       if (rc != NALU_OK) return rc;
       return nalu_assemble_momentum_mass(ls, m, rho_nm1, rho_n, rho_np1, u_nm1, u_n, u_np1, gjp, dt, gamma, lumped,
                                          balanced_force);
@@ -1355,23 +1350,11 @@ void launch_momentum_seg(nalu_linsys* ls, const nalu_mesh* m, const double* u, c
   };
   const bool f32 = fp32_assembly(ls);
   if (ma) {  // the mass part is FP64 arithmetic at either precision of the momentum part
-    if (lumped) {
-      if (skew && shifted)
-        f32 ? go(k_momentum_seg_gather<true, true, float, 2>) : go(k_momentum_seg_gather<true, true, double, 2>);
-      else if (skew)
-        f32 ? go(k_momentum_seg_gather<true, false, float, 2>) : go(k_momentum_seg_gather<true, false, double, 2>);
-      else if (shifted)
-        f32 ? go(k_momentum_seg_gather<false, true, float, 2>) : go(k_momentum_seg_gather<false, true, double, 2>);
-      else f32 ? go(k_momentum_seg_gather<false, false, float, 2>) : go(k_momentum_seg_gather<false, false, double, 2>);
-    } else {
-      if (skew && shifted)
-        f32 ? go(k_momentum_seg_gather<true, true, float, 1>) : go(k_momentum_seg_gather<true, true, double, 1>);
-      else if (skew)
-        f32 ? go(k_momentum_seg_gather<true, false, float, 1>) : go(k_momentum_seg_gather<true, false, double, 1>);
-      else if (shifted)
-        f32 ? go(k_momentum_seg_gather<false, true, float, 1>) : go(k_momentum_seg_gather<false, true, double, 1>);
-      else f32 ? go(k_momentum_seg_gather<false, false, float, 1>) : go(k_momentum_seg_gather<false, false, double, 1>);
-    }
+    // one-pass forms for the default advection options only (compile time)
+    NALU_REQUIRE(!skew && !shifted, NALU_ERR_UNSUPPORTED,
+                 "momentum gather: one pass with mass needs skew = shifted = 0");
+    if (lumped) f32 ? go(k_momentum_seg_gather<false, false, float, 2>) : go(k_momentum_seg_gather<false, false, double, 2>);
+    else f32 ? go(k_momentum_seg_gather<false, false, float, 1>) : go(k_momentum_seg_gather<false, false, double, 1>);
   } else {
     if (skew && shifted) f32 ? go(k_momentum_seg_gather<true, true, float>) : go(k_momentum_seg_gather<true, true, double>);
     else if (skew) f32 ? go(k_momentum_seg_gather<true, false, float>) : go(k_momentum_seg_gather<true, false, double>);
@@ -1424,10 +1407,16 @@ extern "C" int nalu_assemble_momentum_with_mass_segregated(
     NALU_REQUIRE(u && mu && mdot && rhs3, NALU_ERR_ARG, "momentum_with_mass_segregated: null field");
     NALU_REQUIRE(rho_nm1 && rho_n && rho_np1 && u_nm1 && u_n && u_np1 && gjp, NALU_ERR_ARG,
                  "momentum_with_mass_segregated: null mass field");
+    if (skew || shifted) {  // one-pass forms exist for the default advection options
+      const int rc = nalu_assemble_momentum_segregated(ls, m, u, mu, mdot, include_divu, skew, shifted, rhs3);
+      if (rc != NALU_OK) return rc;
+      return nalu_assemble_momentum_mass_segregated(ls, m, rho_nm1, rho_n, rho_np1, u_nm1, u_n, u_np1, gjp, dt, gamma,
+                                                    lumped, balanced_force, rhs3);
+    }
     PhaseTimer t(ls, PH_ASSEMBLE);
     const MassArgs A{{rho_nm1, rho_n, rho_np1}, {u_nm1, u_n, u_np1}, gjp, dt, gamma[0], gamma[1], gamma[2],
                      balanced_force ? 1 : 0};
-    launch_momentum_seg(ls, m, u, mu, mdot, include_divu, skew != 0, shifted != 0, rhs3, &A, lumped != 0);
+    launch_momentum_seg(ls, m, u, mu, mdot, include_divu, false, false, rhs3, &A, lumped != 0);
     return NALU_OK;
   });
 }
